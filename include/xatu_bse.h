@@ -1,0 +1,178 @@
+/*
+ * xatu_bse.h -- C-ABI of the MI355X (gfx950) Bethe-Salpeter kernel builder and
+ * exciton solver. This is the drop-in boundary for the hot path of Xatu's
+ * ExcitonTB: the body of the Fourier part of ExcitonTB::initializeResultsH0
+ * (src/ExcitonTB.cpp:707-787), ExcitonTB::BShamiltonian (src/ExcitonTB.cpp:840-921)
+ * and ExcitonTB::diagonalizeRaw (src/ExcitonTB.cpp:930-959, with
+ * src/davidson.cpp:5-61). The single-particle band problem (src/ExcitonTB.cpp:737-758)
+ * stays with the caller, which passes the eigenvalue/eigenvector stacks in.
+ *
+ * Conventions
+ *  - Plain C types only. Complex numbers are interleaved (re, im) pairs of
+ *    doubles (or floats where stated), laid out exactly as Armadillo stores
+ *    cx_mat / cx_cube (column-major), so a reference caller can pass
+ *    eigvecKStack.memptr() and receive HBS into HBS.memptr().
+ *  - Every call is synchronous on the context's own HIP stream; host buffers
+ *    are owned by the caller; device buffers are owned by the context.
+ *  - Return value 0 (XATU_OK) on success, otherwise an XATU_ERR_* code, with
+ *    a message from xatu_last_error(). The reference throws
+ *    std::invalid_argument / std::logic_error at the same points.
+ */
+#ifndef XATU_BSE_H
+#define XATU_BSE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct xatu_ctx xatu_ctx;
+
+/* Status codes */
+#define XATU_OK 0
+#define XATU_ERR_INVALID_ARGUMENT 1 /* reference: std::invalid_argument */
+#define XATU_ERR_LOGIC 2            /* reference: std::logic_error, call order */
+#define XATU_ERR_DEVICE 3           /* HIP / rocBLAS / rocSOLVER failure */
+#define XATU_ERR_OUT_OF_MEMORY 4
+#define XATU_ERR_NOT_CONVERGED 5
+#define XATU_ERR_NOT_HERMITIAN 6    /* reference: davidson_method invalid_argument */
+
+/* Arithmetic of a build or of a solve */
+#define XATU_FP64 0
+#define XATU_FP32 1
+
+/* Interaction potentials (src/ExcitonTB.cpp:347-399) */
+#define XATU_POTENTIAL_KELDYSH 0
+#define XATU_POTENTIAL_COULOMB 1
+
+/* Solve methods (src/ExcitonTB.cpp:940-954) */
+#define XATU_SOLVE_DIAG 0     /* dense: all eigenpairs, ascending ("diag") */
+#define XATU_SOLVE_DAVIDSON 1 /* block Davidson, lowest nstates ("davidson"; also serves "sparse") */
+
+/* Hellmann-Feynman parameters */
+#define XATU_PARAM_R0 0
+#define XATU_PARAM_EPS_M 1
+#define XATU_PARAM_EPS_S 2
+
+/* Screened interaction set-up: ExcitonTB attributes used by initializeResultsH0. */
+typedef struct xatu_interaction {
+    double eps_m;          /* dielectric constant of the embedding medium */
+    double eps_s;          /* dielectric constant of the substrate */
+    double r0;             /* screening length (Angstrom), must be non-zero */
+    double regularization; /* V(0) is V(regularization); <= 1e-10 selects |a1| (ExcitonTB.cpp:803-805) */
+    double cutoff;         /* cutoff_ in units of |a1|: cells with |R| < |a1|*cutoff + 1e-5 are summed,
+                              and V(r) = 0 for r > |a1|*cutoff + 1e-5 (ExcitonTB.cpp:350,710) */
+    int ncell;             /* cells per axis of the centred supercell the cells are drawn from */
+    int total_cells;       /* normalisation of the lattice sums, (ncell*factor)^ndim (ExcitonTB.cpp:808) */
+    int potential;          /* XATU_POTENTIAL_* for the direct term */
+    int exchange_potential; /* XATU_POTENTIAL_* for the exchange term */
+    int exchange;           /* non-zero: also compute V(Q) for the exchange term */
+    double Q[3];            /* centre-of-mass momentum (1/Angstrom, Cartesian) */
+} xatu_interaction;
+
+typedef struct xatu_solve_options {
+    int method;      /* XATU_SOLVE_DIAG or XATU_SOLVE_DAVIDSON */
+    int precision;   /* XATU_FP64: products with the FP64 matrix.
+                        XATU_FP32: products with the FP32 matrix, then FP64 refinement */
+    int matrix_free; /* non-zero: no resident matrix; tiles recomputed inside each product */
+    int nstates;     /* number of lowest eigenpairs wanted */
+    int block;       /* Davidson block size; 0 selects max(2*nstates, 8) */
+    int max_basis;   /* Davidson basis size before restart; 0 selects a default */
+    int max_iter;    /* 0 selects a default */
+    double tol;      /* residual 2-norm tolerance in eV; 0 selects 1e-6 (FP64) / 1e-4 (FP32 inner) */
+} xatu_solve_options;
+
+typedef struct xatu_solve_report {
+    int iterations;          /* Davidson iterations (both phases) */
+    int products;            /* matrix-vector products (vectors) */
+    int restarts;
+    int refine_iterations;   /* FP64 refinement iterations after the FP32 phase */
+    int converged;           /* 1 if every wanted residual met tol */
+    double max_residual;     /* FP64 residual ||H x - e x|| max over the wanted states */
+    double ms_total;         /* wall time of the solve (ms) */
+    double ms_products;      /* time in the matrix-vector products (ms, HIP events) */
+    double ms_rayleigh_ritz; /* time in the host Rayleigh-Ritz eigenproblems (ms) */
+} xatu_solve_report;
+
+typedef struct xatu_timings {
+    double ms_fourier;      /* last xatu_fourier_sums (HIP events) */
+    double ms_build;        /* last xatu_build_bse: all its kernels (HIP events) */
+    double ms_build_kernel; /* last xatu_build_bse: the assembly kernel alone (HIP events) */
+    double ms_matvec_kernel;/* average duration of the dense product kernel over the last solve */
+    int n_matvec_kernel;    /* number of dense product kernel launches in the last solve */
+} xatu_timings;
+
+/* ---- context ---- */
+int xatu_create(xatu_ctx** ctx, int device);
+void xatu_destroy(xatu_ctx* ctx);
+const char* xatu_last_error(const xatu_ctx* ctx);
+
+/* Lattice and k-mesh geometry (Lattice.cpp:32-47,140-162).
+ * bravais: ndim x 3 row-major (row = lattice vector), Angstrom.
+ * motif: natoms x 3 row-major atomic positions; norb_per_atom: orbitals of each atom
+ * (the basis orders orbitals atom by atom, SystemTB/ExcitonTB.cpp:535-543).
+ * mesh_n: points per axis of the full Monkhorst-Pack mesh the lattice sums are taken on
+ * (ncell*factor, Lattice.cpp:172-197). */
+int xatu_set_lattice(xatu_ctx* ctx, int ndim, const double* bravais, int natoms, const double* motif,
+                     const int* norb_per_atom, int mesh_n);
+
+/* Band stacks (ExcitonTB.cpp:717-758): nb = nv + nc bands per k, valence first.
+ * eigval_k, eigval_kq: nb x nk column-major (eigvalKStack, eigvalKQStack).
+ * eigvec_k, eigvec_kq: complex norb x nb x nk (eigvecKStack, eigvecKQStack), phase-fixed.
+ * k_offsets: nk x ndim row-major integers, the offset of each k-point from k-point 0 in
+ * units of the mesh spacing 1/mesh_n of each reciprocal vector; k' - k maps to the mesh
+ * point (k'-k) mod mesh as Lattice::findEquivalentPointBZ does (Lattice.cpp:251-279). */
+int xatu_set_bands(xatu_ctx* ctx, int nk, int nv, int nc, const double* eigval_k, const double* eigval_kq,
+                   const double* eigvec_k, const double* eigvec_kq, const int* k_offsets);
+
+/* Lattice Fourier sums V_ab(q) on every mesh point and, if exchange, V_ab(Q)
+ * (ExcitonTB.cpp:438-476, 695-697, 761-786). The potential table V_ab(R) is evaluated
+ * once in FP64 (Keldysh: Struve H0 minus Neumann Y0, ExcitonTB.cpp:306-366). */
+int xatu_fourier_sums(xatu_ctx* ctx, const xatu_interaction* inter);
+
+/* Assemble the BSE matrix in the given arithmetic and keep it resident in HBM
+ * (ExcitonTB.cpp:840-921). scissor in eV. If hbs_out is non-null it receives the full
+ * N x N matrix, column-major complex (double pairs for FP64, float pairs for FP32). */
+int xatu_build_bse(xatu_ctx* ctx, int precision, double scissor, void* hbs_out);
+
+/* Same, for an explicit subset basis (ExcitonTB::BShamiltonian(const arma::imat&)):
+ * basis is nbasis x 3 row-major {valence column, conduction column, k index}, columns
+ * indexing the band stacks (bandToIndex). FP64 only; the result is copied to hbs_out
+ * (nbasis x nbasis) and not kept resident. */
+int xatu_build_bse_basis(xatu_ctx* ctx, int nbasis, const int* basis, double scissor, double* hbs_out);
+
+/* Copy a resident matrix to the host (same layouts as xatu_build_bse). */
+int xatu_get_bse(xatu_ctx* ctx, int precision, void* hbs_out);
+/* Release a resident matrix. */
+int xatu_free_bse(xatu_ctx* ctx, int precision);
+
+/* Diagonalise (ExcitonTB.cpp:930-959). eigval_out: nstates (all N for DIAG) ascending.
+ * eigvec_out (nullable): N x nstates complex128 column-major. */
+int xatu_solve(xatu_ctx* ctx, const xatu_solve_options* opt, double* eigval_out, double* eigvec_out,
+               xatu_solve_report* report);
+
+/* y = H x for nvec vectors (complex128, N x nvec column-major) with the resident matrix of
+ * the given precision, or matrix-free when matrix_free != 0. */
+int xatu_bse_matvec(xatu_ctx* ctx, int precision, int matrix_free, int nvec, const double* x, double* y);
+
+/* Hellmann-Feynman gradients dE_n/dtheta = psi_n^H (dH/dtheta) psi_n for the screening
+ * parameters (XATU_PARAM_*), without forming dH. eigvec: N x nstates complex128 (null: the
+ * eigenvectors of the last solve). grad_out: nparams x nstates, state index fastest. */
+int xatu_hf_gradients(xatu_ctx* ctx, int nstates, const double* eigvec, int nparams, const int* params,
+                      double* grad_out);
+
+/* Read-backs for parity checks.
+ * ft_out: natoms x natoms x nmesh complex128 (ftMotifStack); ftq_out: natoms x natoms (ftMotifQ).
+ * cells: number of lattice vectors nR and (if coefs non-null) nR x ndim integer coefficients. */
+int xatu_get_fourier(xatu_ctx* ctx, double* ft_out, double* ftq_out);
+int xatu_get_cells(xatu_ctx* ctx, int* n_cells, int* coefs);
+/* Per-block error of the resident FP32 matrix against the resident FP64 matrix: the
+ * matrix is cut into blocks of kblock k-points per side; block_err (nblk x nblk,
+ * column-major) receives max |H32 - H64| per block, max_abs the overall maximum. */
+int xatu_compare_builds(xatu_ctx* ctx, int kblock, double* block_err, double* max_abs);
+int xatu_get_timings(const xatu_ctx* ctx, xatu_timings* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* XATU_BSE_H */

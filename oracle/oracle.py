@@ -1,0 +1,412 @@
+"""CPU FP64 oracle for the Xatu BSE hot path -- TEST INFRASTRUCTURE ONLY.
+
+Used by ``tests/``, ``__graft_entry__.smoke()`` and the CPU-baseline leg of
+``bench.py`` as the checker of the HIP path; the product never imports it.
+
+Pieces and the reference code they restate (file:line in alejandrojuria/xatu):
+
+- ``parse_model``          src/ConfigurationBase.cpp:43-147, src/SystemConfiguration.cpp:33-228
+- ``System``               src/Lattice.cpp:56-131, 140-279 (reciprocal lattice, MP / reduced /
+                           shifted meshes, findEquivalentPointBZ), src/SystemTB.cpp:60-125
+                           (H(k) and eig_sym -> numpy.linalg.eigh = LAPACK zheevd)
+- ``fix_global_phase``     src/ExcitonTB.cpp:673-683
+- ``Exciton``              src/ExcitonTB.cpp:20-34, 128-144 (bands), 707-815
+                           (initializeResultsH0, initializeHamiltonian), 840-921 (BShamiltonian,
+                           through liboracle.so), 930-959 (diag = numpy eigh)
+- ``davidson_reference``   src/davidson.cpp:5-61, restated line by line (including its
+                           identity guess and dense (M - theta I) corrections)
+- ``liboracle.so``         oracle.cpp: potentials, lattice sums and the per-element BSE loop.
+
+Parity status: every function above is pinned by the reference's regression values
+(test/tests.cpp) and by closed forms / invariants in tests/test_oracle.py, except
+the explicit dV/dr0 kind of ``ft_stack`` (no reference counterpart), which is pinned
+by central finite differences of the potential in the same tests.
+"""
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+import numpy as np
+
+_HERE = Path(__file__).resolve().parent
+_LIB = None
+
+POT_KELDYSH, POT_COULOMB, POT_KELDYSH_DR0, POT_KELDYSH_DEPS = 0, 1, 2, 3
+_POT_NAMES = {"keldysh": POT_KELDYSH, "coulomb": POT_COULOMB}
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+def lib():
+    """Load (building if needed) oracle/liboracle.so."""
+    global _LIB
+    if _LIB is None:
+        so = _HERE / "liboracle.so"
+        if not so.exists():
+            import subprocess
+            subprocess.run(["make", "-s", "-C", str(_HERE)], check=True)
+        L = ctypes.CDLL(str(so))
+        L.oracle_stvh0.restype = ctypes.c_double
+        L.oracle_stvh0.argtypes = [ctypes.c_double]
+        L.oracle_y0.restype = ctypes.c_double
+        L.oracle_y0.argtypes = [ctypes.c_double]
+        L.oracle_h1.restype = ctypes.c_double
+        L.oracle_h1.argtypes = [ctypes.c_double]
+        L.oracle_potential.restype = ctypes.c_double
+        L.oracle_potential.argtypes = [ctypes.c_int] + [ctypes.c_double] * 6
+        L.oracle_truncate_supercell.restype = ctypes.c_int
+        L.oracle_truncate_supercell.argtypes = [ctypes.c_int, _dp, ctypes.c_int, ctypes.c_double, _dp, ctypes.c_int]
+        L.oracle_motif_ft.restype = None
+        L.oracle_motif_ft.argtypes = ([ctypes.c_int, _dp, ctypes.c_int, _dp, _dp, ctypes.c_int]
+                                      + [ctypes.c_double] * 6 + [_dp])
+        L.oracle_ft_stack.restype = None
+        L.oracle_ft_stack.argtypes = ([ctypes.c_int, _dp, ctypes.c_int, _dp, ctypes.c_int, _dp, ctypes.c_int]
+                                      + [ctypes.c_double] * 6 + [_dp])
+        L.oracle_find_equivalent.restype = ctypes.c_int
+        L.oracle_find_equivalent.argtypes = [ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int]
+        L.oracle_bse.restype = None
+        L.oracle_bse.argtypes = [ctypes.c_int, _ip, ctypes.c_int, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp,
+                                 _dp, ctypes.c_int, _ip, _dp, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp,
+                                 _dp, ctypes.c_int, ctypes.c_int, _dp]
+        _LIB = L
+    return _LIB
+
+
+def _d(a):
+    return np.ascontiguousarray(a).ctypes.data_as(_dp)
+
+
+# --------------------------------------------------------------------------- parsing
+
+def _blocks(path):
+    """'#' argument blocks of a Xatu text file (ConfigurationBase.cpp:57-119)."""
+    text = Path(path).read_text()
+    ends_nl = text.endswith("\n")
+    lines = text.split("\n")
+    if ends_nl:
+        lines = lines[:-1]
+    found, contents = [], {}
+    for i, line in enumerate(lines):
+        if "#" in line and i + 1 < len(lines):
+            found.append("".join(line[line.index("#") + 1:].split()).lower())
+    arg, content = "", []
+    for i, line in enumerate(lines):
+        if "#" in line:
+            if content:
+                contents[arg] = content
+                content = []
+            arg = "".join(line[line.index("#") + 1:].split()).lower()
+        elif not line or "!" in line:
+            pass
+        else:
+            content.append(line.replace(",", " ").replace(";", " "))
+        if i + 1 == len(lines) and not ends_nl:
+            contents[arg] = content
+    return found, contents
+
+
+def _matrices(content):
+    cplx = ("i" in content[0]) or ("j" in content[0])
+    dim = 0
+    for line in content:
+        if "&" in line:
+            break
+        dim += 1
+    mats, cur, row = [], np.zeros((dim, dim), complex), 0
+    for line in content:
+        if "&" in line:
+            mats.append(cur)
+            cur, row = np.zeros((dim, dim), complex), 0
+            continue
+        toks = line.split()
+        vals = []
+        if cplx:
+            for t in range(0, len(toks), 2):
+                vals.append(float(toks[t]) + 1j * float(toks[t + 1].rstrip("ij")))
+        else:
+            vals = [float(t) for t in toks]
+        cur[row, :len(vals)] = vals
+        row += 1
+    return np.array(mats)
+
+
+def parse_model(path):
+    found, c = _blocks(path)
+    m = {}
+    for arg in found:
+        content = c.get(arg, [])
+        if arg == "dimension":
+            m["ndim"] = int(content[0].split()[0])
+        elif arg == "bravaislattice":
+            m["bravais"] = np.array([[float(x) for x in l.split()] for l in content])
+        elif arg == "motif":
+            m["motif"] = np.array([[float(x) for x in l.split()[:4]] for l in content])
+        elif arg == "norbitals":
+            m["norbitals"] = [int(x) for x in content[0].split()]
+        elif arg == "filling":
+            m["filling"] = int(float(content[0].split()[0]))
+        elif arg == "bravaisvectors":
+            m["cells"] = np.array([[float(x) for x in l.split()] for l in content])
+        elif arg == "hamiltonian":
+            m["H"] = _matrices(content)
+        elif arg == "overlap":
+            m["S"] = _matrices(content)
+    return m
+
+
+# --------------------------------------------------------------------------- system
+
+def fix_global_phase(V):
+    """Columns scaled so that the sum of each column is real and >= 0."""
+    s = V.sum(axis=-2, keepdims=True)
+    return V * np.exp(-1j * np.angle(s))
+
+
+class System:
+    def __init__(self, model):
+        if isinstance(model, (str, Path)):
+            model = parse_model(model)
+        self.ndim = model["ndim"]
+        self.bravais = np.asarray(model["bravais"], float)
+        self.motif = np.asarray(model["motif"], float)
+        self.natoms = len(self.motif)
+        self.orbitals = model["norbitals"]
+        self.cells = np.asarray(model["cells"], float)
+        self.H = np.asarray(model["H"], complex)
+        self.S = model.get("S")
+        self.filling = model["filling"]
+        self.fermi = self.filling - 1
+        self.orb_per_atom = [self.orbitals[int(s)] for s in self.motif[:, 3]]
+        self.basisdim = sum(self.orb_per_atom)
+        self.a = float(np.linalg.norm(self.bravais[0]))
+        if self.ndim == 1:
+            self.recip = 2 * np.pi * self.bravais / np.linalg.norm(self.bravais) ** 2
+        else:
+            A = self.bravais[:, :self.ndim]
+            self.recip = np.zeros((self.ndim, 3))
+            for i in range(self.ndim):
+                e = np.zeros(self.ndim)
+                e[i] = 2 * np.pi
+                self.recip[i, :self.ndim] = np.linalg.solve(A, e)
+        self.inv = np.linalg.inv(self.recip @ self.recip.T)
+        self.factor = 1
+        self.kpoints = None
+        self.meshBZ = None
+
+    # Lattice.cpp:293-312 (first coefficient fastest)
+    def combinations(self, n, centered=False):
+        shift = n // 2 if centered else 0
+        grids = np.meshgrid(*[np.arange(n) - shift] * self.ndim, indexing="ij")
+        return np.stack([g.ravel(order="F") for g in grids], axis=1).astype(float)
+
+    def mp_mesh(self, n, factor=1):
+        c = self.combinations(n)
+        if n % 2 == 1:
+            c = c + 0.5
+        return ((2 * c - n) / (2 * n * factor)) @ self.recip
+
+    def brillouin_zone_mesh(self, n):
+        self.kpoints = self.mp_mesh(n)
+        self.meshBZ = self.kpoints.copy()
+
+    def reduced_brillouin_zone_mesh(self, n, factor):
+        self.brillouin_zone_mesh(n * factor)
+        self.kpoints = self.mp_mesh(n, factor)
+        self.factor = factor
+
+    def shift_bz(self, shift):
+        self.kpoints = self.kpoints + np.asarray(shift, float)
+
+    def truncate_supercell(self, ncell, radius):
+        out = np.zeros((ncell ** self.ndim, 3))
+        n = lib().oracle_truncate_supercell(self.ndim, _d(self.bravais), ncell, radius, _d(out), len(out))
+        return out[:n].copy()
+
+    def hamiltonian(self, k):
+        k = np.atleast_2d(k)
+        ph = np.exp(1j * (k @ self.cells.T))  # nk x ncells
+        return np.einsum("kr,rij->kij", ph, self.H)
+
+    def solve_bands(self, k):
+        h = self.hamiltonian(k)
+        if self.S is not None and len(self.S):
+            s = np.einsum("kr,rij->kij", np.exp(1j * (np.atleast_2d(k) @ self.cells.T)), self.S)
+            w, v = np.linalg.eigh(s)
+            root = np.einsum("kij,kj,klj->kil", v, 1 / np.sqrt(w), v.conj())
+            h = root @ h @ root
+        return np.linalg.eigh(h)
+
+    def find_equivalent(self, k, ncell):
+        k = np.asarray(k, float)
+        return lib().oracle_find_equivalent(self.ndim, _d(self.recip), _d(self.inv), _d(k), ncell, self.factor)
+
+
+# --------------------------------------------------------------------------- exciton
+
+class Exciton:
+    """The reference ExcitonTB pipeline on the CPU (realspace mode, lattice gauge)."""
+
+    def __init__(self, system, ncell, nbands=1, nrmbands=0, parameters=(1, 5, 1), Q=(0, 0, 0), bands=None):
+        self.sys = system if isinstance(system, System) else System(system)
+        self.ncell = ncell
+        self.eps_m, self.eps_s, self.r0 = map(float, parameters)
+        if self.r0 == 0:
+            raise ValueError("Error: r0 must be non-zero")
+        self.Q = np.asarray(Q, float)
+        self.cutoff = ncell / 2.5
+        self.scissor = 0.0
+        self.exchange = False
+        self.potential = "keldysh"
+        self.exchange_potential = "keldysh"
+        self.regularization = 0.0
+        f = self.sys.fermi
+        if bands is None:
+            self.valence = list(range(f - nbands - nrmbands + 1, f - nrmbands + 1))
+            self.conduction = list(range(f + 1 + nrmbands, f + nbands + nrmbands + 1))
+        else:
+            self.valence = [b + f for b in bands if b <= 0]
+            self.conduction = [b + f for b in bands if b > 0]
+        self.band_list = self.valence + self.conduction
+
+    def initialize(self):
+        s = self.sys
+        if s.kpoints is None:
+            raise ValueError("Error: BZ mesh must be initialized first")
+        if self.regularization < 1e-10:
+            self.regularization = s.a
+        self.nk = len(s.kpoints)
+        self.nv, self.nc = len(self.valence), len(self.conduction)
+        self.N = self.nk * self.nv * self.nc
+        self.total_cells = (self.ncell * s.factor) ** s.ndim
+        # basis rows {vcol, ccol, k}, order k > c > v (Exciton.tpp:128-147)
+        b = []
+        for k in range(self.nk):
+            for ic in range(self.nc):
+                for iv in range(self.nv):
+                    b.append((iv, self.nv + ic, k))
+        self.basis = np.array(b, dtype=np.int32)
+        bl = np.array(self.band_list)
+        w, v = s.solve_bands(s.kpoints)
+        v = fix_global_phase(v)
+        self.eigval_k = np.asfortranarray(w[:, bl].T)                    # nb x nk
+        self.eigvec_k = np.asfortranarray(np.transpose(v[:, :, bl], (1, 2, 0)))  # norb x nb x nk
+        if np.linalg.norm(self.Q) != 0:
+            w, v = s.solve_bands(s.kpoints + self.Q)
+            v = fix_global_phase(v)
+            self.eigval_kq = np.asfortranarray(w[:, bl].T)
+            self.eigvec_kq = np.asfortranarray(np.transpose(v[:, :, bl], (1, 2, 0)))
+        else:
+            self.eigval_kq = self.eigval_k.copy(order="F")
+            self.eigvec_kq = self.eigvec_k.copy(order="F")
+        self.cells = s.truncate_supercell(self.ncell, s.a * self.cutoff)
+        self.ft = self.ft_stack(_POT_NAMES[self.potential])
+        self.ftq = self.ft_at(self.Q, _POT_NAMES[self.exchange_potential]) if self.exchange \
+            else np.zeros((s.natoms, s.natoms), complex, order="F")
+
+    def _pot_args(self):
+        return (self.eps_m, self.eps_s, self.r0, self.regularization, self.sys.a * self.cutoff + 1e-5,
+                float(self.total_cells))
+
+    def ft_stack(self, kind):
+        s = self.sys
+        mesh = np.ascontiguousarray(s.meshBZ)
+        out = np.zeros((s.natoms, s.natoms, len(mesh)), complex, order="F")
+        lib().oracle_ft_stack(s.natoms, _d(s.motif[:, :3]), len(self.cells), _d(self.cells), len(mesh), _d(mesh),
+                              kind, *self._pot_args(), out.ctypes.data_as(_dp))
+        return out
+
+    def ft_at(self, k, kind):
+        s = self.sys
+        out = np.zeros((s.natoms, s.natoms), complex, order="F")
+        lib().oracle_motif_ft(s.natoms, _d(s.motif[:, :3]), len(self.cells), _d(self.cells),
+                              _d(np.asarray(k, float)), kind, *self._pot_args(), out.ctypes.data_as(_dp))
+        return out
+
+    def bse(self, ft=None, ftq=None, eigval_k=None, eigval_kq=None, scissor=None, eigvec_k=None,
+            eigvec_kq=None, exchange=None):
+        """Full HBS (+ conjugate lower triangle) from the restated per-element loop."""
+        s = self.sys
+        ft = self.ft if ft is None else ft
+        ftq = self.ftq if ftq is None else ftq
+        ek = self.eigval_k if eigval_k is None else eigval_k
+        ekq = self.eigval_kq if eigval_kq is None else eigval_kq
+        vk = self.eigvec_k if eigvec_k is None else eigvec_k
+        vkq = self.eigvec_kq if eigvec_kq is None else eigvec_kq
+        sc = self.scissor if scissor is None else scissor
+        ex = self.exchange if exchange is None else exchange
+        orb_atom = np.repeat(np.arange(s.natoms), s.orb_per_atom).astype(np.int32)
+        out = np.zeros((self.N, self.N), complex, order="F")
+        kp = np.ascontiguousarray(s.kpoints)
+
+        def fp(a):
+            a = np.asfortranarray(a)
+            return a, a.ctypes.data_as(_dp)
+
+        ft_, ftp = fp(ft)
+        ftq_, ftqp = fp(ftq)
+        vk_, vkp = fp(vk)
+        vkq_, vkqp = fp(vkq)
+        ek_, ekp = fp(ek)
+        ekq_, ekqp = fp(ekq)
+        lib().oracle_bse(self.N, self.basis.ctypes.data_as(_ip), self.nk, _d(kp), s.basisdim,
+                         len(self.band_list), vkp, vkqp, ekp, ekqp, s.natoms,
+                         orb_atom.ctypes.data_as(_ip), ftp, ftqp, int(bool(ex)), float(sc), s.ndim,
+                         _d(s.recip), _d(s.inv), self.ncell, s.factor, out.ctypes.data_as(_dp))
+        return out
+
+
+def davidson_reference(M, neigval, tol=1e-8):
+    """davidson_method restated (src/davidson.cpp:5-61); returns (eigval[:neigval], eigvec (N x 2 neigval))."""
+    n = M.shape[0]
+    if not np.allclose(M, M.conj().T, atol=0, rtol=0):
+        raise ValueError("davidson_method: provided matrix must be hermitian")
+    max_iterations = n // 2
+    k = 2 * neigval
+    guess = np.eye(n, k, dtype=complex)
+    aux = np.ones(neigval)
+    eigval = eigvec = None
+    for i in range(max_iterations):
+        Q, _ = np.linalg.qr(guess, mode="reduced")
+        P = Q.conj().T @ M @ Q
+        eigval, v = np.linalg.eigh(P)
+        eigvec = Q[:, :(i + 1) * k] @ v[:, :k]
+        new = []
+        for j in range(k):
+            w = (M - eigval[j] * np.eye(n)) @ eigvec[:, j]
+            new.append(w / (eigval[j] - M[j, j]))
+        Q = np.concatenate([Q, np.array(new).T], axis=1)
+        if np.linalg.norm(eigval[:neigval] - aux) < tol:
+            break
+        aux = eigval[:neigval].copy()
+        guess = Q
+    return eigval[:neigval], eigvec
+
+
+def detect_degeneracies(eigval, n, precision=6):
+    """src/utils.cpp:121-152."""
+    pairs, prev, deg = [], eigval[0], 1
+    thr = 10.0 ** (-precision)
+    for i in range(1, n):
+        e = eigval[i]
+        if abs(e - prev) < thr:
+            deg += 1
+        else:
+            pairs.append((prev, deg))
+            deg = 1
+        prev = e
+    pairs.append((prev, deg))
+    return pairs
+
+
+def array2hash(a):
+    """Test fingerprint (include/xatu/utils.hpp:61-78); rows/cols as Armadillo sees them."""
+    a = np.asarray(a)
+    if a.ndim == 1:
+        a = a[:, None]
+    i = np.arange(a.shape[0])[:, None]
+    j = np.arange(a.shape[1])[None, :]
+    c = a * (i * j + j + 1)
+    return float((np.abs(a).sum() + np.count_nonzero(a) + (np.real(c) * 1.5).sum() + (np.imag(c) * 1.5).sum()
+                  + (np.imag(c) * np.real(c)).sum()) / a.size)

@@ -1,0 +1,508 @@
+// C-ABI entry points (include/xatu_bse.h). Each call validates its inputs the way
+// the reference does, runs on the context's stream and converts exceptions into
+// status codes plus a per-context message.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+
+#include "context.hpp"
+
+using namespace xatu_dev;
+
+struct xatu_ctx {
+    Context c;
+};
+
+namespace {
+
+template <typename F>
+int guarded(xatu_ctx* ctx, F&& f) {
+    if (!ctx) return XATU_ERR_INVALID_ARGUMENT;
+    try {
+        XATU_HIP(hipSetDevice(ctx->c.device));
+        f(ctx->c);
+        ctx->c.err.clear();
+        return XATU_OK;
+    } catch (const Error& e) {
+        ctx->c.err = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        ctx->c.err = "host allocation failed";
+        return XATU_ERR_OUT_OF_MEMORY;
+    } catch (const std::exception& e) {
+        ctx->c.err = e.what();
+        return XATU_ERR_DEVICE;
+    }
+}
+
+// Centred supercell combinations within a sphere (Lattice::generateCombinations with
+// centered=true and Lattice::truncateSupercell, src/Lattice.cpp:293-341): coefficients
+// run over [0, n) - n/2 with the first one fastest; a cell is kept if |R| < radius + 1e-5.
+std::vector<int> truncate_supercell(const LatticeDesc& lat, int ncell, double radius) {
+    std::vector<int> out;
+    const int shift = ncell / 2;
+    long long total = 1;
+    for (int j = 0; j < lat.ndim; ++j) total *= ncell;
+    for (long long i = 0; i < total; ++i) {
+        int m[3] = {0, 0, 0};
+        long long rem = i;
+        for (int j = 0; j < lat.ndim; ++j) {
+            m[j] = static_cast<int>(rem % ncell) - shift;
+            rem /= ncell;
+        }
+        double v[3] = {0.0, 0.0, 0.0};
+        for (int j = 0; j < lat.ndim; ++j) {
+            v[0] += m[j] * lat.bravais[3 * j + 0];
+            v[1] += m[j] * lat.bravais[3 * j + 1];
+            v[2] += m[j] * lat.bravais[3 * j + 2];
+        }
+        const double d = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        if (d < radius + 1E-5) {
+            for (int j = 0; j < lat.ndim; ++j) out.push_back(m[j]);
+        }
+    }
+    return out;
+}
+
+PotDesc make_pot(const Context& c, int kind) {
+    if (kind != XATU_POTENTIAL_KELDYSH && kind != XATU_POTENTIAL_COULOMB) {
+        throw Error(XATU_ERR_INVALID_ARGUMENT, "selectPotential(): potential must be either 'keldysh' or 'coulomb'");
+    }
+    PotDesc p{};
+    p.kind = kind;
+    p.eps_bar = (c.inter.eps_m + c.inter.eps_s) / 2;
+    p.r0 = c.inter.r0;
+    p.reg = (c.inter.regularization < 1E-10) ? c.a1norm : c.inter.regularization;
+    p.cutoff = c.a1norm * c.inter.cutoff + 1E-5;
+    return p;
+}
+
+template <typename T>
+__global__ void diag_from_kernel(const T* __restrict__ H, int64_t N, double* __restrict__ diag) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < N) diag[i] = static_cast<double>(H[i + i * N].x);
+}
+
+void refresh_diag(Context& c, int precision) {
+    c.d_diag.alloc(c.N);
+    if (precision == XATU_FP64) {
+        hipLaunchKernelGGL(diag_from_kernel<double2>, dim3((c.N + 255) / 256), dim3(256), 0, c.stream, c.d_H64.p,
+                           c.N, c.d_diag.p);
+    } else {
+        hipLaunchKernelGGL(diag_from_kernel<float2>, dim3((c.N + 255) / 256), dim3(256), 0, c.stream, c.d_H32.p,
+                           c.N, c.d_diag.p);
+    }
+    XATU_HIP(hipGetLastError());
+    c.h_diag.resize(c.N);
+    XATU_HIP(hipMemcpyAsync(c.h_diag.data(), c.d_diag.p, c.N * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    XATU_HIP(hipStreamSynchronize(c.stream));
+}
+
+}  // namespace
+
+namespace xatu_dev {
+
+// Product with the resident matrix of the given precision (or matrix-free).
+void apply_operator(Context& c, int precision, int matrix_free, const double2* X, double2* Y, int nvec,
+                    double* ms_kernel) {
+    EventTimer ev(c.stream);
+    if (matrix_free) {
+        ev.start();
+        launch_matfree64(c, X, Y, nvec, c.stream);
+        ev.stop();
+    } else if (precision == XATU_FP64) {
+        if (!c.built64) throw Error(XATU_ERR_LOGIC, "FP64 BSE matrix is not resident");
+        ev.start();
+        launch_matvec64(c, X, Y, nvec, c.stream);
+        ev.stop();
+    } else {
+        if (!c.built32) throw Error(XATU_ERR_LOGIC, "FP32 BSE matrix is not resident");
+        const size_t n = static_cast<size_t>(c.N) * nvec;
+        c.s_x32.alloc(std::max(c.s_x32.n, n));
+        c.s_y32.alloc(std::max(c.s_y32.n, n));
+        launch_convert_64_to_32(X, c.s_x32.p, n, c.stream);
+        ev.start();
+        launch_matvec32(c, c.s_x32.p, c.s_y32.p, nvec, c.stream);
+        ev.stop();
+        launch_convert_32_to_64(c.s_y32.p, Y, n, c.stream);
+    }
+    const double ms = ev.ms();
+    if (ms_kernel) *ms_kernel = ms;
+    if (!matrix_free) {
+        c.tim.ms_matvec_kernel += ms;
+        c.tim.n_matvec_kernel += 1;
+    }
+}
+
+}  // namespace xatu_dev
+
+extern "C" {
+
+int xatu_create(xatu_ctx** out, int device) {
+    if (!out) return XATU_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    xatu_ctx* ctx = new (std::nothrow) xatu_ctx();
+    if (!ctx) return XATU_ERR_OUT_OF_MEMORY;
+    ctx->c.device = device;
+    int rc = guarded(ctx, [&](Context& c) {
+        int ndev = 0;
+        XATU_HIP(hipGetDeviceCount(&ndev));
+        if (device < 0 || device >= ndev) throw Error(XATU_ERR_INVALID_ARGUMENT, "no such HIP device");
+        XATU_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+        XATU_BLAS(rocblas_create_handle(&c.blas));
+        XATU_BLAS(rocblas_set_stream(c.blas, c.stream));
+    });
+    if (rc != XATU_OK) {
+        xatu_destroy(ctx);
+        return rc;
+    }
+    *out = ctx;
+    return XATU_OK;
+}
+
+void xatu_destroy(xatu_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->c.device);
+    if (ctx->c.stream) (void)hipStreamSynchronize(ctx->c.stream);
+    if (ctx->c.blas) rocblas_destroy_handle(ctx->c.blas);
+    hipStream_t s = ctx->c.stream;
+    delete ctx;  // frees device buffers
+    if (s) (void)hipStreamDestroy(s);
+}
+
+const char* xatu_last_error(const xatu_ctx* ctx) {
+    if (!ctx) return "null context";
+    return ctx->c.err.c_str();
+}
+
+int xatu_set_lattice(xatu_ctx* ctx, int ndim, const double* bravais, int natoms, const double* motif,
+                     const int* norb_per_atom, int mesh_n) {
+    return guarded(ctx, [&](Context& c) {
+        if (ndim < 1 || ndim > 3) throw Error(XATU_ERR_INVALID_ARGUMENT, "ndim must be 1, 2 or 3");
+        if (natoms < 1 || natoms > kMaxAtoms) throw Error(XATU_ERR_INVALID_ARGUMENT, "natoms must be in [1, 8]");
+        if (!bravais || !motif || !norb_per_atom) throw Error(XATU_ERR_INVALID_ARGUMENT, "null pointer");
+        if (mesh_n < 1) throw Error(XATU_ERR_INVALID_ARGUMENT, "mesh_n must be positive");
+        LatticeDesc L{};
+        L.ndim = ndim;
+        L.natoms = natoms;
+        L.mesh_n = mesh_n;
+        long long nm = 1;
+        for (int j = 0; j < ndim; ++j) nm *= mesh_n;
+        if (nm > (1LL << 30)) throw Error(XATU_ERR_INVALID_ARGUMENT, "mesh too large");
+        L.nmesh = static_cast<int>(nm);
+        for (int i = 0; i < 3 * ndim; ++i) L.bravais[i] = bravais[i];
+        for (int i = 0; i < 3 * natoms; ++i) L.tau[i] = motif[i];
+        L.orb_begin[0] = 0;
+        for (int a = 0; a < natoms; ++a) {
+            if (norb_per_atom[a] < 1) throw Error(XATU_ERR_INVALID_ARGUMENT, "every atom needs >= 1 orbital");
+            L.orb_begin[a + 1] = L.orb_begin[a] + norb_per_atom[a];
+        }
+        L.norb = L.orb_begin[natoms];
+        c.lat = L;
+        c.a1norm = std::sqrt(bravais[0] * bravais[0] + bravais[1] * bravais[1] + bravais[2] * bravais[2]);
+        c.have_lattice = true;
+        c.have_bands = c.have_ft = c.built64 = c.built32 = false;
+    });
+}
+
+int xatu_set_bands(xatu_ctx* ctx, int nk, int nv, int nc, const double* eigval_k, const double* eigval_kq,
+                   const double* eigvec_k, const double* eigvec_kq, const int* k_offsets) {
+    return guarded(ctx, [&](Context& c) {
+        c.require_lattice();
+        if (nv < 1 || nc < 1) throw Error(XATU_ERR_INVALID_ARGUMENT, "Error: Exciton object must have some bands");
+        if (nk < 1) throw Error(XATU_ERR_INVALID_ARGUMENT, "Error: BZ mesh must be initialized first");
+        if (!eigval_k || !eigvec_k || !k_offsets) throw Error(XATU_ERR_INVALID_ARGUMENT, "null pointer");
+        if (!eigval_kq) eigval_kq = eigval_k;
+        if (!eigvec_kq) eigvec_kq = eigvec_k;
+        const int nb = nv + nc;
+        if (nb > c.lat.norb) {
+            throw Error(XATU_ERR_INVALID_ARGUMENT, "Error: Number of bands cannot be higher than actual material bands");
+        }
+        c.nk = nk;
+        c.nv = nv;
+        c.nc = nc;
+        c.nb = nb;
+        c.N = static_cast<int64_t>(nk) * nv * nc;
+        const size_t ne = static_cast<size_t>(nb) * nk;
+        const size_t nvec = static_cast<size_t>(c.lat.norb) * nb * nk;
+        c.h_eval_k.assign(eigval_k, eigval_k + ne);
+        c.h_eval_kq.assign(eigval_kq, eigval_kq + ne);
+        c.d_eval_k.alloc(ne);
+        c.d_eval_kq.alloc(ne);
+        c.d_evec_k.alloc(nvec);
+        c.d_evec_kq.alloc(nvec);
+        XATU_HIP(hipMemcpyAsync(c.d_eval_k.p, eigval_k, ne * sizeof(double), hipMemcpyHostToDevice, c.stream));
+        XATU_HIP(hipMemcpyAsync(c.d_eval_kq.p, eigval_kq, ne * sizeof(double), hipMemcpyHostToDevice, c.stream));
+        XATU_HIP(hipMemcpyAsync(c.d_evec_k.p, eigvec_k, nvec * sizeof(double2), hipMemcpyHostToDevice, c.stream));
+        XATU_HIP(hipMemcpyAsync(c.d_evec_kq.p, eigvec_kq, nvec * sizeof(double2), hipMemcpyHostToDevice, c.stream));
+        std::vector<int> coord(static_cast<size_t>(nk) * 3, 0);
+        c.h_koff.assign(k_offsets, k_offsets + static_cast<size_t>(nk) * c.lat.ndim);
+        for (int k = 0; k < nk; ++k)
+            for (int j = 0; j < c.lat.ndim; ++j) coord[3 * k + j] = k_offsets[k * c.lat.ndim + j];
+        c.d_kcoord.alloc(coord.size());
+        XATU_HIP(hipMemcpyAsync(c.d_kcoord.p, coord.data(), coord.size() * sizeof(int), hipMemcpyHostToDevice,
+                                c.stream));
+        XATU_HIP(hipStreamSynchronize(c.stream));
+        c.have_bands = true;
+        c.built64 = c.built32 = false;
+    });
+}
+
+int xatu_fourier_sums(xatu_ctx* ctx, const xatu_interaction* inter) {
+    return guarded(ctx, [&](Context& c) {
+        c.require_lattice();
+        if (!inter) throw Error(XATU_ERR_INVALID_ARGUMENT, "null interaction");
+        if (inter->r0 == 0) throw Error(XATU_ERR_INVALID_ARGUMENT, "Error: r0 must be non-zero");
+        if (inter->ncell < 1) throw Error(XATU_ERR_INVALID_ARGUMENT, "ncell must be a positive number");
+        if (inter->total_cells < 1) throw Error(XATU_ERR_INVALID_ARGUMENT, "total_cells must be positive");
+        c.inter = *inter;
+        c.pot_d = make_pot(c, inter->potential);
+        c.pot_x = inter->exchange ? make_pot(c, inter->exchange_potential) : c.pot_d;
+        c.h_cells = truncate_supercell(c.lat, inter->ncell, c.a1norm * inter->cutoff);
+        c.nR = static_cast<int>(c.h_cells.size() / c.lat.ndim);
+        if (c.nR == 0) throw Error(XATU_ERR_INVALID_ARGUMENT, "no lattice vectors inside the cutoff");
+        std::vector<int> padded(static_cast<size_t>(c.nR) * 3, 0);
+        for (int r = 0; r < c.nR; ++r)
+            for (int j = 0; j < c.lat.ndim; ++j) padded[3 * r + j] = c.h_cells[r * c.lat.ndim + j];
+        c.d_cells.alloc(padded.size());
+        XATU_HIP(hipMemcpyAsync(c.d_cells.p, padded.data(), padded.size() * sizeof(int), hipMemcpyHostToDevice,
+                                c.stream));
+        const int npairs = c.lat.natoms * (c.lat.natoms + 1) / 2;
+        const int na2 = c.lat.natoms * c.lat.natoms;
+        c.d_vtab.alloc(static_cast<size_t>(npairs) * c.nR);
+        c.d_twiddle.alloc(2 * c.lat.mesh_n);
+        c.d_ft.alloc(static_cast<size_t>(na2) * c.lat.nmesh);
+        c.d_ftq.alloc(na2);
+        EventTimer ev(c.stream);
+        ev.start();
+        launch_potential_table(c, c.pot_d, c.d_vtab.p, c.stream);
+        launch_twiddles(c.lat.mesh_n, c.d_twiddle.p, c.stream);
+        launch_fourier_mesh(c, c.stream);
+        if (inter->exchange) {
+            c.d_vtabx.alloc(static_cast<size_t>(npairs) * c.nR);
+            launch_potential_table(c, c.pot_x, c.d_vtabx.p, c.stream);
+            launch_fourier_q(c, c.stream);
+        } else {
+            XATU_HIP(hipMemsetAsync(c.d_ftq.p, 0, na2 * sizeof(double2), c.stream));
+        }
+        ev.stop();
+        c.tim.ms_fourier = ev.ms();
+        c.have_ft = true;
+        c.built64 = c.built32 = false;
+    });
+}
+
+int xatu_build_bse(xatu_ctx* ctx, int precision, double scissor, void* hbs_out) {
+    return guarded(ctx, [&](Context& c) {
+        c.require_lattice();
+        c.require_bands();
+        c.require_ft();
+        if (precision != XATU_FP64 && precision != XATU_FP32) throw Error(XATU_ERR_INVALID_ARGUMENT, "bad precision");
+        c.scissor = scissor;
+        const size_t nn = static_cast<size_t>(c.N) * static_cast<size_t>(c.N);
+        EventTimer ev(c.stream), evk(c.stream);
+        ev.start();
+        if (c.inter.exchange) {
+            const size_t ne = static_cast<size_t>(c.nk) * c.lat.natoms * c.nc * c.nv;
+            c.d_e1.alloc(ne);
+            c.d_e2.alloc(ne);
+            launch_exchange_factors(c, c.stream);
+        } else {
+            c.d_e1.alloc(1);
+            c.d_e2.alloc(1);
+        }
+        if (precision == XATU_FP64) {
+            c.built64 = false;
+            c.d_H64.alloc(nn);
+            evk.start();
+            launch_build_tiles64(c, c.d_H64.p, c.stream);
+            evk.stop();
+        } else {
+            c.built32 = false;
+            c.d_H32.alloc(nn);
+            evk.start();
+            launch_build_tiles32(c, c.d_H32.p, c.stream);
+            evk.stop();
+        }
+        ev.stop();
+        c.tim.ms_build = ev.ms();
+        c.tim.ms_build_kernel = evk.ms();
+        if (precision == XATU_FP64) {
+            c.built64 = true;
+        } else {
+            c.built32 = true;
+        }
+        refresh_diag(c, precision);
+        if (hbs_out) {
+            const size_t bytes = nn * (precision == XATU_FP64 ? sizeof(double2) : sizeof(float2));
+            const void* src = precision == XATU_FP64 ? static_cast<const void*>(c.d_H64.p)
+                                                     : static_cast<const void*>(c.d_H32.p);
+            XATU_HIP(hipMemcpy(hbs_out, src, bytes, hipMemcpyDeviceToHost));
+        }
+    });
+}
+
+int xatu_build_bse_basis(xatu_ctx* ctx, int nbasis, const int* basis, double scissor, double* hbs_out) {
+    return guarded(ctx, [&](Context& c) {
+        c.require_lattice();
+        c.require_bands();
+        c.require_ft();
+        if (nbasis < 1 || !basis || !hbs_out) throw Error(XATU_ERR_INVALID_ARGUMENT, "bad basis");
+        for (int i = 0; i < nbasis; ++i) {
+            const int v = basis[3 * i], cc = basis[3 * i + 1], k = basis[3 * i + 2];
+            if (v < 0 || v >= c.nb || cc < 0 || cc >= c.nb || k < 0 || k >= c.nk) {
+                throw Error(XATU_ERR_INVALID_ARGUMENT, "basis entry out of range");
+            }
+        }
+        c.scissor = scissor;
+        DevBuf<int> d_basis;
+        d_basis.alloc(static_cast<size_t>(nbasis) * 3);
+        XATU_HIP(hipMemcpyAsync(d_basis.p, basis, d_basis.bytes(), hipMemcpyHostToDevice, c.stream));
+        DevBuf<double2> H;
+        H.alloc(static_cast<size_t>(nbasis) * nbasis);
+        // exchange factors are computed on the fly by the element kernel
+        launch_build_basis(c, nbasis, d_basis.p, H.p, c.stream);
+        XATU_HIP(hipMemcpyAsync(hbs_out, H.p, H.bytes(), hipMemcpyDeviceToHost, c.stream));
+        XATU_HIP(hipStreamSynchronize(c.stream));
+    });
+}
+
+int xatu_get_bse(xatu_ctx* ctx, int precision, void* hbs_out) {
+    return guarded(ctx, [&](Context& c) {
+        if (!hbs_out) throw Error(XATU_ERR_INVALID_ARGUMENT, "null output");
+        const size_t nn = static_cast<size_t>(c.N) * static_cast<size_t>(c.N);
+        if (precision == XATU_FP64) {
+            if (!c.built64) throw Error(XATU_ERR_LOGIC, "FP64 BSE matrix is not resident");
+            XATU_HIP(hipMemcpy(hbs_out, c.d_H64.p, nn * sizeof(double2), hipMemcpyDeviceToHost));
+        } else {
+            if (!c.built32) throw Error(XATU_ERR_LOGIC, "FP32 BSE matrix is not resident");
+            XATU_HIP(hipMemcpy(hbs_out, c.d_H32.p, nn * sizeof(float2), hipMemcpyDeviceToHost));
+        }
+    });
+}
+
+int xatu_free_bse(xatu_ctx* ctx, int precision) {
+    return guarded(ctx, [&](Context& c) {
+        XATU_HIP(hipStreamSynchronize(c.stream));
+        if (precision == XATU_FP64) {
+            c.d_H64.release();
+            c.built64 = false;
+        } else {
+            c.d_H32.release();
+            c.built32 = false;
+        }
+    });
+}
+
+int xatu_solve(xatu_ctx* ctx, const xatu_solve_options* opt, double* eigval_out, double* eigvec_out,
+               xatu_solve_report* report) {
+    return guarded(ctx, [&](Context& c) {
+        if (!opt || !eigval_out) throw Error(XATU_ERR_INVALID_ARGUMENT, "null argument");
+        xatu_solve_report rep{};
+        c.tim.ms_matvec_kernel = 0.0;
+        c.tim.n_matvec_kernel = 0;
+        if (opt->method == XATU_SOLVE_DIAG) {
+            solve_dense_diag(c, opt->nstates, eigval_out, eigvec_out);
+            rep.converged = 1;
+        } else if (opt->method == XATU_SOLVE_DAVIDSON) {
+            if (!opt->matrix_free) {
+                if (opt->precision == XATU_FP64 && !c.built64) {
+                    throw Error(XATU_ERR_LOGIC, "diagonalizeRaw(): BSE Hamiltonian is not initialized.");
+                }
+                if (opt->precision == XATU_FP32 && !c.built32) {
+                    throw Error(XATU_ERR_LOGIC, "diagonalizeRaw(): BSE Hamiltonian is not initialized.");
+                }
+            }
+            if (c.h_diag.size() != static_cast<size_t>(c.N)) {
+                throw Error(XATU_ERR_LOGIC, "diagonalizeRaw(): BSE Hamiltonian is not initialized.");
+            }
+            solve_davidson(c, *opt, eigval_out, eigvec_out, &rep);
+        } else {
+            throw Error(XATU_ERR_INVALID_ARGUMENT, "unknown solve method");
+        }
+        if (c.tim.n_matvec_kernel > 0) c.tim.ms_matvec_kernel /= c.tim.n_matvec_kernel;
+        if (report) *report = rep;
+    });
+}
+
+int xatu_bse_matvec(xatu_ctx* ctx, int precision, int matrix_free, int nvec, const double* x, double* y) {
+    return guarded(ctx, [&](Context& c) {
+        if (nvec < 1 || !x || !y) throw Error(XATU_ERR_INVALID_ARGUMENT, "bad vectors");
+        c.require_bands();
+        const size_t n = static_cast<size_t>(c.N) * nvec;
+        DevBuf<double2> X, Y;
+        X.alloc(n);
+        Y.alloc(n);
+        XATU_HIP(hipMemcpyAsync(X.p, x, n * sizeof(double2), hipMemcpyHostToDevice, c.stream));
+        apply_operator(c, precision, matrix_free, X.p, Y.p, nvec, nullptr);
+        XATU_HIP(hipMemcpyAsync(y, Y.p, n * sizeof(double2), hipMemcpyDeviceToHost, c.stream));
+        XATU_HIP(hipStreamSynchronize(c.stream));
+    });
+}
+
+int xatu_hf_gradients(xatu_ctx* ctx, int nstates, const double* eigvec, int nparams, const int* params,
+                      double* grad_out) {
+    return guarded(ctx, [&](Context& c) {
+        c.require_ft();
+        if (nstates < 1 || nparams < 1 || !params || !grad_out) throw Error(XATU_ERR_INVALID_ARGUMENT, "bad args");
+        for (int i = 0; i < nparams; ++i) {
+            if (params[i] < XATU_PARAM_R0 || params[i] > XATU_PARAM_EPS_S) {
+                throw Error(XATU_ERR_INVALID_ARGUMENT, "unknown gradient parameter");
+            }
+        }
+        DevBuf<double2> V;
+        const double2* dv;
+        if (eigvec) {
+            V.alloc(static_cast<size_t>(c.N) * nstates);
+            XATU_HIP(hipMemcpyAsync(V.p, eigvec, V.bytes(), hipMemcpyHostToDevice, c.stream));
+            dv = V.p;
+        } else {
+            if (c.last_nstates < nstates) throw Error(XATU_ERR_LOGIC, "no eigenvectors from a previous solve");
+            dv = c.d_last_vec.p;
+        }
+        hf_gradients(c, nstates, dv, nparams, params, grad_out);
+    });
+}
+
+int xatu_get_fourier(xatu_ctx* ctx, double* ft_out, double* ftq_out) {
+    return guarded(ctx, [&](Context& c) {
+        c.require_ft();
+        const size_t na2 = static_cast<size_t>(c.lat.natoms) * c.lat.natoms;
+        if (ft_out) XATU_HIP(hipMemcpy(ft_out, c.d_ft.p, na2 * c.lat.nmesh * sizeof(double2), hipMemcpyDeviceToHost));
+        if (ftq_out) XATU_HIP(hipMemcpy(ftq_out, c.d_ftq.p, na2 * sizeof(double2), hipMemcpyDeviceToHost));
+    });
+}
+
+int xatu_get_cells(xatu_ctx* ctx, int* n_cells, int* coefs) {
+    return guarded(ctx, [&](Context& c) {
+        c.require_ft();
+        if (n_cells) *n_cells = c.nR;
+        if (coefs) std::memcpy(coefs, c.h_cells.data(), c.h_cells.size() * sizeof(int));
+    });
+}
+
+int xatu_compare_builds(xatu_ctx* ctx, int kblock, double* block_err, double* max_abs) {
+    return guarded(ctx, [&](Context& c) {
+        if (!c.built64 || !c.built32) throw Error(XATU_ERR_LOGIC, "both FP64 and FP32 matrices must be resident");
+        if (kblock < 1) throw Error(XATU_ERR_INVALID_ARGUMENT, "kblock must be positive");
+        const int rows = kblock * c.nc * c.nv;
+        const int nblk = static_cast<int>((c.N + rows - 1) / rows);
+        DevBuf<double> d;
+        d.alloc(static_cast<size_t>(nblk) * nblk);
+        launch_compare(c, kblock, d.p, c.stream);
+        std::vector<double> h(d.n);
+        XATU_HIP(hipMemcpyAsync(h.data(), d.p, d.bytes(), hipMemcpyDeviceToHost, c.stream));
+        XATU_HIP(hipStreamSynchronize(c.stream));
+        if (block_err) std::memcpy(block_err, h.data(), d.bytes());
+        if (max_abs) *max_abs = *std::max_element(h.begin(), h.end());
+    });
+}
+
+int xatu_get_timings(const xatu_ctx* ctx, xatu_timings* out) {
+    if (!ctx || !out) return XATU_ERR_INVALID_ARGUMENT;
+    *out = ctx->c.tim;
+    return XATU_OK;
+}
+
+}  // extern "C"

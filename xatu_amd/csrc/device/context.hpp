@@ -1,0 +1,210 @@
+// Internal state of an xatu_ctx and helpers shared by the device sources.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../../include/xatu_bse.h"
+
+namespace xatu_dev {
+
+// Error carrying an XATU_ERR_* status through the C-ABI layer.
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define XATU_HIP(call)                                                                          \
+    do {                                                                                        \
+        hipError_t e_ = (call);                                                                 \
+        if (e_ != hipSuccess) {                                                                 \
+            throw ::xatu_dev::Error(e_ == hipErrorOutOfMemory ? XATU_ERR_OUT_OF_MEMORY          \
+                                                              : XATU_ERR_DEVICE,                \
+                                    std::string(#call) + ": " + hipGetErrorString(e_));         \
+        }                                                                                       \
+    } while (0)
+
+#define XATU_BLAS(call)                                                                         \
+    do {                                                                                        \
+        rocblas_status s_ = (call);                                                             \
+        if (s_ != rocblas_status_success) {                                                     \
+            throw ::xatu_dev::Error(XATU_ERR_DEVICE, std::string(#call) + ": rocblas status " + \
+                                                         std::to_string(static_cast<int>(s_))); \
+        }                                                                                       \
+    } while (0)
+
+// Owning device allocation.
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void alloc(size_t count) {
+        if (count == n && p) return;
+        release();
+        if (count == 0) return;
+        XATU_HIP(hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)));
+        n = count;
+    }
+    size_t bytes() const { return n * sizeof(T); }
+};
+
+// Geometry and lattice-sum description shared with the kernels (by value).
+struct LatticeDesc {
+    int ndim;
+    int natoms;
+    int norb;
+    int mesh_n;   // points per axis of the full mesh
+    int nmesh;    // mesh_n^ndim
+    double bravais[9];
+    double tau[8 * 3];  // atomic positions (natoms <= 8)
+    int orb_begin[9];   // orbital offset of each atom, orb_begin[natoms] = norb
+};
+
+constexpr int kMaxAtoms = 8;
+
+// Potential parameters for the device evaluation of V(r).
+struct PotDesc {
+    int kind;         // XATU_POTENTIAL_*
+    double eps_bar;   // (eps_m + eps_s) / 2
+    double r0;
+    double reg;       // regularization distance
+    double cutoff;    // |a1|*cutoff_ + 1e-5
+};
+
+struct Context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    rocblas_handle blas = nullptr;
+    std::string err;
+
+    // lattice
+    bool have_lattice = false;
+    LatticeDesc lat{};
+    double a1norm = 0.0;
+
+    // bands
+    bool have_bands = false;
+    int nk = 0, nv = 0, nc = 0, nb = 0;
+    int64_t N = 0;
+    std::vector<double> h_eval_k, h_eval_kq;  // nb x nk
+    std::vector<int> h_koff;                  // nk x ndim
+    DevBuf<double> d_eval_k, d_eval_kq;       // nb x nk
+    DevBuf<double2> d_evec_k, d_evec_kq;      // norb x nb x nk
+    DevBuf<int> d_kq;                         // nk: mesh linear index of each k (offset coords)
+    DevBuf<int> d_kcoord;                     // nk x 3 mesh coordinates (padded to 3)
+
+    // interaction
+    bool have_ft = false;
+    xatu_interaction inter{};
+    PotDesc pot_d{}, pot_x{};
+    int nR = 0;
+    std::vector<int> h_cells;   // nR x ndim integer coefficients
+    DevBuf<int> d_cells;        // nR x 3 (padded)
+    DevBuf<double> d_vtab;      // npairs x nR direct potential table
+    DevBuf<double> d_vtabx;     // npairs x nR exchange potential table
+    DevBuf<double2> d_twiddle;  // 2*mesh_n roots of unity
+    DevBuf<double2> d_ft;       // natoms x natoms x nmesh
+    DevBuf<double2> d_ftq;      // natoms x natoms
+    DevBuf<double2> d_e1, d_e2; // exchange factors: nk x natoms x (nc*nv)
+
+    // resident matrices
+    double scissor = 0.0;
+    bool built64 = false, built32 = false;
+    DevBuf<double2> d_H64;
+    DevBuf<float2> d_H32;
+    std::vector<double> h_diag;  // diagonal of H (FP64), N
+    DevBuf<double> d_diag;
+
+    // scratch for FP32 products
+    DevBuf<float2> s_x32, s_y32;
+
+    // last solve
+    int last_nstates = 0;
+    DevBuf<double2> d_last_vec;  // N x last_nstates
+    std::vector<double> last_val;
+
+    // timings
+    xatu_timings tim{};
+
+    void require_lattice() const {
+        if (!have_lattice) throw Error(XATU_ERR_LOGIC, "lattice not set (call xatu_set_lattice first)");
+    }
+    void require_bands() const {
+        if (!have_bands) throw Error(XATU_ERR_LOGIC, "Error: BZ mesh must be initialized first");
+    }
+    void require_ft() const {
+        if (!have_ft) throw Error(XATU_ERR_LOGIC, "lattice Fourier sums not computed (call xatu_fourier_sums)");
+    }
+};
+
+// Simple event timer on the context stream.
+struct EventTimer {
+    hipEvent_t a = nullptr, b = nullptr;
+    hipStream_t s;
+    explicit EventTimer(hipStream_t st) : s(st) {
+        XATU_HIP(hipEventCreate(&a));
+        XATU_HIP(hipEventCreate(&b));
+    }
+    ~EventTimer() {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+    void start() { XATU_HIP(hipEventRecord(a, s)); }
+    void stop() { XATU_HIP(hipEventRecord(b, s)); }
+    double ms() {
+        XATU_HIP(hipEventSynchronize(b));
+        float t = 0.f;
+        XATU_HIP(hipEventElapsedTime(&t, a, b));
+        return static_cast<double>(t);
+    }
+};
+
+// Kernel launchers implemented in the .hip translation units.
+void launch_potential_table(const Context& c, const PotDesc& pot, double* out, hipStream_t s);
+void launch_twiddles(int mesh_n, double2* out, hipStream_t s);
+void launch_fourier_mesh(const Context& c, hipStream_t s);
+void launch_fourier_q(const Context& c, hipStream_t s);
+void launch_exchange_factors(const Context& c, hipStream_t s);
+void launch_build_tiles64(const Context& c, double2* H, hipStream_t s);
+void launch_build_tiles32(const Context& c, float2* H, hipStream_t s);
+void launch_build_basis(const Context& c, int nbasis, const int* d_basis, double2* H, hipStream_t s);
+void launch_diag_extract(const Context& c, const double2* H, double* diag, hipStream_t s);
+void launch_compare(const Context& c, int kblock, double* d_blockerr, hipStream_t s);
+
+// Dense products Y = H X (N x nvec, complex) for the resident matrices.
+void launch_matvec64(const Context& c, const double2* X, double2* Y, int nvec, hipStream_t s);
+void launch_matvec32(const Context& c, const float2* X, float2* Y, int nvec, hipStream_t s);
+// Matrix-free product in FP64 / FP32 arithmetic (tiles recomputed).
+void launch_matfree64(const Context& c, const double2* X, double2* Y, int nvec, hipStream_t s);
+void launch_matfree32(const Context& c, const float2* X, float2* Y, int nvec, hipStream_t s);
+
+// Small vector kernels used by the solver.
+void launch_convert_64_to_32(const double2* x, float2* y, size_t n, hipStream_t s);
+void launch_convert_32_to_64(const float2* x, double2* y, size_t n, hipStream_t s);
+
+// Solver entry points (davidson.cpp).
+void solve_dense_diag(Context& c, int nstates, double* eigval_out, double* eigvec_out);
+void solve_davidson(Context& c, const xatu_solve_options& opt, double* eigval_out, double* eigvec_out,
+                    xatu_solve_report* rep);
+void apply_operator(Context& c, int precision, int matrix_free, const double2* X, double2* Y, int nvec,
+                    double* ms_kernel);
+
+// Hellmann-Feynman gradients (hf.hip / capi.cpp).
+void hf_gradients(Context& c, int nstates, const double2* d_vec, int nparams, const int* params,
+                  double* grad_out);
+
+}  // namespace xatu_dev

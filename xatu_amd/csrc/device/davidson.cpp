@@ -1,0 +1,371 @@
+// Eigensolvers for the resident (or matrix-free) BSE operator.
+//
+// Dense path ("diag", reference src/ExcitonTB.cpp:940-943 -> LAPACK zheevd):
+// rocSOLVER zheevd on a device copy of the FP64 matrix, all eigenpairs ascending.
+//
+// Block Davidson ("davidson", reference src/davidson.cpp:5-61, redesigned):
+// the basis V, its image W = H V, Ritz vectors and residuals live in HBM in FP64;
+// products H V use the FP64 or the FP32 matrix (or the matrix-free product);
+// the projected matrix V^H W is formed incrementally (rocBLAS zgemm, FP64) and
+// diagonalised on the host (Rayleigh-Ritz); corrections use the diagonal
+// preconditioner r / (theta - diag H), are orthogonalised twice against V
+// (classical Gram-Schmidt, FP64) and orthonormalised by an eigen-decomposition
+// of their Gram matrix, dropping dependent directions. The basis restarts from
+// the current Ritz block when it would exceed max_basis.
+// Unlike the reference, the guess is the unit vectors on the smallest diagonal
+// entries (the reference uses the first 2*nstates unit vectors, :19), no dense
+// (M - theta I) is ever formed (:37), and convergence is on residual norms
+// rather than on the change of the eigenvalues (:43).
+// The FP32 form runs this loop with FP32 products to a loose tolerance and then
+// refines in FP64: it restarts from the FP32 Ritz block and iterates with FP64
+// products until the FP64 residuals meet the tolerance.
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <numeric>
+
+#include "../common/heev.hpp"
+#include "context.hpp"
+
+namespace xatu_dev {
+
+void launch_residual(const double2* X, const double2* AX, const double* theta, int64_t N, int nvec, double2* R,
+                     double* partial, int* nblocks, hipStream_t s);
+void launch_precondition(const double2* R, const double* theta, const double* diag, int64_t N, int nvec, double2* T,
+                         hipStream_t s);
+void launch_unit_vectors(const int* idx, int64_t N, int nvec, double2* V, hipStream_t s);
+
+namespace {
+
+using cd = std::complex<double>;
+
+const rocblas_double_complex kOne{1.0, 0.0};
+const rocblas_double_complex kZero{0.0, 0.0};
+const rocblas_double_complex kMinusOne{-1.0, 0.0};
+
+inline rocblas_double_complex* bz(double2* p) { return reinterpret_cast<rocblas_double_complex*>(p); }
+inline const rocblas_double_complex* bz(const double2* p) {
+    return reinterpret_cast<const rocblas_double_complex*>(p);
+}
+
+double now_ms() {
+    using namespace std::chrono;
+    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+struct Workspace {
+    DevBuf<double2> V, W, X, AX, R, Tb, tmp;
+    DevBuf<double2> P;       // projections (mmax x b)
+    DevBuf<double2> Y;       // Ritz coefficients (mmax x b)
+    DevBuf<double> theta;    // b
+    DevBuf<double> partial;  // residual partial sums
+    DevBuf<float2> x32, y32;
+};
+
+class Davidson {
+public:
+    Davidson(Context& c, int precision, int matrix_free, int nev, int block, int mmax, xatu_solve_report* rep)
+        : c_(c), prec_(precision), mf_(matrix_free), nev_(nev), b_(block), mmax_(mmax), rep_(rep) {
+        N_ = c.N;
+        ws_.V.alloc(static_cast<size_t>(N_) * mmax_);
+        ws_.W.alloc(static_cast<size_t>(N_) * mmax_);
+        ws_.X.alloc(static_cast<size_t>(N_) * b_);
+        ws_.AX.alloc(static_cast<size_t>(N_) * b_);
+        ws_.R.alloc(static_cast<size_t>(N_) * b_);
+        ws_.Tb.alloc(static_cast<size_t>(N_) * b_);
+        ws_.P.alloc(static_cast<size_t>(mmax_) * b_);
+        ws_.Y.alloc(static_cast<size_t>(mmax_) * b_);
+        ws_.theta.alloc(b_);
+        ws_.partial.alloc(static_cast<size_t>(b_) * ((N_ + 255) / 256));
+        T_.assign(static_cast<size_t>(mmax_) * mmax_, cd(0.0, 0.0));
+    }
+
+    // Start from the unit vectors on the b smallest diagonal entries.
+    void init_unit() {
+        std::vector<int> idx(N_);
+        std::iota(idx.begin(), idx.end(), 0);
+        std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return c_.h_diag[a] < c_.h_diag[b]; });
+        idx.resize(b_);
+        DevBuf<int> d_idx;
+        d_idx.alloc(b_);
+        XATU_HIP(hipMemcpyAsync(d_idx.p, idx.data(), b_ * sizeof(int), hipMemcpyHostToDevice, c_.stream));
+        XATU_HIP(hipMemsetAsync(ws_.V.p, 0, static_cast<size_t>(N_) * b_ * sizeof(double2), c_.stream));
+        launch_unit_vectors(d_idx.p, N_, b_, ws_.V.p, c_.stream);
+        m_ = 0;
+        append_block(b_);
+        XATU_HIP(hipStreamSynchronize(c_.stream));
+    }
+
+    // Start from an orthonormal block already in X (N x b): V = X.
+    void init_from_X() {
+        XATU_HIP(hipMemcpyAsync(ws_.V.p, ws_.X.p, static_cast<size_t>(N_) * b_ * sizeof(double2),
+                                hipMemcpyDeviceToDevice, c_.stream));
+        m_ = 0;
+        append_block(b_);
+    }
+
+    // Iterate to convergence; returns true if converged.
+    bool run(double tol, int maxit) {
+        for (int it = 0; it < maxit; ++it) {
+            rayleigh_ritz();
+            ritz_and_residuals();
+            rep_->iterations++;
+            double worst = 0.0;
+            for (int i = 0; i < nev_; ++i) worst = std::max(worst, res_[i]);
+            last_worst_ = worst;
+            if (worst < tol) return true;
+            if (m_ + b_ > mmax_) restart();
+            if (!add_corrections()) return false;  // no new direction: stagnated
+        }
+        return false;
+    }
+
+    void set_precision(int precision, int matrix_free) {
+        prec_ = precision;
+        mf_ = matrix_free;
+    }
+
+    double worst() const { return last_worst_; }
+    const std::vector<double>& theta() const { return theta_; }
+    const double2* ritz_vectors() const { return ws_.X.p; }
+    double products_ms() const { return prod_ms_; }
+    double rr_ms() const { return rr_ms_; }
+
+private:
+    // W[:, m:m+nb] = H V[:, m:m+nb], then extend the projected matrix.
+    void append_block(int nb) {
+        double2* Vn = ws_.V.p + static_cast<size_t>(m_) * N_;
+        double2* Wn = ws_.W.p + static_cast<size_t>(m_) * N_;
+        double ms = 0.0;
+        apply_operator(c_, prec_, mf_, Vn, Wn, nb, &ms);
+        prod_ms_ += ms;
+        rep_->products += nb;
+        const int mn = m_ + nb;
+        // P = V[:, 0:mn]^H W_new  (mn x nb)
+        XATU_BLAS(rocblas_zgemm(c_.blas, rocblas_operation_conjugate_transpose, rocblas_operation_none, mn, nb,
+                                static_cast<int>(N_), &kOne, bz(ws_.V.p), static_cast<int>(N_), bz(Wn),
+                                static_cast<int>(N_), &kZero, bz(ws_.P.p), mn));
+        std::vector<cd> P(static_cast<size_t>(mn) * nb);
+        XATU_HIP(hipMemcpyAsync(P.data(), ws_.P.p, P.size() * sizeof(cd), hipMemcpyDeviceToHost, c_.stream));
+        XATU_HIP(hipStreamSynchronize(c_.stream));
+        // rows of the existing basis
+        for (int j = 0; j < nb; ++j) {
+            for (int i = 0; i < m_; ++i) {
+                const cd v = P[i + static_cast<size_t>(j) * mn];
+                T_[i + static_cast<size_t>(m_ + j) * mmax_] = v;
+                T_[(m_ + j) + static_cast<size_t>(i) * mmax_] = std::conj(v);
+            }
+        }
+        // new diagonal block, symmetrised: (P[m+i, j] + conj(P[m+j, i])) / 2
+        for (int j = 0; j < nb; ++j)
+            for (int i = 0; i <= j; ++i) {
+                const cd pij = P[(m_ + i) + static_cast<size_t>(j) * mn];
+                const cd pji = P[(m_ + j) + static_cast<size_t>(i) * mn];
+                cd avg = 0.5 * (pij + std::conj(pji));
+                if (i == j) avg = cd(avg.real(), 0.0);
+                T_[(m_ + i) + static_cast<size_t>(m_ + j) * mmax_] = avg;
+                T_[(m_ + j) + static_cast<size_t>(m_ + i) * mmax_] = std::conj(avg);
+            }
+        m_ = mn;
+    }
+
+    void rayleigh_ritz() {
+        const double t0 = now_ms();
+        std::vector<double> w(m_);
+        std::vector<cd> Z(static_cast<size_t>(m_) * m_);
+        xatu_linalg::heev(m_, T_.data(), mmax_, w.data(), Z.data());
+        theta_.assign(w.begin(), w.begin() + b_);
+        Yb_.assign(Z.begin(), Z.begin() + static_cast<size_t>(m_) * b_);
+        rr_ms_ += now_ms() - t0;
+    }
+
+    void ritz_and_residuals() {
+        XATU_HIP(hipMemcpyAsync(ws_.Y.p, Yb_.data(), Yb_.size() * sizeof(cd), hipMemcpyHostToDevice, c_.stream));
+        XATU_HIP(hipMemcpyAsync(ws_.theta.p, theta_.data(), b_ * sizeof(double), hipMemcpyHostToDevice, c_.stream));
+        const int n = static_cast<int>(N_);
+        XATU_BLAS(rocblas_zgemm(c_.blas, rocblas_operation_none, rocblas_operation_none, n, b_, m_, &kOne,
+                                bz(ws_.V.p), n, bz(ws_.Y.p), m_, &kZero, bz(ws_.X.p), n));
+        XATU_BLAS(rocblas_zgemm(c_.blas, rocblas_operation_none, rocblas_operation_none, n, b_, m_, &kOne,
+                                bz(ws_.W.p), n, bz(ws_.Y.p), m_, &kZero, bz(ws_.AX.p), n));
+        int nblk = 0;
+        launch_residual(ws_.X.p, ws_.AX.p, ws_.theta.p, N_, b_, ws_.R.p, ws_.partial.p, &nblk, c_.stream);
+        std::vector<double> part(static_cast<size_t>(nblk) * b_);
+        XATU_HIP(hipMemcpyAsync(part.data(), ws_.partial.p, part.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                c_.stream));
+        XATU_HIP(hipStreamSynchronize(c_.stream));
+        res_.assign(b_, 0.0);
+        for (int j = 0; j < b_; ++j) {
+            double s = 0.0;
+            for (int i = 0; i < nblk; ++i) s += part[i + static_cast<size_t>(j) * nblk];
+            res_[j] = std::sqrt(s);
+        }
+    }
+
+    void restart() {
+        const size_t bytes = static_cast<size_t>(N_) * b_ * sizeof(double2);
+        XATU_HIP(hipMemcpyAsync(ws_.V.p, ws_.X.p, bytes, hipMemcpyDeviceToDevice, c_.stream));
+        XATU_HIP(hipMemcpyAsync(ws_.W.p, ws_.AX.p, bytes, hipMemcpyDeviceToDevice, c_.stream));
+        std::fill(T_.begin(), T_.end(), cd(0.0, 0.0));
+        for (int i = 0; i < b_; ++i) T_[i + static_cast<size_t>(i) * mmax_] = cd(theta_[i], 0.0);
+        m_ = b_;
+        rep_->restarts++;
+    }
+
+    // Orthogonalise the block Tb (N x nb) against V[:, 0:m] (two passes).
+    void project_out(double2* Tb, int nb) {
+        const int n = static_cast<int>(N_);
+        for (int pass = 0; pass < 2; ++pass) {
+            XATU_BLAS(rocblas_zgemm(c_.blas, rocblas_operation_conjugate_transpose, rocblas_operation_none, m_, nb,
+                                    n, &kOne, bz(ws_.V.p), n, bz(Tb), n, &kZero, bz(ws_.P.p), m_));
+            XATU_BLAS(rocblas_zgemm(c_.blas, rocblas_operation_none, rocblas_operation_none, n, nb, m_, &kMinusOne,
+                                    bz(ws_.V.p), n, bz(ws_.P.p), m_, &kOne, bz(Tb), n));
+        }
+    }
+
+    // Orthonormalise the block Tb in place via its Gram matrix; returns the kept count
+    // and writes the orthonormal vectors to V[:, m:m+kept].
+    int orthonormalise(double2* Tb, int nb) {
+        const int n = static_cast<int>(N_);
+        std::vector<cd> G(static_cast<size_t>(nb) * nb);
+        XATU_BLAS(rocblas_zgemm(c_.blas, rocblas_operation_conjugate_transpose, rocblas_operation_none, nb, nb, n,
+                                &kOne, bz(Tb), n, bz(Tb), n, &kZero, bz(ws_.P.p), nb));
+        XATU_HIP(hipMemcpyAsync(G.data(), ws_.P.p, G.size() * sizeof(cd), hipMemcpyDeviceToHost, c_.stream));
+        XATU_HIP(hipStreamSynchronize(c_.stream));
+        std::vector<double> lam(nb);
+        std::vector<cd> U(static_cast<size_t>(nb) * nb);
+        xatu_linalg::heev(nb, G.data(), nb, lam.data(), U.data());
+        const double lmax = std::max(lam[nb - 1], 0.0);
+        std::vector<cd> S;
+        int kept = 0;
+        for (int j = nb - 1; j >= 0; --j) {
+            if (lam[j] > 1e-20 && lam[j] > 1e-14 * lmax) {
+                const double sc = 1.0 / std::sqrt(lam[j]);
+                for (int i = 0; i < nb; ++i) S.push_back(U[i + static_cast<size_t>(j) * nb] * sc);
+                ++kept;
+            }
+        }
+        if (kept == 0) return 0;
+        XATU_HIP(hipMemcpyAsync(ws_.P.p, S.data(), S.size() * sizeof(cd), hipMemcpyHostToDevice, c_.stream));
+        double2* dst = ws_.V.p + static_cast<size_t>(m_) * N_;
+        XATU_BLAS(rocblas_zgemm(c_.blas, rocblas_operation_none, rocblas_operation_none, n, kept, nb, &kOne, bz(Tb),
+                                n, bz(ws_.P.p), nb, &kZero, bz(dst), n));
+        return kept;
+    }
+
+    bool add_corrections() {
+        const int nb = std::min(b_, mmax_ - m_);
+        if (nb <= 0) return false;
+        launch_precondition(ws_.R.p, ws_.theta.p, c_.d_diag.p, N_, nb, ws_.Tb.p, c_.stream);
+        project_out(ws_.Tb.p, nb);
+        int kept = orthonormalise(ws_.Tb.p, nb);
+        if (kept == 0) return false;
+        // second pass on the orthonormal block for numerical safety
+        double2* Vn = ws_.V.p + static_cast<size_t>(m_) * N_;
+        XATU_HIP(hipMemcpyAsync(ws_.Tb.p, Vn, static_cast<size_t>(N_) * kept * sizeof(double2),
+                                hipMemcpyDeviceToDevice, c_.stream));
+        project_out(ws_.Tb.p, kept);
+        kept = orthonormalise(ws_.Tb.p, kept);
+        if (kept == 0) return false;
+        append_block(kept);
+        return true;
+    }
+
+    Context& c_;
+    int prec_, mf_, nev_, b_, mmax_;
+    xatu_solve_report* rep_;
+    int64_t N_ = 0;
+    int m_ = 0;
+    Workspace ws_;
+    std::vector<cd> T_;
+    std::vector<double> theta_, res_;
+    std::vector<cd> Yb_;
+    double prod_ms_ = 0.0, rr_ms_ = 0.0, last_worst_ = 0.0;
+};
+
+}  // namespace
+
+void solve_dense_diag(Context& c, int nstates, double* eigval_out, double* eigvec_out) {
+    if (!c.built64) throw Error(XATU_ERR_LOGIC, "diagonalizeRaw(): BSE Hamiltonian is not initialized.");
+    const int n = static_cast<int>(c.N);
+    DevBuf<double2> A;
+    A.alloc(static_cast<size_t>(n) * n);
+    XATU_HIP(hipMemcpyAsync(A.p, c.d_H64.p, A.bytes(), hipMemcpyDeviceToDevice, c.stream));
+    DevBuf<double> D, E;
+    DevBuf<int> info;
+    D.alloc(n);
+    E.alloc(n);
+    info.alloc(1);
+    XATU_BLAS(rocsolver_zheevd(c.blas, rocblas_evect_original, rocblas_fill_lower, n, bz(A.p), n, D.p, E.p,
+                               info.p));
+    int hinfo = 0;
+    XATU_HIP(hipMemcpyAsync(&hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    XATU_HIP(hipStreamSynchronize(c.stream));
+    if (hinfo != 0) throw Error(XATU_ERR_NOT_CONVERGED, "zheevd failed to converge");
+    const int ns = (nstates <= 0 || nstates > n) ? n : nstates;
+    XATU_HIP(hipMemcpy(eigval_out, D.p, ns * sizeof(double), hipMemcpyDeviceToHost));
+    if (eigvec_out) {
+        XATU_HIP(hipMemcpy(eigvec_out, A.p, static_cast<size_t>(n) * ns * sizeof(double2), hipMemcpyDeviceToHost));
+    }
+    c.last_nstates = ns;
+    c.d_last_vec.alloc(static_cast<size_t>(n) * ns);
+    XATU_HIP(hipMemcpyAsync(c.d_last_vec.p, A.p, static_cast<size_t>(n) * ns * sizeof(double2),
+                            hipMemcpyDeviceToDevice, c.stream));
+    c.last_val.assign(eigval_out, eigval_out + ns);
+    XATU_HIP(hipStreamSynchronize(c.stream));
+}
+
+void solve_davidson(Context& c, const xatu_solve_options& opt, double* eigval_out, double* eigvec_out,
+                    xatu_solve_report* rep) {
+    const int64_t N = c.N;
+    const int nev = opt.nstates;
+    if (nev <= 0 || nev > N) throw Error(XATU_ERR_INVALID_ARGUMENT, "nstates must be in [1, N]");
+    int b = opt.block > 0 ? opt.block : std::max(2 * nev, 8);
+    b = static_cast<int>(std::min<int64_t>(b, N));
+    if (b < nev) throw Error(XATU_ERR_INVALID_ARGUMENT, "Davidson block must be >= nstates");
+    int mmax = opt.max_basis > 0 ? opt.max_basis : 4 * b;
+    mmax = static_cast<int>(std::min<int64_t>(mmax, N));
+    mmax = std::max(mmax, std::min<int>(2 * b, static_cast<int>(N)));
+    if (mmax < b + 1 && N > b) mmax = b + 1;
+    const int maxit = opt.max_iter > 0 ? opt.max_iter : 400;
+    const double tol64 = opt.tol > 0 ? opt.tol : 1e-6;
+
+    const double t0 = now_ms();
+    Davidson dav(c, opt.precision, opt.matrix_free, nev, b, mmax, rep);
+    dav.init_unit();
+    bool ok;
+    if (opt.precision == XATU_FP32) {
+        // inner FP32 phase to a loose tolerance, then FP64 refinement
+        const double tol32 = std::max(tol64, 1e-4);
+        dav.run(tol32, maxit);
+        const int it32 = rep->iterations;
+        dav.set_precision(XATU_FP64, opt.matrix_free || !c.built64);
+        dav.init_from_X();
+        ok = dav.run(tol64, maxit);
+        rep->refine_iterations = rep->iterations - it32;
+    } else {
+        ok = dav.run(tol64, maxit);
+    }
+    rep->converged = ok ? 1 : 0;
+    rep->max_residual = dav.worst();
+    rep->ms_products = dav.products_ms();
+    rep->ms_rayleigh_ritz = dav.rr_ms();
+
+    const std::vector<double>& th = dav.theta();
+    std::copy(th.begin(), th.begin() + nev, eigval_out);
+    c.last_nstates = nev;
+    c.last_val.assign(th.begin(), th.begin() + nev);
+    c.d_last_vec.alloc(static_cast<size_t>(N) * nev);
+    XATU_HIP(hipMemcpyAsync(c.d_last_vec.p, dav.ritz_vectors(), static_cast<size_t>(N) * nev * sizeof(double2),
+                            hipMemcpyDeviceToDevice, c.stream));
+    if (eigvec_out) {
+        XATU_HIP(hipMemcpyAsync(eigvec_out, dav.ritz_vectors(), static_cast<size_t>(N) * nev * sizeof(double2),
+                                hipMemcpyDeviceToHost, c.stream));
+    }
+    XATU_HIP(hipStreamSynchronize(c.stream));
+    rep->ms_total = now_ms() - t0;
+}
+
+}  // namespace xatu_dev

@@ -1,0 +1,95 @@
+// Element-wise kernels used by the Davidson solver.
+#include "context.hpp"
+#include "cplx.hpp"
+
+namespace xatu_dev {
+
+namespace {
+
+__global__ void conv64to32_kernel(const double2* __restrict__ x, float2* __restrict__ y, size_t n) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = make_float2(static_cast<float>(x[i].x), static_cast<float>(x[i].y));
+}
+
+__global__ void conv32to64_kernel(const float2* __restrict__ x, double2* __restrict__ y, size_t n) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = make_double2(static_cast<double>(x[i].x), static_cast<double>(x[i].y));
+}
+
+// R[:, j] = AX[:, j] - theta_j X[:, j]; norms2[j] += |R[:, j]|^2 via block partials.
+// grid: (ceil(N/256), nvec)
+__global__ void residual_kernel(const double2* __restrict__ X, const double2* __restrict__ AX,
+                                const double* __restrict__ theta, int64_t N, double2* __restrict__ R,
+                                double* __restrict__ partial) {
+    __shared__ double s[256];
+    const int j = blockIdx.y;
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    double acc = 0.0;
+    if (i < N) {
+        const double2 x = X[i + j * N];
+        const double2 ax = AX[i + j * N];
+        const double th = theta[j];
+        const double2 r = make_double2(ax.x - th * x.x, ax.y - th * x.y);
+        R[i + j * N] = r;
+        acc = r.x * r.x + r.y * r.y;
+    }
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st) s[threadIdx.x] += s[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[static_cast<size_t>(j) * gridDim.x + blockIdx.x] = s[0];
+}
+
+// Diagonal (Davidson) preconditioner: T[:, j] = R[:, j] / (theta_j - d), with the
+// denominator kept away from zero.
+__global__ void precondition_kernel(const double2* __restrict__ R, const double* __restrict__ theta,
+                                    const double* __restrict__ diag, int64_t N, double2* __restrict__ T) {
+    const int j = blockIdx.y;
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    double den = theta[j] - diag[i];
+    if (fabs(den) < 1e-3) den = (den >= 0.0) ? 1e-3 : -1e-3;
+    const double2 r = R[i + j * N];
+    T[i + j * N] = make_double2(r.x / den, r.y / den);
+}
+
+// V[:, j] = e_{idx[j]}
+__global__ void unit_vectors_kernel(const int* __restrict__ idx, int64_t N, int nvec, double2* __restrict__ V) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < nvec) V[idx[j] + static_cast<int64_t>(j) * N] = make_double2(1.0, 0.0);
+}
+
+}  // namespace
+
+void launch_convert_64_to_32(const double2* x, float2* y, size_t n, hipStream_t s) {
+    hipLaunchKernelGGL(conv64to32_kernel, dim3((n + 255) / 256), dim3(256), 0, s, x, y, n);
+    XATU_HIP(hipGetLastError());
+}
+
+void launch_convert_32_to_64(const float2* x, double2* y, size_t n, hipStream_t s) {
+    hipLaunchKernelGGL(conv32to64_kernel, dim3((n + 255) / 256), dim3(256), 0, s, x, y, n);
+    XATU_HIP(hipGetLastError());
+}
+
+void launch_residual(const double2* X, const double2* AX, const double* theta, int64_t N, int nvec, double2* R,
+                     double* partial, int* nblocks, hipStream_t s) {
+    const int nbk = static_cast<int>((N + 255) / 256);
+    *nblocks = nbk;
+    hipLaunchKernelGGL(residual_kernel, dim3(nbk, nvec), dim3(256), 0, s, X, AX, theta, N, R, partial);
+    XATU_HIP(hipGetLastError());
+}
+
+void launch_precondition(const double2* R, const double* theta, const double* diag, int64_t N, int nvec, double2* T,
+                         hipStream_t s) {
+    hipLaunchKernelGGL(precondition_kernel, dim3((N + 255) / 256, nvec), dim3(256), 0, s, R, theta, diag, N, T);
+    XATU_HIP(hipGetLastError());
+}
+
+void launch_unit_vectors(const int* idx, int64_t N, int nvec, double2* V, hipStream_t s) {
+    hipLaunchKernelGGL(unit_vectors_kernel, dim3((nvec + 63) / 64), dim3(64), 0, s, idx, N, nvec, V);
+    XATU_HIP(hipGetLastError());
+}
+
+}  // namespace xatu_dev

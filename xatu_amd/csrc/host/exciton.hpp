@@ -1,0 +1,153 @@
+// ExcitonTB / ResultTB facade: the reference's public API
+// (include/xatu/ExcitonTB.hpp:24-157, include/xatu/Exciton.hpp:40-105,
+// include/xatu/Result.hpp:11-66) backed by the C-ABI of include/xatu_bse.h.
+// The single-particle problem (bands and phase-fixed eigenvectors per k) runs on
+// the host; the lattice Fourier sums, the BSE assembly and the diagonalisation run
+// on the GPU with the matrix resident in HBM.
+#pragma once
+
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../../include/xatu_bse.h"
+#include "system.hpp"
+
+namespace xatu {
+
+class ResultTB;
+
+class ExcitonTB {
+public:
+    // Constructors (ExcitonTB.hpp:72-88, src/ExcitonTB.cpp:91-210).
+    ExcitonTB(const SystemConfiguration& config, int ncell = 20, int nbands = 1, int nrmbands = 0,
+              const std::vector<double>& parameters = {1, 5, 1}, const std::vector<double>& Q = {0., 0., 0.});
+    ExcitonTB(const SystemConfiguration& config, int ncell, const std::vector<long long>& bands,
+              const std::vector<double>& parameters = {1, 5, 1}, const std::vector<double>& Q = {0., 0., 0.});
+    ExcitonTB(const SystemConfiguration& config, const ExcitonConfiguration& excitonConfig);
+    ExcitonTB(std::shared_ptr<SystemTB> sys, int ncell = 20, int nbands = 1, int nrmbands = 0,
+              const std::vector<double>& parameters = {1, 5, 1}, const std::vector<double>& Q = {0., 0., 0.});
+    ExcitonTB(std::shared_ptr<SystemTB> sys, int ncell, const std::vector<long long>& bands,
+              const std::vector<double>& parameters = {1, 5, 1}, const std::vector<double>& Q = {0., 0., 0.});
+    ~ExcitonTB();
+    ExcitonTB(const ExcitonTB&) = delete;
+    ExcitonTB& operator=(const ExcitonTB&) = delete;
+
+    // Setters (ExcitonTB.hpp:93-98, Exciton.hpp:97-103)
+    void setParameters(const std::vector<double>& parameters);
+    void setParameters(double eps_m, double eps_s, double r0);
+    void setGauge(const std::string& gauge);
+    void setMode(const std::string& mode);
+    void setReciprocalVectors(int n);
+    void setRegularization(double regularization);
+    void setUnitCells(int ncell);
+    void setBands(const std::vector<long long>& bands);
+    void setBands(int nbands, int nrmbands);
+    void setQ(const std::vector<double>& Q);
+    void setCutoff(double cutoff);
+    void setScissor(double shift);
+    void setExchange(bool exchange);
+    void setPotential(const std::string& potential);
+    void setExchangePotential(const std::string& potential);
+    // Build options of this implementation (no counterpart in the reference):
+    // "fp64" (default) or "fp32" arithmetic for the assembly and the products;
+    // matrix-free products; Davidson block size; HIP device.
+    void setPrecision(const std::string& precision);
+    void setMatrixFree(bool matrixFree);
+    void setDavidsonBlock(int block);
+    void setDavidsonTolerance(double tol);
+    void setDevice(int device);
+
+    // Pipeline (ExcitonTB.hpp:140-143)
+    void brillouinZoneMesh(int ncell);
+    void initializeHamiltonian();
+    void BShamiltonian();
+    void BShamiltonian(const std::vector<long long>& basis);  // nbasis x 3 row-major {v, c, k}
+    std::unique_ptr<ResultTB> diagonalize(const std::string& method = "diag", int nstates = 8);
+
+    // Read-only state (Exciton.hpp:52-86)
+    std::shared_ptr<SystemTB> system;
+    int ncell = 0, nbands = 0, nrmbands = 0;
+    long long totalCells = 0;
+    long long excitonbasisdim = 0;
+    double eps_m = 1, eps_s = 5, r0 = 1, regularization = 0, cutoff = 0, scissor = 0;
+    bool exchange = false;
+    std::string gauge = "lattice", mode = "realspace", potential = "keldysh", exchangePotential = "keldysh";
+    int nReciprocalVectors = 1;
+    std::vector<long long> bands, valenceBands, conductionBands;
+    std::vector<long long> bandList;
+    std::vector<double> Q{0., 0., 0.};
+    std::vector<long long> basisStates;        // N x 3 row-major {v, c, k}
+    std::vector<double> eigvalKStack, eigvalKQStack;    // nb x nk
+    std::vector<cdouble> eigvecKStack, eigvecKQStack;   // basisdim x nb x nk
+    std::map<int, int> bandToIndex;
+    std::string precision = "fp64";
+    bool matrixFree = false;
+    int davidsonBlock = 0;
+    double davidsonTol = 0.0;
+    int device = 0;
+
+    // The resident BSE matrix copied to the host as a full N x N column-major matrix.
+    std::vector<cdouble> HBS() const;
+    std::vector<cdouble> HBS32() const;  // FP32 matrix (widened), when built in fp32
+    bool hasHBS() const { return built_; }
+    // Lattice Fourier stacks (ftMotifStack natoms x natoms x nmesh, ftMotifQ natoms x natoms).
+    std::vector<cdouble> ftMotifStack() const;
+    std::vector<cdouble> ftMotifQ() const;
+    std::vector<int> cellCoefficients() const;  // nR x ndim
+    // y = H x with the resident matrix (or matrix-free), x: N x nvec column-major.
+    std::vector<cdouble> applyBSE(const std::vector<cdouble>& x, int nvec, const std::string& precision,
+                                  bool matrixFree) const;
+    // Hellmann-Feynman gradients of the given states with respect to "r0", "eps_m", "eps_s".
+    std::vector<double> hfGradients(const std::vector<cdouble>& eigvec, int nstates,
+                                    const std::vector<std::string>& params) const;
+    xatu_timings timings() const;
+    xatu_solve_report lastSolveReport() const { return lastReport_; }
+    xatu_ctx* context() const { return ctx_; }
+
+    // Basis helpers (Exciton.tpp:128-199)
+    std::vector<long long> createBasis(const std::vector<long long>& conduction,
+                                       const std::vector<long long>& valence) const;
+    // Fix the phase of every column so that the sum of its coefficients is real >= 0
+    // (ExcitonTB.cpp:673-683).
+    static void fixGlobalPhase(std::vector<cdouble>& coefs, int rows, int cols);
+
+private:
+    void initializeExcitonAttributes(int ncell, const std::vector<long long>& bands,
+                                     const std::vector<double>& parameters, const std::vector<double>& Q);
+    void initializeExcitonAttributes(const ExcitonConfiguration& cfg);
+    void splitBands(const std::vector<long long>& bands);
+    void setBandsFromCount(int nbands, int nrmbands);
+    void initializeResultsH0();
+    void ensureContext();
+    void check(int rc) const;
+
+    xatu_ctx* ctx_ = nullptr;
+    bool built_ = false;
+    bool built32_ = false;
+    bool initialized_ = false;
+    xatu_solve_report lastReport_{};
+};
+
+class ResultTB {
+public:
+    ResultTB(ExcitonTB* exciton, std::vector<double> eigval, std::vector<cdouble> eigvec, int ncols);
+    ExcitonTB* exciton;
+    std::vector<double> eigval;   // ascending
+    std::vector<cdouble> eigvec;  // N x ncols column-major
+    int ncols = 0;
+    long long nrows = 0;
+
+    // Observables (Result.tpp:16-60)
+    double kineticEnergy(int stateindex) const;
+    double potentialEnergy(int stateindex) const;
+    // Output formats (Result.tpp:404-448)
+    void writeEigenvalues(FILE* textfile, int n = 0) const;
+    void writeStates(FILE* textfile, int n = 0) const;
+};
+
+// Degenerate-level grouping of sorted energies (src/utils.cpp:121-152).
+std::vector<std::vector<double>> detectDegeneracies(const std::vector<double>& eigval, int n, int precision);
+
+}  // namespace xatu

@@ -1,0 +1,318 @@
+// Python bindings of the C++ facade (used by the test-suite and bench). Names and
+// argument meanings follow the reference C++ API; matrices come back as numpy arrays
+// with the reference's (Armadillo, column-major) shapes.
+#include <pybind11/complex.h>
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include <cstdio>
+#include <cstring>
+
+#include "exciton.hpp"
+
+namespace py = pybind11;
+using namespace xatu;
+
+namespace {
+
+py::array_t<double> rows(const std::vector<double>& v, int ncol) {
+    const py::ssize_t n = ncol ? static_cast<py::ssize_t>(v.size() / ncol) : 0;
+    py::array_t<double> a({n, static_cast<py::ssize_t>(ncol)});
+    if (!v.empty()) std::memcpy(a.mutable_data(), v.data(), v.size() * sizeof(double));
+    return a;
+}
+
+py::array_t<long long> irows(const std::vector<long long>& v, int ncol) {
+    const py::ssize_t n = static_cast<py::ssize_t>(v.size() / ncol);
+    py::array_t<long long> a({n, static_cast<py::ssize_t>(ncol)});
+    if (!v.empty()) std::memcpy(a.mutable_data(), v.data(), v.size() * sizeof(long long));
+    return a;
+}
+
+template <typename T>
+py::array_t<T> fortran(const std::vector<T>& v, std::vector<py::ssize_t> shape) {
+    py::array_t<T, py::array::f_style> a(shape);
+    if (!v.empty()) std::memcpy(a.mutable_data(), v.data(), v.size() * sizeof(T));
+    return a;
+}
+
+std::vector<cdouble> to_vec(py::array_t<cdouble, py::array::f_style | py::array::forcecast> a) {
+    return std::vector<cdouble>(a.data(), a.data() + a.size());
+}
+
+FILE* open_or_throw(const std::string& name) {
+    FILE* f = std::fopen(name.c_str(), "w");
+    if (!f) throw std::runtime_error("cannot open " + name);
+    return f;
+}
+
+}  // namespace
+
+PYBIND11_MODULE(_xatu, m) {
+    m.doc() = "MI355X BSE kernel builder and exciton solver: ExcitonTB facade over the C-ABI";
+
+    py::class_<SystemConfiguration>(m, "SystemConfiguration")
+        .def(py::init<const std::string&>(), py::arg("filename"))
+        .def_readonly("foundArguments", &SystemConfiguration::foundArguments)
+        .def_property_readonly("ndim", [](const SystemConfiguration& c) { return c.systemInfo.ndim; })
+        .def_property_readonly("filling", [](const SystemConfiguration& c) { return c.systemInfo.filling; })
+        .def_property_readonly("norbitals", [](const SystemConfiguration& c) { return c.systemInfo.norbitals; })
+        .def_property_readonly("bravaisLattice",
+                               [](const SystemConfiguration& c) { return rows(c.systemInfo.bravaisLattice, 3); })
+        .def_property_readonly("motif", [](const SystemConfiguration& c) { return rows(c.systemInfo.motif, 4); })
+        .def_property_readonly("bravaisVectors",
+                               [](const SystemConfiguration& c) { return rows(c.systemInfo.bravaisVectors, 3); })
+        .def_property_readonly("hamiltonian",
+                               [](const SystemConfiguration& c) {
+                                   const py::ssize_t d = c.systemInfo.matrixDim;
+                                   return fortran(c.systemInfo.hamiltonian, {d, d, c.systemInfo.nHamiltonian});
+                               })
+        .def_property_readonly("overlap", [](const SystemConfiguration& c) {
+            const py::ssize_t d = c.systemInfo.matrixDim;
+            return fortran(c.systemInfo.overlap, {d, d, c.systemInfo.nOverlap});
+        });
+
+    py::class_<ExcitonConfiguration>(m, "ExcitonConfiguration")
+        .def(py::init<const std::string&>(), py::arg("filename"))
+        .def_property_readonly("label", [](const ExcitonConfiguration& c) { return c.excitonInfo.label; })
+        .def_property_readonly("ncell", [](const ExcitonConfiguration& c) { return c.excitonInfo.ncell; })
+        .def_property_readonly("nbands", [](const ExcitonConfiguration& c) { return c.excitonInfo.nbands; })
+        .def_property_readonly("submeshFactor", [](const ExcitonConfiguration& c) { return c.excitonInfo.submeshFactor; })
+        .def_property_readonly("bands", [](const ExcitonConfiguration& c) { return c.excitonInfo.bands; })
+        .def_property_readonly("Q", [](const ExcitonConfiguration& c) { return c.excitonInfo.Q; })
+        .def_property_readonly("shift", [](const ExcitonConfiguration& c) { return c.excitonInfo.shift; })
+        .def_property_readonly("cutoff", [](const ExcitonConfiguration& c) { return c.excitonInfo.cutoff; })
+        .def_property_readonly("eps", [](const ExcitonConfiguration& c) { return c.excitonInfo.eps; })
+        .def_property_readonly("mode", [](const ExcitonConfiguration& c) { return c.excitonInfo.mode; })
+        .def_property_readonly("exchange", [](const ExcitonConfiguration& c) { return c.excitonInfo.exchange; })
+        .def_property_readonly("scissor", [](const ExcitonConfiguration& c) { return c.excitonInfo.scissor; })
+        .def_property_readonly("nReciprocalVectors",
+                               [](const ExcitonConfiguration& c) { return c.excitonInfo.nReciprocalVectors; })
+        .def_property_readonly("potential", [](const ExcitonConfiguration& c) { return c.excitonInfo.potential; })
+        .def_property_readonly("exchangePotential",
+                               [](const ExcitonConfiguration& c) { return c.excitonInfo.exchangePotential; })
+        .def_property_readonly("regularization",
+                               [](const ExcitonConfiguration& c) { return c.excitonInfo.regularization; });
+
+    py::class_<SystemTB, std::shared_ptr<SystemTB>>(m, "SystemTB")
+        .def(py::init<const SystemConfiguration&>())
+        .def_readonly("ndim", &SystemTB::ndim)
+        .def_readonly("natoms", &SystemTB::natoms)
+        .def_readonly("ncells", &SystemTB::ncells)
+        .def_readonly("nk", &SystemTB::nk)
+        .def_readonly("factor", &SystemTB::factor)
+        .def_readonly("a", &SystemTB::a)
+        .def_readonly("c", &SystemTB::c)
+        .def_readonly("unitCellArea", &SystemTB::unitCellArea)
+        .def_readonly("basisdim", &SystemTB::basisdim)
+        .def_readonly("filling", &SystemTB::filling)
+        .def_readonly("fermiLevel", &SystemTB::fermiLevel)
+        .def_readonly("isTriangular", &SystemTB::isTriangular)
+        .def_readonly("meshPoints", &SystemTB::meshPoints)
+        .def_property_readonly("orbitals", [](const SystemTB& s) { return s.orbitals; })
+        .def_property_readonly("bravaisLattice", [](const SystemTB& s) { return rows(s.bravaisLattice, 3); })
+        .def_property_readonly("reciprocalLattice", [](const SystemTB& s) { return rows(s.reciprocalLattice, 3); })
+        .def_property_readonly("motif", [](const SystemTB& s) { return rows(s.motif, 4); })
+        .def_property_readonly("unitCellList", [](const SystemTB& s) { return rows(s.unitCellList, 3); })
+        .def_property_readonly("kpoints", [](const SystemTB& s) { return rows(s.kpoints, 3); })
+        .def_property_readonly("meshBZ", [](const SystemTB& s) { return rows(s.meshBZ, 3); })
+        .def_property_readonly("kpointOffsets",
+                               [](const SystemTB& s) {
+                                   std::vector<long long> v(s.kpointOffsets.begin(), s.kpointOffsets.end());
+                                   return irows(v, s.ndim);
+                               })
+        .def("orbitalsPerAtom", &SystemTB::orbitalsPerAtom)
+        .def("brillouinZoneMesh", &SystemTB::brillouinZoneMesh)
+        .def("reducedBrillouinZoneMesh", &SystemTB::reducedBrillouinZoneMesh)
+        .def("shiftBZ", &SystemTB::shiftBZ)
+        .def("setFilling", &SystemTB::setFilling)
+        .def("setAU", &SystemTB::setAU)
+        .def("setTriangular", &SystemTB::setTriangular)
+        .def("calculateInverseReciprocalMatrix", &SystemTB::calculateInverseReciprocalMatrix)
+        .def("findEquivalentPointBZ",
+             [](const SystemTB& s, std::vector<double> k, int ncell) { return s.findEquivalentPointBZ(k.data(), ncell); })
+        .def("truncateSupercell", [](const SystemTB& s, int ncell, double radius) {
+            return rows(s.truncateSupercell(ncell, radius), 3);
+        })
+        .def("hamiltonian",
+             [](const SystemTB& s, std::vector<double> k) {
+                 return fortran(s.hamiltonian(k.data()), {s.basisdim, s.basisdim});
+             })
+        .def("solveBands", [](const SystemTB& s, std::vector<double> k) {
+            std::vector<double> w;
+            std::vector<cdouble> v;
+            s.solveBands(k.data(), w, v);
+            return py::make_tuple(py::array_t<double>(w.size(), w.data()), fortran(v, {s.basisdim, s.basisdim}));
+        });
+
+    py::class_<xatu_timings>(m, "Timings")
+        .def_readonly("ms_fourier", &xatu_timings::ms_fourier)
+        .def_readonly("ms_build", &xatu_timings::ms_build)
+        .def_readonly("ms_build_kernel", &xatu_timings::ms_build_kernel)
+        .def_readonly("ms_matvec_kernel", &xatu_timings::ms_matvec_kernel)
+        .def_readonly("n_matvec_kernel", &xatu_timings::n_matvec_kernel);
+
+    py::class_<xatu_solve_report>(m, "SolveReport")
+        .def_readonly("iterations", &xatu_solve_report::iterations)
+        .def_readonly("products", &xatu_solve_report::products)
+        .def_readonly("restarts", &xatu_solve_report::restarts)
+        .def_readonly("refine_iterations", &xatu_solve_report::refine_iterations)
+        .def_readonly("converged", &xatu_solve_report::converged)
+        .def_readonly("max_residual", &xatu_solve_report::max_residual)
+        .def_readonly("ms_total", &xatu_solve_report::ms_total)
+        .def_readonly("ms_products", &xatu_solve_report::ms_products)
+        .def_readonly("ms_rayleigh_ritz", &xatu_solve_report::ms_rayleigh_ritz);
+
+    py::class_<ResultTB>(m, "ResultTB")
+        .def_property_readonly("eigval", [](const ResultTB& r) { return py::array_t<double>(r.eigval.size(), r.eigval.data()); })
+        .def_property_readonly("eigvec", [](const ResultTB& r) { return fortran(r.eigvec, {r.nrows, r.ncols}); })
+        .def("kineticEnergy", &ResultTB::kineticEnergy)
+        .def("potentialEnergy", &ResultTB::potentialEnergy)
+        .def("writeEigenvalues",
+             [](const ResultTB& r, const std::string& name, int n) {
+                 FILE* f = open_or_throw(name);
+                 r.writeEigenvalues(f, n);
+                 std::fclose(f);
+             },
+             py::arg("filename"), py::arg("n") = 0)
+        .def("writeStates",
+             [](const ResultTB& r, const std::string& name, int n) {
+                 FILE* f = open_or_throw(name);
+                 r.writeStates(f, n);
+                 std::fclose(f);
+             },
+             py::arg("filename"), py::arg("n") = 0);
+
+    py::class_<ExcitonTB>(m, "ExcitonTB")
+        .def(py::init<const SystemConfiguration&, int, int, int, const std::vector<double>&, const std::vector<double>&>(),
+             py::arg("config"), py::arg("ncell") = 20, py::arg("nbands") = 1, py::arg("nrmbands") = 0,
+             py::arg("parameters") = std::vector<double>{1, 5, 1}, py::arg("Q") = std::vector<double>{0., 0., 0.})
+        .def(py::init<const SystemConfiguration&, int, const std::vector<long long>&, const std::vector<double>&,
+                      const std::vector<double>&>(),
+             py::arg("config"), py::arg("ncell"), py::arg("bands"), py::arg("parameters") = std::vector<double>{1, 5, 1},
+             py::arg("Q") = std::vector<double>{0., 0., 0.})
+        .def(py::init<const SystemConfiguration&, const ExcitonConfiguration&>())
+        .def(py::init<std::shared_ptr<SystemTB>, int, int, int, const std::vector<double>&, const std::vector<double>&>(),
+             py::arg("system"), py::arg("ncell") = 20, py::arg("nbands") = 1, py::arg("nrmbands") = 0,
+             py::arg("parameters") = std::vector<double>{1, 5, 1}, py::arg("Q") = std::vector<double>{0., 0., 0.})
+        .def("setParameters", py::overload_cast<const std::vector<double>&>(&ExcitonTB::setParameters))
+        .def("setGauge", &ExcitonTB::setGauge)
+        .def("setMode", &ExcitonTB::setMode)
+        .def("setReciprocalVectors", &ExcitonTB::setReciprocalVectors)
+        .def("setRegularization", &ExcitonTB::setRegularization)
+        .def("setUnitCells", &ExcitonTB::setUnitCells)
+        .def("setBands", py::overload_cast<const std::vector<long long>&>(&ExcitonTB::setBands))
+        .def("setQ", &ExcitonTB::setQ)
+        .def("setCutoff", &ExcitonTB::setCutoff)
+        .def("setScissor", &ExcitonTB::setScissor)
+        .def("setExchange", &ExcitonTB::setExchange)
+        .def("setPotential", &ExcitonTB::setPotential)
+        .def("setExchangePotential", &ExcitonTB::setExchangePotential)
+        .def("setPrecision", &ExcitonTB::setPrecision)
+        .def("setMatrixFree", &ExcitonTB::setMatrixFree)
+        .def("setDavidsonBlock", &ExcitonTB::setDavidsonBlock)
+        .def("setDavidsonTolerance", &ExcitonTB::setDavidsonTolerance)
+        .def("setDevice", &ExcitonTB::setDevice)
+        .def("brillouinZoneMesh", &ExcitonTB::brillouinZoneMesh)
+        .def("initializeHamiltonian", &ExcitonTB::initializeHamiltonian, py::call_guard<py::gil_scoped_release>())
+        .def("BShamiltonian", [](ExcitonTB& e) {
+            py::gil_scoped_release rel;
+            e.BShamiltonian();
+        })
+        .def("diagonalize", &ExcitonTB::diagonalize, py::arg("method") = "diag", py::arg("nstates") = 8,
+             py::call_guard<py::gil_scoped_release>())
+        .def_property_readonly("system", [](const ExcitonTB& e) { return e.system; })
+        .def_readonly("ncell", &ExcitonTB::ncell)
+        .def_readonly("nbands", &ExcitonTB::nbands)
+        .def_readonly("nrmbands", &ExcitonTB::nrmbands)
+        .def_readonly("totalCells", &ExcitonTB::totalCells)
+        .def_readonly("excitonbasisdim", &ExcitonTB::excitonbasisdim)
+        .def_readonly("eps_m", &ExcitonTB::eps_m)
+        .def_readonly("eps_s", &ExcitonTB::eps_s)
+        .def_readonly("r0", &ExcitonTB::r0)
+        .def_readonly("regularization", &ExcitonTB::regularization)
+        .def_readonly("cutoff", &ExcitonTB::cutoff)
+        .def_readonly("scissor", &ExcitonTB::scissor)
+        .def_readonly("exchange", &ExcitonTB::exchange)
+        .def_readonly("mode", &ExcitonTB::mode)
+        .def_readonly("gauge", &ExcitonTB::gauge)
+        .def_readonly("potential", &ExcitonTB::potential)
+        .def_readonly("exchangePotential", &ExcitonTB::exchangePotential)
+        .def_readonly("bands", &ExcitonTB::bands)
+        .def_readonly("valenceBands", &ExcitonTB::valenceBands)
+        .def_readonly("conductionBands", &ExcitonTB::conductionBands)
+        .def_readonly("bandList", &ExcitonTB::bandList)
+        .def_readonly("Q", &ExcitonTB::Q)
+        .def_readonly("bandToIndex", &ExcitonTB::bandToIndex)
+        .def_readonly("precision", &ExcitonTB::precision)
+        .def_property_readonly("basisStates", [](const ExcitonTB& e) { return irows(e.basisStates, 3); })
+        .def_property_readonly("eigvalKStack",
+                               [](const ExcitonTB& e) {
+                                   const py::ssize_t nb = e.bandList.size();
+                                   return fortran(e.eigvalKStack, {nb, e.system->nk});
+                               })
+        .def_property_readonly("eigvalKQStack",
+                               [](const ExcitonTB& e) {
+                                   const py::ssize_t nb = e.bandList.size();
+                                   return fortran(e.eigvalKQStack, {nb, e.system->nk});
+                               })
+        .def_property_readonly("eigvecKStack",
+                               [](const ExcitonTB& e) {
+                                   const py::ssize_t nb = e.bandList.size();
+                                   return fortran(e.eigvecKStack, {e.system->basisdim, nb, e.system->nk});
+                               })
+        .def_property_readonly("eigvecKQStack",
+                               [](const ExcitonTB& e) {
+                                   const py::ssize_t nb = e.bandList.size();
+                                   return fortran(e.eigvecKQStack, {e.system->basisdim, nb, e.system->nk});
+                               })
+        .def_property_readonly("HBS",
+                               [](const ExcitonTB& e) {
+                                   const py::ssize_t n = e.excitonbasisdim;
+                                   return fortran(e.HBS(), {n, n});
+                               })
+        .def_property_readonly("HBS32",
+                               [](const ExcitonTB& e) {
+                                   const py::ssize_t n = e.excitonbasisdim;
+                                   return fortran(e.HBS32(), {n, n});
+                               })
+        .def_property_readonly("ftMotifStack",
+                               [](const ExcitonTB& e) {
+                                   const py::ssize_t na = e.system->natoms;
+                                   std::vector<cdouble> v = e.ftMotifStack();
+                                   return fortran(v, {na, na, static_cast<py::ssize_t>(v.size() / (na * na))});
+                               })
+        .def_property_readonly("ftMotifQ",
+                               [](const ExcitonTB& e) {
+                                   const py::ssize_t na = e.system->natoms;
+                                   return fortran(e.ftMotifQ(), {na, na});
+                               })
+        .def("cellCoefficients",
+             [](const ExcitonTB& e) {
+                 std::vector<int> c = e.cellCoefficients();
+                 std::vector<long long> v(c.begin(), c.end());
+                 return irows(v, e.system->ndim);
+             })
+        .def("applyBSE",
+             [](const ExcitonTB& e, py::array_t<cdouble, py::array::f_style | py::array::forcecast> x,
+                const std::string& prec, bool mf) {
+                 const int nvec = x.ndim() == 1 ? 1 : static_cast<int>(x.shape(1));
+                 std::vector<cdouble> y = e.applyBSE(to_vec(x), nvec, prec, mf);
+                 return fortran(y, {static_cast<py::ssize_t>(e.excitonbasisdim), nvec});
+             },
+             py::arg("x"), py::arg("precision") = "fp64", py::arg("matrixFree") = false)
+        .def("hfGradients",
+             [](const ExcitonTB& e, py::object eigvec, int nstates, const std::vector<std::string>& params) {
+                 std::vector<cdouble> v;
+                 if (!eigvec.is_none()) v = to_vec(eigvec.cast<py::array_t<cdouble, py::array::f_style | py::array::forcecast>>());
+                 std::vector<double> g = e.hfGradients(v, nstates, params);
+                 return fortran(g, {nstates, static_cast<py::ssize_t>(params.size())});
+             },
+             py::arg("eigvec"), py::arg("nstates"), py::arg("params"))
+        .def("timings", &ExcitonTB::timings)
+        .def("lastSolveReport", &ExcitonTB::lastSolveReport)
+        .def("contextAddress", [](const ExcitonTB& e) { return reinterpret_cast<std::uintptr_t>(e.context()); });
+
+    m.def("detectDegeneracies", &detectDegeneracies, py::arg("eigval"), py::arg("n"), py::arg("precision") = 6);
+}
