@@ -400,13 +400,19 @@ def detect_degeneracies(eigval, n, precision=6):
     return pairs
 
 
-def array2hash(a):
-    """Test fingerprint (include/xatu/utils.hpp:61-78); rows/cols as Armadillo sees them."""
+def array2hash(a, aliased=False):
+    """Test fingerprint (include/xatu/utils.hpp:61-78); rows/cols as Armadillo sees them.
+
+    ``aliased=True`` reproduces a call on an Armadillo subview (e.g. ``mat.row(i)``):
+    there ``T copyArray(array)`` is another view of the same memory, so the scaled
+    values also feed the |.| and non-zero terms.
+    """
     a = np.asarray(a)
     if a.ndim == 1:
         a = a[:, None]
     i = np.arange(a.shape[0])[:, None]
     j = np.arange(a.shape[1])[None, :]
     c = a * (i * j + j + 1)
-    return float((np.abs(a).sum() + np.count_nonzero(a) + (np.real(c) * 1.5).sum() + (np.imag(c) * 1.5).sum()
-                  + (np.imag(c) * np.real(c)).sum()) / a.size)
+    base = c if aliased else a
+    return float((np.abs(base).sum() + np.count_nonzero(base) + (np.real(c) * 1.5).sum()
+                  + (np.imag(c) * 1.5).sum() + (np.imag(c) * np.real(c)).sum()) / a.size)

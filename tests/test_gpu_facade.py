@@ -1,0 +1,145 @@
+"""The reference's regression suite (test/tests.cpp) run end to end through the
+C++ ExcitonTB facade: host bands -> C-ABI -> HIP lattice sums, assembly and
+diagonalisation on the GPU."""
+import numpy as np
+import pytest
+
+import oracle as O
+import xatu_amd as X
+from cases import MODELS, oracle_case, rel_err
+
+pytestmark = pytest.mark.gpu
+
+HBN = str(MODELS / "hBN.model")
+HBN_SPIN = str(MODELS / "hBN_spinful.model")
+MOS2 = str(MODELS / "MoS2.model")
+
+
+def levels(eigval, n):
+    return X.detectDegeneracies(list(eigval), n, 6)
+
+
+def check_levels(got, expected, tol):
+    for (e, d), (ee, dd) in zip(got, expected):
+        assert abs(e - ee) < tol, (got, expected)
+        assert d == dd, (got, expected)
+
+
+def hbn(method, nstates=3):
+    ex = X.ExcitonTB(X.SystemConfiguration(HBN), 20, 1, 0, [1, 1, 10])
+    ex.brillouinZoneMesh(20)
+    ex.initializeHamiltonian()
+    ex.BShamiltonian()
+    return ex, ex.diagonalize(method, nstates)
+
+
+@pytest.mark.parametrize("method", ["diag", "davidson", "sparse"])
+def test_hbn_energies(method):
+    # test/tests.cpp:150-244
+    ex, res = hbn(method)
+    check_levels(levels(res.eigval, 3), [(5.335690, 2), (6.074062, 1)], 1e-4)
+
+
+def test_hbn_matrix_matches_oracle():
+    ex, _ = hbn("diag")
+    ref = oracle_case(HBN, 20, 1, 0, (1, 1, 10))
+    assert np.allclose(ex.eigvecKStack, ref.eigvec_k, atol=1e-12, rtol=0)
+    assert rel_err(ex.HBS, ref.bse()) < 1e-10
+    assert rel_err(ex.ftMotifStack, ref.ft) < 1e-12
+
+
+def test_hbn_kwf_hash():
+    # test/tests.cpp:281-321
+    ex, res = hbn("diag", 2)
+    v = res.eigvec
+    kp = ex.system.kpoints
+    dk = np.linalg.norm(kp[1] - kp[0])
+    kwf = np.zeros(len(kp), complex)
+    for n in range(2):
+        kwf += (np.abs(v[:, n]) ** 2).reshape(len(kp), 1).sum(1) / dk
+    assert abs(O.array2hash(kwf) - 1.086145105) < 1e-5
+
+
+def test_hbn_spinful():
+    # test/tests.cpp:401-431
+    ex = X.ExcitonTB(X.SystemConfiguration(HBN_SPIN), 20, 2, 0, [1, 1, 10])
+    ex.brillouinZoneMesh(20)
+    ex.initializeHamiltonian()
+    ex.BShamiltonian()
+    res = ex.diagonalize("diag", 12)
+    check_levels(levels(res.eigval, 12), [(5.335690, 8), (6.074062, 4)], 1e-4)
+
+
+def mos2(Q=(0, 0, 0), exchange=False, factor=1, shift=None, method="diag", nstates=4):
+    ex = X.ExcitonTB(X.SystemConfiguration(MOS2), 12, 2, 0, [1., 4., 13.55], list(Q))
+    ex.setExchange(exchange)
+    if factor > 1:
+        ex.system.reducedBrillouinZoneMesh(12, factor)
+        ex.system.shiftBZ(list(shift))
+    else:
+        ex.brillouinZoneMesh(12)
+    ex.initializeHamiltonian()
+    ex.BShamiltonian()
+    return ex, ex.diagonalize(method, nstates)
+
+
+def test_mos2_energies():
+    # test/tests.cpp:547-577
+    _, res = mos2()
+    check_levels(levels(res.eigval, 2), [(1.768783, 2)], 1e-4)
+
+
+@pytest.mark.parametrize("q,expected", [(-0.1, 1.810464), (-0.05, 1.780106), (0.0, 1.768783), (0.05, 1.780106),
+                                        (0.1, 1.810464)])
+def test_mos2_exciton_bands(q, expected):
+    # test/tests.cpp:660-702
+    _, res = mos2(Q=(0, q, 0))
+    check_levels(levels(res.eigval, 2)[:1], [(expected, 2)], 1e-5)
+
+
+def test_mos2_exchange():
+    # test/tests.cpp:704-737
+    _, res = mos2(Q=(0, 0.1, 0), exchange=True)
+    check_levels(levels(res.eigval, 4), [(1.810464, 2), (1.825740, 1), (1.855544, 1)], 1e-5)
+
+
+def test_mos2_reduced_bz():
+    # test/tests.cpp:739-774
+    _, res = mos2(factor=2, shift=(0.6628, -1.1480, 0))
+    check_levels(levels(res.eigval, 4), [(1.795378, 1), (1.809810, 1), (1.935880, 1), (1.950567, 1)], 1e-4)
+
+
+def test_mos2_kwf_hash():
+    # test/tests.cpp:579-619
+    ex, res = mos2()
+    v = res.eigvec
+    kp = ex.system.kpoints
+    dk = np.linalg.norm(kp[1] - kp[0])
+    kwf = np.zeros(len(kp), complex)
+    for n in range(2):
+        kwf += (np.abs(v[:, n]) ** 2).reshape(len(kp), 4).sum(1) / dk
+    assert abs(O.array2hash(kwf) - 1.1814790902) < 1e-5
+
+
+def test_mos2_davidson_fp32_mixed():
+    ex = X.ExcitonTB(X.SystemConfiguration(MOS2), 12, 2, 0, [1., 4., 13.55], [0, 0.1, 0])
+    ex.setExchange(True)
+    ex.brillouinZoneMesh(12)
+    ex.initializeHamiltonian()
+    ex.BShamiltonian()              # FP64 (kept for the refinement)
+    ex.setPrecision("fp32")
+    ex.BShamiltonian()              # FP32
+    res = ex.diagonalize("davidson", 4)
+    check_levels(levels(res.eigval, 4), [(1.810464, 2), (1.825740, 1), (1.855544, 1)], 1e-5)
+
+
+def test_result_observables(tmp_path):
+    ex, res = hbn("diag", 3)
+    k = res.kineticEnergy(0)
+    p = res.potentialEnergy(0)
+    assert k + p == pytest.approx(res.eigval[0], abs=1e-12)
+    assert k > res.eigval[0]  # bound state: negative potential energy
+    f = tmp_path / "hbn.eigval"
+    res.writeEigenvalues(str(f), 3)
+    toks = f.read_text().split()
+    assert int(toks[0]) == 400 and float(toks[1]) == pytest.approx(5.335690, abs=1e-6)

@@ -1,0 +1,143 @@
+"""GPU parity tests through the C-ABI (include/xatu_bse.h).
+
+Each case builds the oracle pipeline on the CPU, hands the oracle's own band
+stacks to the C-ABI, and compares:
+  lattice sums   <= 1e-12 relative to max |V(q)|
+  FP64 BSE matrix <= 1e-10 relative to max |H| and exactly Hermitian
+  FP64 energies  <= 1e-6 eV (dense and Davidson)
+  FP32 matrix     per-block error reported, energies (FP32 products + FP64
+                  refinement) <= 1e-4 eV (0.1 meV)
+"""
+import numpy as np
+import pytest
+
+from cases import MODELS, gpu_context, oracle_case, rel_err
+
+pytestmark = pytest.mark.gpu
+
+HBN = MODELS / "hBN.model"
+MOS2 = MODELS / "MoS2.model"
+
+CASES = {
+    "hbn20": dict(model=HBN, ncell=20, nbands=1, params=(1, 1, 10)),
+    "hbn30_cut20_x": dict(model=HBN, ncell=30, nbands=1, params=(1, 1, 10), cutoff=20, exchange=True),
+    "hbn_coulomb": dict(model=HBN, ncell=9, nbands=1, params=(1, 1, 10), potential="coulomb", exchange=True,
+                        exchange_potential="coulomb", scissor=0.5),
+    "mos2_q_x": dict(model=MOS2, ncell=8, nbands=2, params=(1., 4., 13.55), Q=(0, 0.1, 0), exchange=True),
+    "mos2_reduced": dict(model=MOS2, ncell=6, nbands=2, params=(1., 4., 13.55), factor=2,
+                         shift=(0.6628, -1.1480, 0)),
+    "mos2_odd": dict(model=MOS2, ncell=7, nbands=2, params=(1., 4., 13.55)),
+    "random18": dict(model="synthetic18", ncell=12, nbands=2, params=(1, 1, 40), cutoff=25, exchange=True),
+}
+
+_cache = {}
+
+
+def get_case(name, request):
+    if name not in _cache:
+        kw = dict(CASES[name])
+        if kw["model"] == "synthetic18":
+            kw["model"] = request.getfixturevalue("synthetic18")
+        print(f"[case {name}] oracle build", flush=True)
+        ex = oracle_case(**kw)
+        H = ex.bse()
+        _cache[name] = (ex, H)
+    return _cache[name]
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_fourier_sums(name, request):
+    ex, _ = get_case(name, request)
+    ctx = gpu_context(ex)
+    cells = ctx.get_cells()
+    assert len(cells) == len(ex.cells)
+    cart = cells @ ex.sys.bravais[:ex.sys.ndim]
+    assert np.abs(cart - ex.cells).max() < 1e-12
+    ft, ftq = ctx.get_fourier()
+    assert rel_err(ft, ex.ft) < 1e-12
+    if ex.exchange:
+        assert rel_err(ftq, ex.ftq) < 1e-12
+    ctx.close()
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_bse_fp64_matrix(name, request):
+    ex, H = get_case(name, request)
+    ctx = gpu_context(ex)
+    Hg = ctx.build(0, scissor=ex.scissor, copy_out=True)
+    assert rel_err(Hg, H) < 1e-10
+    assert np.array_equal(Hg, Hg.conj().T)
+    assert np.all(np.imag(np.diag(Hg)) == 0)
+    ctx.close()
+
+
+@pytest.mark.parametrize("name", ["hbn20", "mos2_q_x", "random18"])
+def test_bse_subset_basis(name, request):
+    ex, H = get_case(name, request)
+    ctx = gpu_context(ex)
+    rng = np.random.default_rng(7)
+    idx = np.sort(rng.choice(ex.N, size=min(ex.N, 150), replace=False))
+    Hs = ctx.build_basis(ex.basis[idx], scissor=ex.scissor)
+    assert rel_err(Hs, H[np.ix_(idx, idx)]) < 1e-10
+    ctx.close()
+
+
+@pytest.mark.parametrize("name", ["hbn20", "mos2_q_x", "mos2_reduced", "random18"])
+def test_dense_diag(name, request):
+    ex, H = get_case(name, request)
+    ctx = gpu_context(ex)
+    ctx.build(0, scissor=ex.scissor)
+    w, v, _ = ctx.solve(method=0, nstates=ex.N)
+    wr = np.linalg.eigvalsh(H)
+    assert np.abs(w - wr).max() < 1e-9
+    r = H @ v[:, :8] - v[:, :8] * w[:8]
+    assert np.abs(r).max() < 1e-9
+    ctx.close()
+
+
+@pytest.mark.parametrize("name,nstates", [("hbn20", 3), ("hbn30_cut20_x", 8), ("mos2_q_x", 4),
+                                          ("random18", 16)])
+def test_davidson_fp64(name, nstates, request):
+    ex, H = get_case(name, request)
+    ctx = gpu_context(ex)
+    ctx.build(0, scissor=ex.scissor)
+    w, v, rep = ctx.solve(method=1, nstates=nstates, precision=0, tol=1e-7)
+    wr = np.linalg.eigvalsh(H)[:nstates]
+    assert rep.converged == 1
+    assert np.abs(w - wr).max() < 1e-6
+    assert np.abs(H @ v - v * w).max() < 1e-6
+    ctx.close()
+
+
+@pytest.mark.parametrize("name,nstates", [("hbn30_cut20_x", 8), ("mos2_q_x", 4), ("random18", 16)])
+def test_fp32_build_and_mixed_solve(name, nstates, request):
+    ex, H = get_case(name, request)
+    ctx = gpu_context(ex)
+    ctx.build(0, scissor=ex.scissor)
+    H32 = ctx.build(1, scissor=ex.scissor, copy_out=True)
+    # FP32 assembly: per-element error at FP32 rounding of O(1 eV) entries
+    err = np.abs(H32.astype(np.complex128) - H)
+    assert err.max() < 2e-6 * max(1.0, np.abs(H).max())
+    blk, mx = ctx.compare_builds(4)
+    assert mx == pytest.approx(err.max(), rel=1e-6, abs=1e-12)
+    print(f"[{name}] FP32 block errors: max {mx:.3e}, median {np.median(blk):.3e}")
+    w, v, rep = ctx.solve(method=1, nstates=nstates, precision=1, tol=1e-6)
+    wr = np.linalg.eigvalsh(H)[:nstates]
+    print(f"[{name}] mixed: per-eigenvalue error (meV): {np.round((w - wr) * 1e3, 6)}")
+    assert np.abs(w - wr).max() < 1e-4
+    assert rep.refine_iterations >= 1
+    ctx.close()
+
+
+@pytest.mark.parametrize("name", ["hbn20", "random18"])
+def test_matvec(name, request):
+    ex, H = get_case(name, request)
+    ctx = gpu_context(ex)
+    ctx.build(0, scissor=ex.scissor)
+    ctx.build(1, scissor=ex.scissor)
+    rng = np.random.default_rng(3)
+    x = rng.normal(size=(ex.N, 5)) + 1j * rng.normal(size=(ex.N, 5))
+    y = H @ x
+    assert rel_err(ctx.matvec(x, precision=0), y) < 1e-12
+    assert rel_err(ctx.matvec(x, precision=1), y) < 1e-5
+    ctx.close()

@@ -1,0 +1,283 @@
+"""Pins of the CPU oracle (CPU only).
+
+The oracle is checked against (i) the reference's own regression values
+(test/tests.cpp), (ii) library implementations of the special functions
+(scipy / mpmath), and (iii) closed forms and invariants of the lattice sums and
+of the BSE matrix, chosen so that a dropped term, a wrong sign or index, or a
+transposed operand fails one of them.
+"""
+import numpy as np
+import pytest
+import scipy.special as sp
+
+import oracle as O
+from cases import MODELS, oracle_case
+
+HBN = MODELS / "hBN.model"
+HBN_SPIN = MODELS / "hBN_spinful.model"
+MOS2 = MODELS / "MoS2.model"
+
+
+def levels(H, n):
+    return O.detect_degeneracies(np.linalg.eigvalsh(H), n, 6)
+
+
+def check_levels(got, expected, tol):
+    assert len(got) >= len(expected)
+    for (e, d), (ee, dd) in zip(got, expected):
+        assert abs(e - ee) < tol, (got, expected)
+        assert d == dd, (got, expected)
+
+
+# ---------------------------------------------------------------- reference KATs
+
+def test_hbn_energies_diag():
+    # test/tests.cpp:150-180
+    ex = oracle_case(HBN, 20, 1, 0, (1, 1, 10))
+    check_levels(levels(ex.bse(), 3), [(5.335690, 2), (6.074062, 1)], 1e-4)
+
+
+def test_hbn_energies_davidson_reference_algorithm():
+    # test/tests.cpp:182-212 with the restated davidson_method
+    ex = oracle_case(HBN, 20, 1, 0, (1, 1, 10))
+    w, v = O.davidson_reference(ex.bse(), 3)
+    assert v.shape[1] == 6  # the reference returns 2 * neigval Ritz vectors
+    check_levels(O.detect_degeneracies(w, 3, 6), [(5.335690, 2), (6.074062, 1)], 1e-4)
+
+
+def test_hbn_spinful():
+    # test/tests.cpp:401-431
+    ex = oracle_case(HBN_SPIN, 20, 2, 0, (1, 1, 10))
+    check_levels(levels(ex.bse(), 12), [(5.335690, 8), (6.074062, 4)], 1e-4)
+
+
+def test_hbn_kwf_hash():
+    # test/tests.cpp:281-321
+    ex = oracle_case(HBN, 20, 1, 0, (1, 1, 10))
+    w, v = np.linalg.eigh(ex.bse())
+    ncomb = ex.nc * ex.nv
+    kp = ex.sys.kpoints
+    dk = np.linalg.norm(kp[1] - kp[0])
+    kwf = np.zeros(len(kp), complex)
+    for n in range(2):
+        a = np.abs(v[:, n]) ** 2
+        kwf += a.reshape(len(kp), ncomb).sum(1) / dk
+    assert abs(O.array2hash(kwf) - 1.086145105) < 1e-5
+
+
+def test_mos2_energies():
+    # test/tests.cpp:547-577
+    ex = oracle_case(MOS2, 12, 2, 0, (1., 4., 13.55))
+    check_levels(levels(ex.bse(), 2), [(1.768783, 2)], 1e-4)
+
+
+@pytest.mark.parametrize("q,expected", [(-0.1, 1.810464), (-0.05, 1.780106), (0.0, 1.768783), (0.05, 1.780106),
+                                        (0.1, 1.810464)])
+def test_mos2_exciton_bands(q, expected):
+    # test/tests.cpp:660-702
+    ex = oracle_case(MOS2, 12, 2, 0, (1., 4., 13.55), Q=(0, q, 0))
+    check_levels(levels(ex.bse(), 2)[:1], [(expected, 2)], 1e-5)
+
+
+def test_mos2_exchange():
+    # test/tests.cpp:704-737
+    ex = oracle_case(MOS2, 12, 2, 0, (1., 4., 13.55), Q=(0, 0.1, 0), exchange=True)
+    check_levels(levels(ex.bse(), 4), [(1.810464, 2), (1.825740, 1), (1.855544, 1)], 1e-5)
+
+
+def test_mos2_reduced_bz():
+    # test/tests.cpp:739-774
+    ex = oracle_case(MOS2, 12, 2, 0, (1., 4., 13.55), factor=2, shift=(0.6628, -1.1480, 0))
+    check_levels(levels(ex.bse(), 4), [(1.795378, 1), (1.809810, 1), (1.935880, 1), (1.950567, 1)], 1e-4)
+
+
+def test_mos2_kwf_hash():
+    # test/tests.cpp:579-619
+    ex = oracle_case(MOS2, 12, 2, 0, (1., 4., 13.55))
+    w, v = np.linalg.eigh(ex.bse())
+    ncomb = ex.nc * ex.nv
+    kp = ex.sys.kpoints
+    dk = np.linalg.norm(kp[1] - kp[0])
+    kwf = np.zeros(len(kp), complex)
+    for n in range(2):
+        kwf += (np.abs(v[:, n]) ** 2).reshape(len(kp), ncomb).sum(1) / dk
+    assert abs(O.array2hash(kwf) - 1.1814790902) < 1e-5
+
+
+def test_model_parsing_hashes():
+    # test/tests.cpp:19-61
+    m = O.parse_model(HBN)
+    assert m["ndim"] == 2 and m["filling"] == 1 and m["norbitals"] == [1, 1]
+    # rows are Armadillo subviews in the reference test: aliased hash
+    for row, h in zip(m["bravais"], [4.55422, 2.05422]):
+        assert abs(O.array2hash(row[None, :], aliased=True) - h) < 1e-4
+    for row, h in zip(m["motif"], [0, 3.90211]):
+        assert abs(O.array2hash(row[None, :], aliased=True) - h) < 1e-4
+    for row, h in zip(m["cells"], [0, -0.110843, 2.38916, 2.05422, 4.55422]):
+        assert abs(O.array2hash(row[None, :], aliased=True) - h) < 1e-4
+    for mat, h in zip(m["H"], [-1.34375, -0.9, -0.9, -0.0375, -0.0375]):
+        assert abs(O.array2hash(mat) - h) < 1e-4
+
+
+# ---------------------------------------------------------------- potentials
+
+# The power series loses accuracy to cancellation as x grows (terms reach ~e^x):
+# 2e-12 up to x = 5 (every configuration here has r/r0 <= 5), looser beyond.
+@pytest.mark.parametrize("x,tol", [(1e-3, 2e-12), (0.05, 2e-12), (0.3, 2e-12), (1.0, 2e-12), (2.5, 2e-12),
+                                   (5.0, 2e-12), (7.0, 2e-11), (13.0, 2e-10), (19.9, 5e-9)])
+def test_stvh0_series_matches_library(x, tol):
+    L = O.lib()
+    assert abs(L.oracle_stvh0(x) - sp.struve(0, x)) <= tol * max(1.0, abs(sp.struve(0, x)))
+    assert abs(L.oracle_y0(x) - sp.y0(x)) <= 1e-13 * max(1.0, abs(sp.y0(x)))
+
+
+@pytest.mark.parametrize("x", [20.5, 30.0, 49.0, 55.0, 120.0])
+def test_stvh0_asymptotic_branch_bounded(x):
+    # published Zhang-Jin branch: the Y0 rational approximation limits accuracy (~1e-7)
+    assert abs(O.lib().oracle_stvh0(x) - sp.struve(0, x)) < 2e-6
+
+
+def test_h1_series():
+    for x in [0.01, 0.5, 2.0, 6.0, 15.0]:
+        assert abs(O.lib().oracle_h1(x) - sp.struve(1, x)) < 1e-12 * max(1.0, sp.struve(1, x))
+
+
+def test_keldysh_closed_form_and_special_cases():
+    L = O.lib()
+    em, es, r0, reg, cut = 1.0, 4.0, 13.55, 3.16, 50.0
+    ebar = (em + es) / 2
+    pref = 1.6021766e-19 / (8e-10 * 8.8541878e-12 * ebar * r0)
+    for r in [0.5, 3.0, 10.0, 40.0]:
+        x = r / r0
+        assert abs(L.oracle_potential(0, r, em, es, r0, reg, cut) - pref * (sp.struve(0, x) - sp.y0(x))) < 1e-11
+    # r == 0 -> V(regularization); r > cutoff -> 0
+    assert L.oracle_potential(0, 0.0, em, es, r0, reg, cut) == L.oracle_potential(0, reg, em, es, r0, reg, 1e9)
+    assert L.oracle_potential(0, cut + 1e-3, em, es, r0, reg, cut) == 0.0
+    # Coulomb
+    c = L.oracle_potential(1, 2.0, em, es, r0, reg, cut)
+    assert abs(c - 1.6021766e-19 / (4e-10 * np.pi * 8.8541878e-12 * 2.0)) < 1e-12
+    assert L.oracle_potential(1, 0.0, em, es, r0, reg, cut) == pytest.approx(
+        1.6021766e-19 * 1e10 / (4 * np.pi * 8.8541878e-12 * reg), rel=1e-15)
+
+
+@pytest.mark.parametrize("r", [0.0, 1.0, 4.3, 25.0])
+def test_dvdr0_matches_finite_difference(r):
+    L = O.lib()
+    em, es, r0, reg, cut = 1.0, 1.0, 40.0, 3.19, 80.0
+    h = 1e-5 * r0
+    fd = (L.oracle_potential(0, r, em, es, r0 + h, reg, cut) - L.oracle_potential(0, r, em, es, r0 - h, reg, cut)) / (2 * h)
+    an = L.oracle_potential(2, r, em, es, r0, reg, cut)
+    assert abs(fd - an) <= 1e-7 * abs(an)
+
+
+def test_deps_kind():
+    L = O.lib()
+    v = L.oracle_potential(0, 2.0, 1.0, 3.0, 10.0, 2.5, 50.0)
+    assert L.oracle_potential(3, 2.0, 1.0, 3.0, 10.0, 2.5, 50.0) == pytest.approx(-v / (2 * 2.0), rel=1e-15)
+
+
+# ---------------------------------------------------------------- lattice
+
+@pytest.mark.parametrize("model,a_len,ncell,cutoff,expected", [
+    ("hbn", 2.5, 30, 20, 851),     # config 1 (hBN 30x30, cutoff 20)
+    ("tri", 3.19, 48, 25, 1901),   # config 2
+    ("tri", 3.19, 36, 25, 1239),   # config 3
+    ("tri", 3.19, 96, 25, 2263),   # config 4
+])
+def test_truncate_supercell_counts(model, a_len, ncell, cutoff, expected, synthetic18):
+    s = O.System(HBN if model == "hbn" else synthetic18)
+    cells = s.truncate_supercell(ncell, s.a * cutoff)
+    assert len(cells) == expected
+    assert np.all(np.linalg.norm(cells, axis=1) < s.a * cutoff + 1e-5)
+
+
+@pytest.mark.parametrize("n,factor,shift", [(6, 1, None), (7, 1, None), (4, 2, (0.3, -0.2, 0)), (5, 3, None)])
+def test_find_equivalent_matches_integer_offsets(n, factor, shift):
+    from cases import k_offsets
+    ex = oracle_case(HBN, n, 1, 0, (1, 1, 10), factor=factor, shift=shift)
+    s = ex.sys
+    off = k_offsets(ex)
+    N = n * factor
+    for i in range(ex.nk):
+        for j in range(ex.nk):
+            q = s.find_equivalent(s.kpoints[j] - s.kpoints[i], n)
+            d = (off[j] - off[i] + N // 2) % N
+            assert q == d[0] + N * d[1]
+
+
+# ---------------------------------------------------------------- lattice sums
+
+def test_lattice_sum_periodic_and_conjugate():
+    ex = oracle_case(MOS2, 6, 1, 0, (1., 4., 13.55))
+    b = ex.sys.recip
+    q = np.array([0.123, -0.31, 0.0])
+    v1 = ex.ft_at(q, O.POT_KELDYSH)
+    v2 = ex.ft_at(q + b[0] - 2 * b[1], O.POT_KELDYSH)
+    assert np.abs(v1 - v2).max() < 1e-11 * np.abs(v1).max()
+    # lower triangle is the exact conjugate; the diagonal is the conjugate of its sum
+    iu = np.triu_indices(3, 1)
+    assert np.array_equal(v1[iu], v1.conj().T[iu])
+    raw = sum(O.lib().oracle_potential(0, np.linalg.norm(R), ex.eps_m, ex.eps_s, ex.r0, ex.regularization,
+                                       ex.sys.a * ex.cutoff + 1e-5) * np.exp(1j * q @ R) for R in ex.cells)
+    assert abs(v1[0, 0] - np.conj(raw) / ex.total_cells) < 1e-14
+
+
+def test_lattice_sum_single_cell_closed_form():
+    # a cutoff below one lattice spacing keeps only R = 0: V_ab = V(|tau_a - tau_b|)/T
+    ex = oracle_case(HBN, 4, 1, 0, (1, 1, 10), cutoff=0.1)
+    assert len(ex.cells) == 1
+    L = O.lib()
+    T = ex.total_cells
+    args = (ex.eps_m, ex.eps_s, ex.r0, ex.regularization, ex.sys.a * ex.cutoff + 1e-5)
+    tau = ex.sys.motif[:, :3]
+    for q in range(ex.ft.shape[2]):
+        for a in range(2):
+            for bb in range(2):
+                r = np.linalg.norm(tau[a] - tau[bb])
+                assert ex.ft[a, bb, q] == pytest.approx(L.oracle_potential(0, r, *args) / T, rel=1e-14)
+
+
+def test_lattice_sum_brute_force():
+    ex = oracle_case(HBN, 5, 1, 0, (1, 1, 10), cutoff=1.3)
+    L = O.lib()
+    args = (ex.eps_m, ex.eps_s, ex.r0, ex.regularization, ex.sys.a * ex.cutoff + 1e-5)
+    tau = ex.sys.motif[:, :3]
+    for qi in [0, 7, 13]:
+        q = ex.sys.meshBZ[qi]
+        for a in range(2):
+            for bb in range(a, 2):
+                s = sum(L.oracle_potential(0, np.linalg.norm(R + tau[a] - tau[bb]), *args) * np.exp(1j * q @ R)
+                        for R in ex.cells)
+                assert abs(ex.ft[a, bb, qi] - s / ex.total_cells) < 1e-13
+
+
+# ---------------------------------------------------------------- BSE invariants
+
+def test_bse_without_interaction_is_band_differences():
+    ex = oracle_case(MOS2, 5, 2, 0, (1., 4., 13.55), scissor=0.37)
+    H = ex.bse(ft=np.zeros_like(ex.ft), ftq=np.zeros_like(ex.ftq))
+    de = []
+    for k in range(ex.nk):
+        for ic in range(ex.nc):
+            for iv in range(ex.nv):
+                de.append(0.37 + ex.eigval_kq[ex.nv + ic, k] - ex.eigval_k[iv, k])
+    assert np.allclose(np.sort(np.linalg.eigvalsh(H)), np.sort(de), atol=1e-12)
+    assert np.abs(H - np.diag(np.diag(H))).max() == 0
+
+
+def test_bse_hermitian_and_trace():
+    ex = oracle_case(MOS2, 5, 2, 0, (1., 4., 13.55), Q=(0, 0.1, 0), exchange=True)
+    H = ex.bse()
+    assert np.array_equal(H, H.conj().T)
+    H0 = ex.bse(ft=np.zeros_like(ex.ft), ftq=np.zeros_like(ex.ftq))
+    Hd = ex.bse(eigval_k=np.zeros_like(ex.eigval_k), eigval_kq=np.zeros_like(ex.eigval_kq))
+    # H = H0 + Hint exactly on the diagonal decomposition
+    assert np.abs(np.trace(H) - np.trace(H0) - np.trace(Hd)).max() < 1e-10
+    assert np.abs(H - H0 - Hd).max() < 1e-12
+
+
+def test_bse_exchange_is_rank_natoms():
+    ex = oracle_case(MOS2, 4, 2, 0, (1., 4., 13.55), Q=(0, 0.1, 0), exchange=True)
+    Hx = ex.bse() - ex.bse(ftq=np.zeros_like(ex.ftq))
+    s = np.linalg.svd(Hx, compute_uv=False)
+    assert s[ex.sys.natoms] < 1e-10 * s[0]

@@ -1,8 +1,9 @@
 // Lattice Fourier sums of the screened interaction (reference: motifFourierTransform /
 // motifFTMatrix / initializeMotifFT, src/ExcitonTB.cpp:438-476, 695-697, 761-786).
 //
-//   V_ab(q) = (1/T) sum_R V(|R + tau_a - tau_b|) exp(i q.R),   a <= b,
-//   V_ba(q) = conj(V_ab(q))   (the reference fills the lower triangle by conjugation)
+//   V_ab(q) = (1/T) sum_R V(|R + tau_a - tau_b|) exp(i q.R),   a < b,
+//   V_ba(q) = conj(V_ab(q)),  and V_aa(q) = conj of its sum (the reference fills the
+//   lower triangle by conjugation, overwriting the diagonal with its conjugate)
 //
 // Design: the potential is evaluated once per (pair, R) into a table V_p(R) instead
 // of once per (q, pair, R). On the Monkhorst-Pack mesh q.R = pi (sum_j t_j m_j) / n
@@ -133,8 +134,10 @@ __global__ __launch_bounds__(kFtThreads) void fourier_mesh_kernel(LatticeDesc la
             const double re = s_red[0][0] * inv_total;
             const double im = s_red[1][0] * inv_total;
             const size_t base = static_cast<size_t>(q) * lat.natoms * lat.natoms;
-            ft[base + a + static_cast<size_t>(lat.natoms) * b] = make_double2(re, im);
-            if (a != b) ft[base + b + static_cast<size_t>(lat.natoms) * a] = make_double2(re, -im);
+            // (a,b) holds the sum and (b,a) its conjugate; for a == b the conjugate is
+            // what remains, as the reference assigns (b,a) after (a,b).
+            if (a != b) ft[base + a + static_cast<size_t>(lat.natoms) * b] = make_double2(re, im);
+            ft[base + b + static_cast<size_t>(lat.natoms) * a] = make_double2(re, -im);
         }
         __syncthreads();
     }
@@ -192,8 +195,8 @@ __global__ __launch_bounds__(kFtThreads) void fourier_q_kernel(LatticeDesc lat, 
             const int b = a + rem;
             const double re = s_red[0][0] * inv_total;
             const double im = s_red[1][0] * inv_total;
-            ftq[a + lat.natoms * b] = make_double2(re, im);
-            if (a != b) ftq[b + lat.natoms * a] = make_double2(re, -im);
+            if (a != b) ftq[a + lat.natoms * b] = make_double2(re, im);
+            ftq[b + lat.natoms * a] = make_double2(re, -im);
         }
         __syncthreads();
     }

@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstring>
 
+#include "../common/heev.hpp"
 #include "exciton.hpp"
 
 namespace py = pybind11;
@@ -315,4 +316,13 @@ PYBIND11_MODULE(_xatu, m) {
         .def("contextAddress", [](const ExcitonTB& e) { return reinterpret_cast<std::uintptr_t>(e.context()); });
 
     m.def("detectDegeneracies", &detectDegeneracies, py::arg("eigval"), py::arg("n"), py::arg("precision") = 6);
+    // The host Hermitian eigensolver (bands, Rayleigh-Ritz), exposed for testing.
+    m.def("hermitianEigh", [](py::array_t<cdouble, py::array::f_style | py::array::forcecast> a) {
+        const int n = static_cast<int>(a.shape(0));
+        if (a.ndim() != 2 || a.shape(1) != n) throw std::invalid_argument("square matrix expected");
+        std::vector<double> w(n);
+        std::vector<cdouble> z(static_cast<size_t>(n) * n);
+        xatu_linalg::heev(n, a.data(), n, w.data(), z.data());
+        return py::make_tuple(py::array_t<double>(n, w.data()), fortran(z, {n, n}));
+    });
 }
