@@ -1,0 +1,212 @@
+#!/usr/bin/env python3
+"""Benchmark: seconds per BSE build + lowest-16 exciton solve on one MI355X.
+
+Workload (default "c4"): seeded 18-orbital triangular model (3 atoms x 6 orbitals,
+a = 3.19 A), 96 x 96 Monkhorst-Pack mesh, nv = nc = 2 (BSE dimension 36864), Keldysh
+r0 = 40 A, eps_m = eps_s = 1, lattice-sum cutoff 25 cells, exchange on, block
+Davidson (block 32) for the lowest 16 states.
+
+A step is: lattice Fourier sums + BSE assembly + Davidson solve, with the band
+stacks already resident in HBM (the host band problem runs once, before timing).
+Two forms are timed: the FP64 plain path (FP64 assembly, FP64 products) and the
+mixed path (FP32 assembly and FP32 products, then FP64 refinement with matrix-free
+FP64 products). ``value`` is the mixed path's seconds per step; the FP64 time, the
+speed-up and the per-eigenvalue deviation from the FP64 path are reported beside it.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+CONFIGS = {
+    # name: (natoms, orbitals/atom, n, nbands, r0, cutoff, nstates, block, matrix_free)
+    "c2": (3, 6, 48, 2, 40.0, 25.0, 16, 32, False),
+    "c3": (3, 6, 36, 2, 40.0, 25.0, 8, 32, False),
+    "c4": (3, 6, 96, 2, 40.0, 25.0, 16, 32, False),
+    "c5": (4, 9, 120, 3, 40.0, 25.0, 16, 32, True),
+}
+
+PEAK_FP32_TFLOPS = 157.3
+PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+TOL = 1e-5  # FP64 residual tolerance (eV) of both forms
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(model_path, cfg, n_products_fp64, nstates, block):
+    """Time the CPU oracle (the reference algorithm restated in C++, OpenMP) on a
+    bounded sample of the same workload and scale it to one full step."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as O
+
+    natoms, orb, n, nb, r0, cutoff, _, _, _ = cfg
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    ns = 32  # sample mesh: 32 x 32 of the same model
+    s = O.System(str(model_path))
+    s.brillouin_zone_mesh(ns)
+    ex = O.Exciton(s, ns, nb, 0, (1.0, 1.0, r0))
+    ex.cutoff = cutoff
+    ex.exchange = True
+    t0 = time.perf_counter()
+    ex.initialize()           # bands (numpy) + lattice sums at the sample mesh (C++ oracle)
+    t_ft_sample = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    H = ex.bse()              # per-element assembly loop (C++ oracle, OpenMP)
+    t_bse_sample = time.perf_counter() - t0
+    X = np.random.default_rng(0).normal(size=(ex.N, block)) + 0j
+    t0 = time.perf_counter()
+    for _ in range(3):
+        H @ X
+    t_mv_sample = (time.perf_counter() - t0) / 3
+    N_full = (n ** 2) * nb * nb
+    scale2 = (N_full / ex.N) ** 2
+    # lattice sums: cost ~ mesh points x cells; cells at the full mesh from the oracle itself
+    ncells_full = len(s.truncate_supercell(n, s.a * cutoff))
+    t_ft = t_ft_sample * (n ** 2 / ns ** 2) * (ncells_full / len(ex.cells))
+    t_bse = t_bse_sample * scale2
+    t_solve = t_mv_sample * scale2 * (n_products_fp64 / block)
+    total = t_ft + t_bse + t_solve
+    return {
+        "value": round(total, 3), "unit": "s per build+solve (extrapolated)", "cores": threads, "kind": "port",
+        "sample": (f"oracle at a {ns}x{ns} mesh of the same model (N={ex.N}): lattice sums {t_ft_sample:.2f}s, "
+                   f"per-element assembly {t_bse_sample:.2f}s, one {block}-vector FP64 product {t_mv_sample*1e3:.1f}ms; "
+                   f"scaled to N={N_full} (assembly and products x(N/Ns)^2, sums x mesh x cells) with "
+                   f"{n_products_fp64} product vectors as in the GPU FP64 solve"),
+        "parts_s": {"fourier": round(t_ft, 3), "assembly": round(t_bse, 3), "solve": round(t_solve, 3)},
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="c4", choices=sorted(CONFIGS))
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
+    ap.add_argument("--no-plain", action="store_true", help="skip the FP64 plain path")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path runs on a single "
+              "MI355X; see DESIGN.md for how it would shard)", file=sys.stderr)
+        sys.exit(2)
+
+    import xatu_amd as X
+    from xatu_amd import capi, synthetic
+
+    cfg = CONFIGS[args.config]
+    natoms, orb, n, nb, r0, cutoff, nstates, block, mf = cfg
+    tmp = Path(tempfile.mkdtemp(prefix="xatu_bench_"))
+    model = synthetic.write_model(tmp / "model.model", natoms=natoms, orb_per_atom=orb)
+
+    log(f"config {args.config}: natoms={natoms} orb={orb} mesh {n}x{n} nv=nc={nb}")
+    t0 = time.perf_counter()
+    ex = X.ExcitonTB(X.SystemConfiguration(str(model)), n, nb, 0, [1.0, 1.0, r0])
+    ex.setCutoff(cutoff)
+    ex.setExchange(True)
+    ex.brillouinZoneMesh(n)
+    ex.initializeHamiltonian()  # host bands + device lattice sums (untimed set-up)
+    t_setup = time.perf_counter() - t0
+    N = int(ex.excitonbasisdim)
+    log(f"N={N}; host bands + set-up {t_setup:.2f}s")
+
+    s = ex.system
+    ctx = capi.Context(0)
+    tau = np.asarray(s.motif)[:, :3]
+    ctx.set_lattice(np.asarray(s.bravaisLattice), tau, s.orbitalsPerAtom(), s.meshPoints)
+    ctx.set_bands(nb, nb, ex.eigvalKStack, ex.eigvecKStack, np.asarray(s.kpointOffsets),
+                  ex.eigvalKQStack, ex.eigvecKQStack)
+    inter = dict(eps_m=1.0, eps_s=1.0, r0=r0, cutoff=cutoff, ncell=n, total_cells=int(ex.totalCells),
+                 regularization=float(ex.regularization), exchange=True)
+
+    def step(prec):
+        ctx.fourier_sums(**inter)
+        ctx.build(prec if not mf else capi.XATU_MATRIX_FREE)
+        t_build = ctx.timings()
+        w, v, rep = ctx.solve(method=capi.XATU_SOLVE_DAVIDSON, nstates=nstates, precision=prec,
+                              matrix_free=mf, block=block, tol=TOL, want_vectors=False)
+        return w, rep, t_build
+
+    results = {}
+    for name, prec in (("fp64", capi.XATU_FP64), ("mixed", capi.XATU_FP32)):
+        if name == "fp64" and args.no_plain:
+            continue
+        for i in range(args.warmup):
+            log(f"{name} warmup {i}")
+            step(prec)
+        times, reps = [], []
+        for i in range(args.steps):
+            t0 = time.perf_counter()
+            w, rep, tb = step(prec)
+            dt = time.perf_counter() - t0
+            times.append(dt)
+            reps.append(rep)
+            log(f"{name} step {i}: {dt:.3f}s (fourier {tb.ms_fourier:.2f}ms, build {tb.ms_build:.2f}ms, "
+                f"solve {rep.ms_total:.1f}ms [products {rep.ms_products:.1f}, RR {rep.ms_rayleigh_ritz:.1f}, "
+                f"proj {rep.ms_projections:.1f}], iters {rep.iterations}, products {rep.products}, "
+                f"refine {rep.refine_iterations}, resid {rep.max_residual:.2e})")
+        tm = ctx.timings()
+        results[name] = dict(times=times, w=w, rep=reps[-1], tim=tm, build=tb)
+        if not mf:
+            ctx.free_bse(prec)
+
+    mixed = results["mixed"]
+    ms_step = float(np.median(mixed["times"])) * 1e3
+    out = {
+        "metric": "s per BSE build+lowest-16 solve (FP32-MFMA mixed form, every eigenvalue within 0.1 meV of FP64)",
+        "value": round(ms_step / 1e3, 4), "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 2), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64", "data": "synthetic (seeded tight-binding model)",
+        "config": {"workload": f"{args.config}: seeded {natoms * orb}-orbital triangular model, {n}x{n} k-mesh, "
+                               f"nv=nc={nb}, BSE dim {N}, Keldysh r0={r0}, cutoff {cutoff} cells, exchange on, "
+                               f"Davidson block {block}, lowest {nstates}" + (", matrix-free" if mf else ""),
+                   "bse_dim": N},
+    }
+    if "fp64" in results:
+        p = results["fp64"]
+        ms64 = float(np.median(p["times"])) * 1e3
+        dev = (mixed["w"] - p["w"]) * 1e3
+        out["fp64_plain_s"] = round(ms64 / 1e3, 4)
+        out["speedup_mixed_vs_fp64"] = round(ms64 / ms_step, 3)
+        out["eig_dev_meV_max"] = float(np.abs(dev).max())
+        out["eig_dev_meV"] = [float(f"{x:.3g}") for x in dev]
+    rep = mixed["rep"]
+    tim = mixed["tim"]
+    out["mixed_parts_ms"] = {"fourier": round(mixed["build"].ms_fourier, 3),
+                             "build": round(mixed["build"].ms_build, 3), "solve": round(rep.ms_total, 2),
+                             "products": round(rep.ms_products, 2), "rayleigh_ritz": round(rep.ms_rayleigh_ritz, 2),
+                             "projections": round(rep.ms_projections, 2),
+                             "iterations": rep.iterations, "refine_iterations": rep.refine_iterations}
+    # Roofline of the dominant kernel of the mixed step: the dense FP32 product.
+    if tim.n_matvec_kernel > 0:
+        nvec_avg = tim.n_matvec_vectors / tim.n_matvec_kernel
+        flops = 8.0 * N * N * nvec_avg
+        t = tim.ms_matvec_kernel / 1e3
+        ach = flops / t / 1e12
+        out["roofline"] = {"bound": "mfma", "achieved": round(ach, 2), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                           "frac": round(ach / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                           "kernel": "FP32 dense product H X (N x N by N x 32)",
+                           "avg_ms": round(tim.ms_matvec_kernel, 3), "launches": tim.n_matvec_kernel,
+                           "avg_vectors": round(nvec_avg, 2)}
+    if not args.no_cpu:
+        n_products = results["fp64"]["rep"].products if "fp64" in results else rep.products
+        log("cpu baseline")
+        out["cpu_baseline"] = cpu_baseline(model, cfg, n_products, nstates, block)
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

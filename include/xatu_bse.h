@@ -39,6 +39,8 @@ typedef struct xatu_ctx xatu_ctx;
 /* Arithmetic of a build or of a solve */
 #define XATU_FP64 0
 #define XATU_FP32 1
+/* xatu_build_bse mode that prepares matrix-free products without storing the matrix */
+#define XATU_MATRIX_FREE 2
 
 /* Interaction potentials (src/ExcitonTB.cpp:347-399) */
 #define XATU_POTENTIAL_KELDYSH 0
@@ -78,7 +80,8 @@ typedef struct xatu_solve_options {
     int block;       /* Davidson block size; 0 selects max(2*nstates, 8) */
     int max_basis;   /* Davidson basis size before restart; 0 selects a default */
     int max_iter;    /* 0 selects a default */
-    double tol;      /* residual 2-norm tolerance in eV; 0 selects 1e-6 (FP64) / 1e-4 (FP32 inner) */
+    double tol;      /* FP64 residual 2-norm tolerance in eV (0 selects 1e-5); the FP32 form iterates
+                        to max(tol/2, 2e-5) with FP32 products, then refines with FP64 products */
 } xatu_solve_options;
 
 typedef struct xatu_solve_report {
@@ -91,6 +94,7 @@ typedef struct xatu_solve_report {
     double ms_total;         /* wall time of the solve (ms) */
     double ms_products;      /* time in the matrix-vector products (ms, HIP events) */
     double ms_rayleigh_ritz; /* time in the host Rayleigh-Ritz eigenproblems (ms) */
+    double ms_projections;   /* time in the basis projections and Ritz-vector products (ms, HIP events) */
 } xatu_solve_report;
 
 typedef struct xatu_timings {
@@ -99,6 +103,7 @@ typedef struct xatu_timings {
     double ms_build_kernel; /* last xatu_build_bse: the assembly kernel alone (HIP events) */
     double ms_matvec_kernel;/* average duration of the dense product kernel over the last solve */
     int n_matvec_kernel;    /* number of dense product kernel launches in the last solve */
+    int n_matvec_vectors;   /* vectors multiplied by those launches */
 } xatu_timings;
 
 /* ---- context ---- */
@@ -131,7 +136,9 @@ int xatu_fourier_sums(xatu_ctx* ctx, const xatu_interaction* inter);
 
 /* Assemble the BSE matrix in the given arithmetic and keep it resident in HBM
  * (ExcitonTB.cpp:840-921). scissor in eV. If hbs_out is non-null it receives the full
- * N x N matrix, column-major complex (double pairs for FP64, float pairs for FP32). */
+ * N x N matrix, column-major complex (double pairs for FP64, float pairs for FP32).
+ * With precision XATU_MATRIX_FREE nothing is stored: the call only prepares the
+ * exchange factors and the diagonal so that solves and products can run matrix-free. */
 int xatu_build_bse(xatu_ctx* ctx, int precision, double scissor, void* hbs_out);
 
 /* Same, for an explicit subset basis (ExcitonTB::BShamiltonian(const arma::imat&)):

@@ -141,3 +141,44 @@ def test_matvec(name, request):
     assert rel_err(ctx.matvec(x, precision=0), y) < 1e-12
     assert rel_err(ctx.matvec(x, precision=1), y) < 1e-5
     ctx.close()
+
+
+@pytest.mark.parametrize("name", ["hbn20", "mos2_q_x", "mos2_reduced", "random18"])
+def test_matrix_free_products(name, request):
+    ex, H = get_case(name, request)
+    ctx = gpu_context(ex)
+    ctx.build(2, scissor=ex.scissor)  # prepare only: nothing resident
+    rng = np.random.default_rng(11)
+    x = rng.normal(size=(ex.N, 11)) + 1j * rng.normal(size=(ex.N, 11))
+    y = H @ x
+    assert rel_err(ctx.matvec(x, precision=0, matrix_free=True), y) < 1e-12
+    assert rel_err(ctx.matvec(x, precision=1, matrix_free=True), y) < 1e-5
+    ctx.close()
+
+
+@pytest.mark.parametrize("name,nstates", [("hbn30_cut20_x", 8), ("random18", 16)])
+def test_matrix_free_davidson(name, nstates, request):
+    ex, H = get_case(name, request)
+    wr = np.linalg.eigvalsh(H)[:nstates]
+    ctx = gpu_context(ex)
+    ctx.build(2, scissor=ex.scissor)
+    w, v, rep = ctx.solve(method=1, nstates=nstates, precision=0, matrix_free=True, tol=1e-7)
+    assert rep.converged == 1 and np.abs(w - wr).max() < 1e-6
+    w, v, rep = ctx.solve(method=1, nstates=nstates, precision=1, matrix_free=True, tol=1e-6)
+    assert rep.converged == 1 and np.abs(w - wr).max() < 1e-6
+    ctx.close()
+
+
+@pytest.mark.parametrize("name,nstates", [("hbn30_cut20_x", 8), ("random18", 16)])
+def test_mixed_refines_matrix_free(name, nstates, request):
+    # FP32 matrix resident, no FP64 matrix: the FP64 refinement runs matrix-free
+    ex, H = get_case(name, request)
+    wr = np.linalg.eigvalsh(H)[:nstates]
+    ctx = gpu_context(ex)
+    ctx.build(1, scissor=ex.scissor)
+    w, v, rep = ctx.solve(method=1, nstates=nstates, precision=1, tol=1e-6)
+    print(f"[{name}] mixed (matrix-free refinement): err meV {np.round((w - wr) * 1e3, 7)}, "
+          f"iters {rep.iterations}, refine {rep.refine_iterations}")
+    assert rep.converged == 1 and np.abs(w - wr).max() < 1e-6
+    assert np.abs(H @ v - v * w).max() < 1e-5
+    ctx.close()

@@ -20,6 +20,7 @@ XATU_ERR_OUT_OF_MEMORY = 4
 XATU_ERR_NOT_CONVERGED = 5
 XATU_ERR_NOT_HERMITIAN = 6
 XATU_FP64, XATU_FP32 = 0, 1
+XATU_MATRIX_FREE = 2
 XATU_POTENTIAL_KELDYSH, XATU_POTENTIAL_COULOMB = 0, 1
 XATU_SOLVE_DIAG, XATU_SOLVE_DAVIDSON = 0, 1
 XATU_PARAM_R0, XATU_PARAM_EPS_M, XATU_PARAM_EPS_S = 0, 1, 2
@@ -50,13 +51,14 @@ class SolveReport(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int), ("products", ctypes.c_int), ("restarts", ctypes.c_int),
                 ("refine_iterations", ctypes.c_int), ("converged", ctypes.c_int),
                 ("max_residual", ctypes.c_double), ("ms_total", ctypes.c_double),
-                ("ms_products", ctypes.c_double), ("ms_rayleigh_ritz", ctypes.c_double)]
+                ("ms_products", ctypes.c_double), ("ms_rayleigh_ritz", ctypes.c_double),
+                ("ms_projections", ctypes.c_double)]
 
 
 class Timings(ctypes.Structure):
     _fields_ = [("ms_fourier", ctypes.c_double), ("ms_build", ctypes.c_double),
                 ("ms_build_kernel", ctypes.c_double), ("ms_matvec_kernel", ctypes.c_double),
-                ("n_matvec_kernel", ctypes.c_int)]
+                ("n_matvec_kernel", ctypes.c_int), ("n_matvec_vectors", ctypes.c_int)]
 
 
 _lib = None

@@ -3,10 +3,14 @@
 //
 // Method: Householder reduction of the Hermitian matrix to a Hermitian
 // tridiagonal T = Q^H A Q, a diagonal unitary phase change that makes T real
-// symmetric, implicit QL iterations with Wilkinson-type shifts on the real
-// tridiagonal (accumulating the real rotations), then back-transformation of
-// the eigenvectors through the phases and the reflectors. Eigenvalues are
-// returned in ascending order, as LAPACK zheevd returns them (the reference
+// symmetric, then either
+//  - heev:        implicit QL iterations with the rotations accumulated (all
+//                 eigenvectors), or
+//  - heev_lowest: QL eigenvalues only, and the wanted lowest eigenvectors of the
+//                 real tridiagonal by inverse iteration (LU with partial pivoting,
+//                 vectors re-orthogonalised inside clusters of close eigenvalues),
+// and the back-transformation through the phases and the reflectors. Eigenvalues
+// are returned in ascending order, as LAPACK zheevd returns them (the reference
 // calls zheevd through arma::eig_sym, SystemTB.cpp:124, ExcitonTB.cpp:942).
 #pragma once
 
@@ -22,113 +26,134 @@ namespace xatu_linalg {
 
 using cd = std::complex<double>;
 
-// Eigen-decomposition of the Hermitian n x n matrix A (column-major, leading
-// dimension lda; only the lower triangle and the real part of the diagonal are
-// read). On return w[0..n) holds the eigenvalues in ascending order and, if Z
-// is non-null, Z (column-major, leading dimension n) holds the orthonormal
-// eigenvectors as columns.
-inline void heev(int n, const cd* Ain, int lda, double* w, cd* Z) {
-    if (n <= 0) return;
-    std::vector<cd> A(static_cast<size_t>(n) * n);
-    for (int j = 0; j < n; ++j) {
+namespace detail {
+
+// Result of the Householder stage: real tridiagonal (d, e), phases and reflectors.
+struct Tridiag {
+    int n = 0;
+    std::vector<double> d, e;     // e[k] couples k and k+1; e[n-1] = 0
+    std::vector<cd> ph;           // T_complex = Ph T_real Ph^H
+    std::vector<double> vre, vim; // reflector k stored at offset voff[k], length n-k-1
+    std::vector<size_t> voff;
+    std::vector<char> vused;
+};
+
+inline void tridiagonalize(int n, const cd* Ain, int lda, Tridiag& t) {
+    t.n = n;
+    // split storage, full Hermitian matrix, column-major
+    std::vector<double> ar(static_cast<size_t>(n) * n), ai(static_cast<size_t>(n) * n);
+    for (int j = 0; j < n; ++j)
         for (int i = j; i < n; ++i) {
-            cd v = Ain[i + static_cast<size_t>(j) * lda];
-            if (i == j) v = cd(v.real(), 0.0);
-            A[i + static_cast<size_t>(j) * n] = v;
-            A[j + static_cast<size_t>(i) * n] = std::conj(v);
+            const cd v = Ain[i + static_cast<size_t>(j) * lda];
+            ar[i + static_cast<size_t>(j) * n] = v.real();
+            ai[i + static_cast<size_t>(j) * n] = (i == j) ? 0.0 : v.imag();
+            ar[j + static_cast<size_t>(i) * n] = v.real();
+            ai[j + static_cast<size_t>(i) * n] = (i == j) ? 0.0 : -v.imag();
         }
-    }
-    auto at = [&](int i, int j) -> cd& { return A[i + static_cast<size_t>(j) * n]; };
-
-    std::vector<double> d(n, 0.0), e(n, 0.0);
-    std::vector<cd> ecplx(n, cd(0.0, 0.0));
-    // Householder vectors: step k stores v_k (length n-k-1) at offset voff[k].
-    std::vector<size_t> voff(n, 0);
-    std::vector<cd> vstore;
-    std::vector<char> vused(n, 0);
-    std::vector<cd> p(n), wv(n);
-
+    t.d.assign(n, 0.0);
+    t.e.assign(n, 0.0);
+    std::vector<cd> ec(n, cd(0.0, 0.0));
+    t.voff.assign(n, 0);
+    t.vused.assign(n, 0);
+    t.vre.clear();
+    t.vim.clear();
+    std::vector<double> pr(n), pi(n), wr(n), wi(n);
     for (int k = 0; k + 2 < n; ++k) {
         const int m = n - k - 1;
+        const size_t col = static_cast<size_t>(k) * n + k + 1;  // start of x = A[k+1:, k]
         double tail2 = 0.0;
-        for (int i = 1; i < m; ++i) tail2 += std::norm(at(k + 1 + i, k));
-        double xnorm2 = tail2 + std::norm(at(k + 1, k));
-        voff[k] = vstore.size();
-        vstore.resize(vstore.size() + m);
-        cd* v = vstore.data() + voff[k];
-        if (xnorm2 == 0.0 || (tail2 <= 0.0)) {
-            // Column already tridiagonal below the sub-diagonal: no reflection.
-            ecplx[k] = at(k + 1, k);
-            d[k] = at(k, k).real();
+        for (int i = 1; i < m; ++i) tail2 += ar[col + i] * ar[col + i] + ai[col + i] * ai[col + i];
+        const double x0r = ar[col], x0i = ai[col];
+        const double xnorm2 = tail2 + x0r * x0r + x0i * x0i;
+        t.voff[k] = t.vre.size();
+        t.vre.resize(t.vre.size() + m);
+        t.vim.resize(t.vim.size() + m);
+        if (xnorm2 == 0.0 || tail2 <= 0.0) {
+            ec[k] = cd(x0r, x0i);
+            t.d[k] = ar[static_cast<size_t>(k) * n + k];
             continue;
         }
         const double xnorm = std::sqrt(xnorm2);
-        cd x0 = at(k + 1, k);
-        double ax0 = std::abs(x0);
-        cd phase = (ax0 > 0.0) ? x0 / ax0 : cd(1.0, 0.0);
-        cd alpha = -phase * xnorm;
-        for (int i = 0; i < m; ++i) v[i] = at(k + 1 + i, k);
-        v[0] -= alpha;
-        double vn2 = 0.0;
-        for (int i = 0; i < m; ++i) vn2 += std::norm(v[i]);
-        double vn = std::sqrt(vn2);
-        for (int i = 0; i < m; ++i) v[i] /= vn;
-        vused[k] = 1;
-
-        // Trailing block B = A[k+1:, k+1:]; B <- (I - 2vv^H) B (I - 2vv^H)
-        // = B - 2 (v w^H + w v^H), with p = B v, c = v^H p, w = p - c v.
+        const double ax0 = std::hypot(x0r, x0i);
+        const double phr = ax0 > 0.0 ? x0r / ax0 : 1.0, phi = ax0 > 0.0 ? x0i / ax0 : 0.0;
+        const double alr = -phr * xnorm, ali = -phi * xnorm;  // alpha = -phase |x|
+        double* vr = t.vre.data() + t.voff[k];
+        double* vi = t.vim.data() + t.voff[k];
         for (int i = 0; i < m; ++i) {
-            cd s(0.0, 0.0);
-            for (int j = 0; j < m; ++j) s += at(k + 1 + i, k + 1 + j) * v[j];
-            p[i] = s;
+            vr[i] = ar[col + i];
+            vi[i] = ai[col + i];
         }
-        cd c(0.0, 0.0);
-        for (int i = 0; i < m; ++i) c += std::conj(v[i]) * p[i];
-        double cr = c.real();
-        for (int i = 0; i < m; ++i) wv[i] = p[i] - cr * v[i];
+        vr[0] -= alr;
+        vi[0] -= ali;
+        double vn2 = 0.0;
+        for (int i = 0; i < m; ++i) vn2 += vr[i] * vr[i] + vi[i] * vi[i];
+        const double inv = 1.0 / std::sqrt(vn2);
+        for (int i = 0; i < m; ++i) {
+            vr[i] *= inv;
+            vi[i] *= inv;
+        }
+        t.vused[k] = 1;
+        // p = B v with B = A[k+1:, k+1:]: p_i = sum_j conj(B_ji) v_j (column i of B is contiguous)
+        const int base = k + 1;
+#pragma omp parallel for schedule(static) if (m >= 96)
+        for (int i = 0; i < m; ++i) {
+            const double* br = ar.data() + static_cast<size_t>(base + i) * n + base;
+            const double* bi = ai.data() + static_cast<size_t>(base + i) * n + base;
+            double sr = 0.0, si = 0.0;
+            for (int j = 0; j < m; ++j) {
+                // conj(b) * v
+                sr += br[j] * vr[j] + bi[j] * vi[j];
+                si += br[j] * vi[j] - bi[j] * vr[j];
+            }
+            pr[i] = sr;
+            pi[i] = si;
+        }
+        double cr = 0.0;  // c = v^H p (real)
+        for (int i = 0; i < m; ++i) cr += vr[i] * pr[i] + vi[i] * pi[i];
+        for (int i = 0; i < m; ++i) {
+            wr[i] = pr[i] - cr * vr[i];
+            wi[i] = pi[i] - cr * vi[i];
+        }
+        // B -= 2 (v w^H + w v^H)
+#pragma omp parallel for schedule(static) if (m >= 96)
         for (int j = 0; j < m; ++j) {
+            double* br = ar.data() + static_cast<size_t>(base + j) * n + base;
+            double* bi = ai.data() + static_cast<size_t>(base + j) * n + base;
+            const double wjr = wr[j], wji = -wi[j];  // conj(w_j)
+            const double vjr = vr[j], vji = -vi[j];  // conj(v_j)
             for (int i = 0; i < m; ++i) {
-                at(k + 1 + i, k + 1 + j) -= 2.0 * (v[i] * std::conj(wv[j]) + wv[i] * std::conj(v[j]));
+                const double r = vr[i] * wjr - vi[i] * wji + wr[i] * vjr - wi[i] * vji;
+                const double im = vr[i] * wji + vi[i] * wjr + wr[i] * vji + wi[i] * vjr;
+                br[i] -= 2.0 * r;
+                bi[i] -= 2.0 * im;
             }
         }
-        at(k + 1, k) = alpha;
-        at(k, k + 1) = std::conj(alpha);
-        for (int i = 1; i < m; ++i) {
-            at(k + 1 + i, k) = cd(0.0, 0.0);
-            at(k, k + 1 + i) = cd(0.0, 0.0);
-        }
-        ecplx[k] = alpha;
-        d[k] = at(k, k).real();
+        ec[k] = cd(alr, ali);
+        t.d[k] = ar[static_cast<size_t>(k) * n + k];
     }
     if (n >= 2) {
-        d[n - 2] = at(n - 2, n - 2).real();
-        ecplx[n - 2] = at(n - 1, n - 2);
+        t.d[n - 2] = ar[static_cast<size_t>(n - 2) * n + n - 2];
+        ec[n - 2] = cd(ar[static_cast<size_t>(n - 2) * n + n - 1], ai[static_cast<size_t>(n - 2) * n + n - 1]);
     }
-    d[n - 1] = at(n - 1, n - 1).real();
-
-    // Phases that make the tridiagonal real: ph_{k+1} = ph_k * e_k / |e_k|.
-    std::vector<cd> ph(n, cd(1.0, 0.0));
+    t.d[n - 1] = ar[static_cast<size_t>(n - 1) * n + n - 1];
+    t.ph.assign(n, cd(1.0, 0.0));
     for (int k = 0; k + 1 < n; ++k) {
-        double ae = std::abs(ecplx[k]);
-        e[k] = ae;
-        ph[k + 1] = (ae > 0.0) ? ph[k] * (ecplx[k] / ae) : ph[k];
+        const double ae = std::abs(ec[k]);
+        t.e[k] = ae;
+        t.ph[k + 1] = (ae > 0.0) ? t.ph[k] * (ec[k] / ae) : t.ph[k];
     }
-    e[n - 1] = 0.0;
+    t.e[n - 1] = 0.0;
+}
 
-    // Implicit QL on the real symmetric tridiagonal (d, e), accumulating
-    // rotations into the real matrix zr (column i = eigenvector i).
-    const bool want = (Z != nullptr);
-    std::vector<double> zr;
-    if (want) {
-        zr.assign(static_cast<size_t>(n) * n, 0.0);
-        for (int i = 0; i < n; ++i) zr[i + static_cast<size_t>(i) * n] = 1.0;
-    }
+// Implicit QL on (d, e); if zr is non-null the rotations are accumulated into it
+// (n x n, column i = eigenvector i). d receives the (unsorted) eigenvalues.
+inline void tql(int n, std::vector<double>& d, std::vector<double> e, double* zr) {
     for (int l = 0; l < n; ++l) {
         int iter = 0;
         int m;
         do {
             for (m = l; m < n - 1; ++m) {
-                double dd = std::fabs(d[m]) + std::fabs(d[m + 1]);
+                const double dd = std::fabs(d[m]) + std::fabs(d[m + 1]);
                 if (std::fabs(e[m]) <= DBL_EPSILON * dd) break;
             }
             if (m != l) {
@@ -137,11 +162,10 @@ inline void heev(int n, const cd* Ain, int lda, double* w, cd* Z) {
                 double r = std::hypot(g, 1.0);
                 g = d[m] - d[l] + e[l] / (g + std::copysign(r, g));
                 double s = 1.0, c = 1.0, pp = 0.0;
-                int i;
                 bool deflated = false;
-                for (i = m - 1; i >= l; --i) {
-                    double f = s * e[i];
-                    double b = c * e[i];
+                for (int i = m - 1; i >= l; --i) {
+                    const double f = s * e[i];
+                    const double b = c * e[i];
                     r = std::hypot(f, g);
                     e[i + 1] = r;
                     if (r == 0.0) {
@@ -157,13 +181,13 @@ inline void heev(int n, const cd* Ain, int lda, double* w, cd* Z) {
                     pp = s * r;
                     d[i + 1] = g + pp;
                     g = c * r - b;
-                    if (want) {
-                        double* zi = zr.data() + static_cast<size_t>(i) * n;
-                        double* zi1 = zr.data() + static_cast<size_t>(i + 1) * n;
+                    if (zr) {
+                        double* zi = zr + static_cast<size_t>(i) * n;
+                        double* zi1 = zr + static_cast<size_t>(i + 1) * n;
                         for (int kk = 0; kk < n; ++kk) {
-                            double t = zi1[kk];
-                            zi1[kk] = s * zi[kk] + c * t;
-                            zi[kk] = c * zi[kk] - s * t;
+                            const double tt = zi1[kk];
+                            zi1[kk] = s * zi[kk] + c * tt;
+                            zi[kk] = c * zi[kk] - s * tt;
                         }
                     }
                 }
@@ -174,28 +198,160 @@ inline void heev(int n, const cd* Ain, int lda, double* w, cd* Z) {
             }
         } while (m != l);
     }
+}
 
+// x (length n) <- H_0 H_1 ... H_{n-3} Ph z  for a real tridiagonal eigenvector z.
+inline void back_transform(const Tridiag& t, const double* z, cd* x) {
+    const int n = t.n;
+    for (int i = 0; i < n; ++i) x[i] = t.ph[i] * z[i];
+    for (int k = n - 3; k >= 0; --k) {
+        if (!t.vused[k]) continue;
+        const int m = n - k - 1;
+        const double* vr = t.vre.data() + t.voff[k];
+        const double* vi = t.vim.data() + t.voff[k];
+        double sr = 0.0, si = 0.0;  // s = v^H x
+        for (int i = 0; i < m; ++i) {
+            const cd xi = x[k + 1 + i];
+            sr += vr[i] * xi.real() + vi[i] * xi.imag();
+            si += vr[i] * xi.imag() - vi[i] * xi.real();
+        }
+        sr *= 2.0;
+        si *= 2.0;
+        for (int i = 0; i < m; ++i) {
+            x[k + 1 + i] -= cd(sr * vr[i] - si * vi[i], sr * vi[i] + si * vr[i]);
+        }
+    }
+}
+
+// Eigenvector of the real symmetric tridiagonal (d, e) for eigenvalue lam by inverse
+// iteration; z is orthogonalised against the columns of prev (a cluster).
+inline void inverse_iteration(const std::vector<double>& d, const std::vector<double>& e, double lam, double tnorm,
+                              const std::vector<const double*>& prev, unsigned seed, double* z) {
+    const int n = static_cast<int>(d.size());
+    const double eps = DBL_EPSILON * std::max(tnorm, DBL_MIN);
+    // LU with partial pivoting of T - lam I: rows of U hold (u0, u1, u2) on the diagonal
+    // and two super-diagonals; l[i] multiplier; piv[i] whether rows i, i+1 were swapped.
+    std::vector<double> u0(n), u1(n, 0.0), u2(n, 0.0), lm(n, 0.0);
+    std::vector<char> piv(n, 0);
+    auto guard = [&](double x) { return (std::fabs(x) < eps) ? (x < 0.0 ? -eps : eps) : x; };
+    {
+        // pending row: alpha at column i, beta at column i+1
+        double alpha = d[0] - lam, beta = (n > 1) ? e[0] : 0.0;
+        for (int i = 0; i < n - 1; ++i) {
+            const double sub = e[i];           // T[i+1, i]
+            const double nd = d[i + 1] - lam;  // T[i+1, i+1]
+            const double ns = (i + 2 < n) ? e[i + 1] : 0.0;
+            if (std::fabs(alpha) >= std::fabs(sub)) {
+                alpha = guard(alpha);
+                const double mult = sub / alpha;
+                lm[i] = mult;
+                u0[i] = alpha;
+                u1[i] = beta;
+                u2[i] = 0.0;
+                alpha = nd - mult * beta;
+                beta = ns;
+            } else {
+                const double mult = alpha / sub;
+                lm[i] = mult;
+                piv[i] = 1;
+                u0[i] = sub;
+                u1[i] = nd;
+                u2[i] = ns;
+                alpha = beta - mult * nd;
+                beta = -mult * ns;
+            }
+        }
+        u0[n - 1] = guard(alpha);
+    }
+    // start vector: fixed pseudo-random
+    unsigned s = seed * 2654435761u + 12345u;
+    for (int i = 0; i < n; ++i) {
+        s = s * 1664525u + 1013904223u;
+        z[i] = 0.5 + (s >> 8) * (1.0 / 16777216.0);
+    }
+    std::vector<double> y(n);
+    for (int it = 0; it < 4; ++it) {
+        // solve L y = z (apply the row operations), then U x = y
+        for (int i = 0; i < n; ++i) y[i] = z[i];
+        for (int i = 0; i < n - 1; ++i) {
+            if (piv[i]) std::swap(y[i], y[i + 1]);
+            y[i + 1] -= lm[i] * y[i];
+        }
+        for (int i = n - 1; i >= 0; --i) {
+            double v = y[i];
+            if (i + 1 < n) v -= u1[i] * z[i + 1];
+            if (i + 2 < n) v -= u2[i] * z[i + 2];
+            z[i] = v / u0[i];
+        }
+        for (const double* p : prev) {
+            double dot = 0.0;
+            for (int i = 0; i < n; ++i) dot += p[i] * z[i];
+            for (int i = 0; i < n; ++i) z[i] -= dot * p[i];
+        }
+        double nrm = 0.0;
+        for (int i = 0; i < n; ++i) nrm += z[i] * z[i];
+        nrm = std::sqrt(nrm);
+        if (!(nrm > 0.0)) {
+            for (int i = 0; i < n; ++i) z[i] = (i == static_cast<int>(seed % n)) ? 1.0 : 0.0;
+            nrm = 1.0;
+        }
+        for (int i = 0; i < n; ++i) z[i] /= nrm;
+    }
+}
+
+}  // namespace detail
+
+// Eigen-decomposition of the Hermitian n x n matrix A (column-major, leading
+// dimension lda; only the lower triangle and the real part of the diagonal are
+// read). On return w[0..n) holds the eigenvalues in ascending order and, if Z
+// is non-null, Z (column-major, leading dimension n) holds the orthonormal
+// eigenvectors as columns.
+inline void heev(int n, const cd* Ain, int lda, double* w, cd* Z) {
+    if (n <= 0) return;
+    detail::Tridiag t;
+    detail::tridiagonalize(n, Ain, lda, t);
+    std::vector<double> d = t.d;
+    std::vector<double> zr;
+    if (Z) {
+        zr.assign(static_cast<size_t>(n) * n, 0.0);
+        for (int i = 0; i < n; ++i) zr[i + static_cast<size_t>(i) * n] = 1.0;
+    }
+    detail::tql(n, d, t.e, Z ? zr.data() : nullptr);
     std::vector<int> order(n);
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return d[a] < d[b]; });
     for (int i = 0; i < n; ++i) w[i] = d[order[i]];
-    if (!want) return;
+    if (!Z) return;
+    for (int c = 0; c < n; ++c) detail::back_transform(t, zr.data() + static_cast<size_t>(order[c]) * n, Z + static_cast<size_t>(c) * n);
+}
 
-    // Eigenvectors: x = H_0 H_1 ... H_{n-3} Ph z.
-    for (int col = 0; col < n; ++col) {
-        const double* zc = zr.data() + static_cast<size_t>(order[col]) * n;
-        cd* x = Z + static_cast<size_t>(col) * n;
-        for (int i = 0; i < n; ++i) x[i] = ph[i] * zc[i];
-        for (int k = n - 3; k >= 0; --k) {
-            if (!vused[k]) continue;
-            const int m = n - k - 1;
-            const cd* v = vstore.data() + voff[k];
-            cd s(0.0, 0.0);
-            for (int i = 0; i < m; ++i) s += std::conj(v[i]) * x[k + 1 + i];
-            s *= 2.0;
-            for (int i = 0; i < m; ++i) x[k + 1 + i] -= s * v[i];
-        }
+// All eigenvalues (ascending, w[0..n)) and the eigenvectors of the nwant lowest
+// (Z: n x nwant column-major).
+inline void heev_lowest(int n, const cd* Ain, int lda, int nwant, double* w, cd* Z) {
+    if (n <= 0) return;
+    nwant = std::min(nwant, n);
+    detail::Tridiag t;
+    detail::tridiagonalize(n, Ain, lda, t);
+    std::vector<double> d = t.d;
+    detail::tql(n, d, t.e, nullptr);
+    std::sort(d.begin(), d.end());
+    for (int i = 0; i < n; ++i) w[i] = d[i];
+    double tnorm = 0.0;
+    for (int i = 0; i < n; ++i) tnorm = std::max(tnorm, std::fabs(t.d[i]) + std::fabs(t.e[i]) + (i ? std::fabs(t.e[i - 1]) : 0.0));
+    const double cluster = 1e-3 * std::max(tnorm, 1e-300);
+    std::vector<double> zs(static_cast<size_t>(n) * nwant);
+    int cstart = 0;
+    for (int i = 0; i < nwant; ++i) {
+        if (i > 0 && w[i] - w[i - 1] > cluster) cstart = i;
+        std::vector<const double*> prev;
+        for (int j = cstart; j < i; ++j) prev.push_back(zs.data() + static_cast<size_t>(j) * n);
+        // separate coincident shifts inside a cluster slightly, as LAPACK stein does
+        double lam = w[i];
+        if (i > cstart && lam - w[i - 1] < 10.0 * DBL_EPSILON * tnorm) lam = w[i - 1] + 10.0 * DBL_EPSILON * tnorm;
+        detail::inverse_iteration(t.d, t.e, lam, tnorm, prev, static_cast<unsigned>(i + 1),
+                                  zs.data() + static_cast<size_t>(i) * n);
     }
+    for (int c = 0; c < nwant; ++c) detail::back_transform(t, zs.data() + static_cast<size_t>(c) * n, Z + static_cast<size_t>(c) * n);
 }
 
 }  // namespace xatu_linalg

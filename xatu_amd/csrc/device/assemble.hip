@@ -82,57 +82,43 @@ __device__ inline void store_h(cx_t<R>* H, int64_t idx, cx_t<R> v) {
     H[idx] = v;
 }
 
-// Structured tile kernel. R: arithmetic type of the assembly (double or float).
-template <typename R, int NA, int NC, int NV, int TK>
-__global__ __launch_bounds__(TK* TK) void bse_tile_kernel(LatticeDesc lat, int nk, const double2* __restrict__ evk,
-                                                          const double2* __restrict__ evkq,
-                                                          const double* __restrict__ eval_k,
-                                                          const double* __restrict__ eval_kq,
-                                                          const int* __restrict__ kcoord,
-                                                          const double2* __restrict__ ft,
-                                                          const double2* __restrict__ e1,
-                                                          const double2* __restrict__ e2, int exchange,
-                                                          double scissor, cx_t<R>* __restrict__ H) {
-    using C = cx_t<R>;
+// Stage the coefficient vectors of TK row k-points (slots [0,TK)) and TK column
+// k-points (slots [TK,2TK)) in LDS: per slot [norb][NB], conduction columns first
+// (from the k+Q stack), then valence columns (from the k stack).
+template <typename R, int NC, int NV, int TK>
+__device__ inline void stage_coefficients(cx_t<R>* s_coef, int norb, int nk, int Kr, int Kc,
+                                          const double2* __restrict__ evk, const double2* __restrict__ evkq,
+                                          int nthreads) {
     constexpr int NB = NC + NV;
-    constexpr int NCV = NC * NV;
-    extern __shared__ unsigned char smem_raw[];
-    C* s_coef = reinterpret_cast<C*>(smem_raw);  // [2*TK][norb][NB]: conduction first, then valence
-    const int norb = lat.norb;
-    const int Kr = blockIdx.x, Kc = blockIdx.y;
-    const int t = threadIdx.x;
-
-    // stage coefficients of the TK row k-points and the TK column k-points
     const int total = 2 * TK * norb * NB;
-    for (int e = t; e < total; e += TK * TK) {
+    for (int e = threadIdx.x; e < total; e += nthreads) {
         const int al = e % norb;
         const int rest = e / norb;
         const int band = rest % NB;
         const int slot = rest / NB;
         const int k = (slot < TK) ? (Kr * TK + slot) : (Kc * TK + slot - TK);
-        C val = mk(R(0), R(0));
+        cx_t<R> val = mk(R(0), R(0));
         if (k < nk) {
-            // band < NC: conduction column NV+band of the k+Q stack; else valence column band-NC of the k stack
-            const double2 g = (band < NC)
-                                  ? evkq[(static_cast<size_t>(k) * NB + (NV + band)) * norb + al]
-                                  : evk[(static_cast<size_t>(k) * NB + (band - NC)) * norb + al];
+            const double2 g = (band < NC) ? evkq[(static_cast<size_t>(k) * NB + (NV + band)) * norb + al]
+                                          : evk[(static_cast<size_t>(k) * NB + (band - NC)) * norb + al];
             val = to_cx<R>(g);
         }
         s_coef[(static_cast<size_t>(slot) * norb + al) * NB + band] = val;
     }
-    __syncthreads();
+}
 
-    const int tr = t % TK, tc = t / TK;
-    const int kr = Kr * TK + tr, kc = Kc * TK + tc;
-    if (kr >= nk || kc >= nk) return;
-    const bool sw = kr > kc;
-    const int ka = sw ? kc : kr;
-    const int kb = sw ? kr : kc;
-    const int sa = sw ? (TK + tc) : tr;
-    const int sb = sw ? tr : (TK + tc);
-    const C* ca_ = s_coef + static_cast<size_t>(sa) * norb * NB;
-    const C* cb_ = s_coef + static_cast<size_t>(sb) * norb * NB;
-
+// The nc*nv x nc*nv block of (D - X) for the k-point pair (ka, kb), ka <= kb, from the
+// staged coefficients ca_ (ka) and cb_ (kb). sink(cA, vA, cB, vB, DX) receives every
+// element. The instruction sequence depends only on the data, so any two threads
+// evaluating the same pair produce bit-identical values.
+template <typename R, int NA, int NC, int NV, typename Sink>
+__device__ inline void pair_block(const LatticeDesc& lat, const cx_t<R>* ca_, const cx_t<R>* cb_, int ka, int kb,
+                                  const double2* __restrict__ ft, const int* __restrict__ kcoord,
+                                  const double2* __restrict__ e1, const double2* __restrict__ e2, int exchange,
+                                  Sink&& sink) {
+    using C = cx_t<R>;
+    constexpr int NB = NC + NV;
+    constexpr int NCV = NC * NV;
     const int q = mesh_index(kcoord, ka, kb, lat.ndim, lat.mesh_n);
     C M[NA][NA];
 #pragma unroll
@@ -157,9 +143,6 @@ __global__ __launch_bounds__(TK* TK) void bse_tile_kernel(LatticeDesc lat, int n
             }
         }
     }
-
-    const int64_t N = static_cast<int64_t>(nk) * NCV;
-    const bool diagblk = (ka == kb);
 #pragma unroll
     for (int cA = 0; cA < NC; ++cA) {
         // rho^c_a(cB) = sum_{alpha in a} conj(C_ka[alpha, cA]) C_kb[alpha, cB]
@@ -206,27 +189,226 @@ __global__ __launch_bounds__(TK* TK) void bse_tile_kernel(LatticeDesc lat, int n
                         for (int a = 0; a < NA; ++a)
                             cfma(X, ex1[a], to_cx<R>(e2[(static_cast<size_t>(kb) * NA + a) * NCV + cB * NV + vB]));
                     }
-                    const C DX = csub(D, X);
-                    const int ia = cA * NV + vA, ib = cB * NV + vB;
-                    const int64_t i = static_cast<int64_t>(ka) * NCV + ia;
-                    const int64_t j = static_cast<int64_t>(kb) * NCV + ib;
-                    if (diagblk) {
-                        if (ia < ib) {
-                            const C f = mk(-DX.x, -DX.y);
-                            H[i + j * N] = f;
-                            H[j + i * N] = cconj(f);
-                        } else if (ia == ib) {
-                            const double e = (scissor + eval_kq[static_cast<size_t>(ka) * NB + NV + cA]) -
-                                             eval_k[static_cast<size_t>(ka) * NB + vA];
-                            const double h = e / 2. - static_cast<double>(DX.x) / 2.;
-                            H[i + i * N] = mk(R(h + h), R(0));
-                        }
-                    } else if (!sw) {
-                        H[i + j * N] = mk(-DX.x, -DX.y);
-                    } else {
-                        H[j + i * N] = mk(-DX.x, DX.y);
+                    sink(cA, vA, cB, vB, csub(D, X));
+                }
+        }
+    }
+}
+
+// Diagonal element: scissor + E_c(k+Q) - E_v(k) - Re(D - X), computed as the reference
+// does, ((scissor + Ec - Ev)/2 - Re(D - X)/2) doubled.
+__device__ inline double diag_value(const double* __restrict__ eval_k, const double* __restrict__ eval_kq, int nb,
+                                    int nv, int k, int c, int v, double scissor, double dx_re) {
+    const double e = (scissor + eval_kq[static_cast<size_t>(k) * nb + nv + c]) - eval_k[static_cast<size_t>(k) * nb + v];
+    const double h = e / 2. - dx_re / 2.;
+    return h + h;
+}
+
+// Structured build kernel: tile of TK x TK k-point pairs, one pair per thread.
+// R: arithmetic type of the assembly (double or float).
+template <typename R, int NA, int NC, int NV, int TK>
+__global__ __launch_bounds__(TK* TK) void bse_tile_kernel(LatticeDesc lat, int nk, const double2* __restrict__ evk,
+                                                          const double2* __restrict__ evkq,
+                                                          const double* __restrict__ eval_k,
+                                                          const double* __restrict__ eval_kq,
+                                                          const int* __restrict__ kcoord,
+                                                          const double2* __restrict__ ft,
+                                                          const double2* __restrict__ e1,
+                                                          const double2* __restrict__ e2, int exchange,
+                                                          double scissor, cx_t<R>* __restrict__ H) {
+    using C = cx_t<R>;
+    constexpr int NB = NC + NV;
+    constexpr int NCV = NC * NV;
+    extern __shared__ unsigned char smem_raw[];
+    C* s_coef = reinterpret_cast<C*>(smem_raw);
+    const int norb = lat.norb;
+    const int Kr = blockIdx.x, Kc = blockIdx.y;
+    stage_coefficients<R, NC, NV, TK>(s_coef, norb, nk, Kr, Kc, evk, evkq, TK * TK);
+    __syncthreads();
+
+    const int t = threadIdx.x;
+    const int tr = t % TK, tc = t / TK;
+    const int kr = Kr * TK + tr, kc = Kc * TK + tc;
+    if (kr >= nk || kc >= nk) return;
+    const bool sw = kr > kc;
+    const int ka = sw ? kc : kr;
+    const int kb = sw ? kr : kc;
+    const C* ca_ = s_coef + static_cast<size_t>(sw ? (TK + tc) : tr) * norb * NB;
+    const C* cb_ = s_coef + static_cast<size_t>(sw ? tr : (TK + tc)) * norb * NB;
+    const int64_t N = static_cast<int64_t>(nk) * NCV;
+    const bool diagblk = (ka == kb);
+    pair_block<R, NA, NC, NV>(lat, ca_, cb_, ka, kb, ft, kcoord, e1, e2, exchange,
+                              [&](int cA, int vA, int cB, int vB, C DX) {
+                                  const int ia = cA * NV + vA, ib = cB * NV + vB;
+                                  const int64_t i = static_cast<int64_t>(ka) * NCV + ia;
+                                  const int64_t j = static_cast<int64_t>(kb) * NCV + ib;
+                                  if (diagblk) {
+                                      if (ia < ib) {
+                                          const C f = mk(-DX.x, -DX.y);
+                                          H[i + j * N] = f;
+                                          H[j + i * N] = cconj(f);
+                                      } else if (ia == ib) {
+                                          const double h = diag_value(eval_k, eval_kq, NB, NV, ka, cA, vA, scissor,
+                                                                      static_cast<double>(DX.x));
+                                          H[i + i * N] = mk(R(h), R(0));
+                                      }
+                                  } else if (!sw) {
+                                      H[i + j * N] = mk(-DX.x, -DX.y);
+                                  } else {
+                                      H[j + i * N] = mk(-DX.x, DX.y);
+                                  }
+                              });
+}
+
+// Diagonal of H only (Davidson preconditioner without a resident matrix): the
+// diagonal tiles of the build grid, where the threads with k_row == k_col evaluate
+// their (k, k) block.
+template <typename R, int NA, int NC, int NV, int TK>
+__global__ __launch_bounds__(TK* TK) void bse_diag_kernel(LatticeDesc lat, int nk, const double2* __restrict__ evk,
+                                                          const double2* __restrict__ evkq,
+                                                          const double* __restrict__ eval_k,
+                                                          const double* __restrict__ eval_kq,
+                                                          const int* __restrict__ kcoord,
+                                                          const double2* __restrict__ ft,
+                                                          const double2* __restrict__ e1,
+                                                          const double2* __restrict__ e2, int exchange,
+                                                          double scissor, double* __restrict__ diag) {
+    using C = cx_t<R>;
+    constexpr int NB = NC + NV;
+    constexpr int NCV = NC * NV;
+    extern __shared__ unsigned char smem_raw[];
+    C* s_coef = reinterpret_cast<C*>(smem_raw);
+    const int norb = lat.norb;
+    const int K = blockIdx.x;
+    stage_coefficients<R, NC, NV, TK>(s_coef, norb, nk, K, K, evk, evkq, TK * TK);
+    __syncthreads();
+    const int t = threadIdx.x;
+    const int tr = t % TK, tc = t / TK;
+    const int k = K * TK + tr;
+    if (tr != tc || k >= nk) return;
+    const C* ca_ = s_coef + static_cast<size_t>(tr) * norb * NB;
+    const C* cb_ = s_coef + static_cast<size_t>(TK + tc) * norb * NB;
+    pair_block<R, NA, NC, NV>(lat, ca_, cb_, k, k, ft, kcoord, e1, e2, exchange,
+                              [&](int cA, int vA, int cB, int vB, C DX) {
+                                  if (cA == cB && vA == vB) {
+                                      diag[static_cast<int64_t>(k) * NCV + cA * NV + vA] =
+                                          diag_value(eval_k, eval_kq, NB, NV, k, cA, vA, scissor,
+                                                     static_cast<double>(DX.x));
+                                  }
+                              });
+}
+
+// Matrix-free product Y = H X: one workgroup owns TK row k-points and sweeps every
+// column tile; each tile of H (TK*NCV x TK*NCV) is recomputed once into LDS and
+// applied to up to kMaxVec vectors, staged kVecChunk at a time.
+constexpr int kVecChunk = 8;
+constexpr int kMaxVec = 32;
+
+template <typename R, int NA, int NC, int NV, int TK>
+__global__ __launch_bounds__(TK* TK) void bse_matfree_kernel(LatticeDesc lat, int nk, const double2* __restrict__ evk,
+                                                             const double2* __restrict__ evkq,
+                                                             const double* __restrict__ eval_k,
+                                                             const double* __restrict__ eval_kq,
+                                                             const int* __restrict__ kcoord,
+                                                             const double2* __restrict__ ft,
+                                                             const double2* __restrict__ e1,
+                                                             const double2* __restrict__ e2, int exchange,
+                                                             double scissor, const cx_t<R>* __restrict__ X,
+                                                             cx_t<R>* __restrict__ Y, int nvec) {
+    using C = cx_t<R>;
+    constexpr int NB = NC + NV;
+    constexpr int NCV = NC * NV;
+    constexpr int ROWS = TK * NCV;
+    constexpr int NT = TK * TK;
+    constexpr int NCHUNK = kMaxVec / kVecChunk;
+    constexpr int PER = (ROWS * kVecChunk + NT - 1) / NT;
+    extern __shared__ unsigned char smem_raw[];
+    const int norb = lat.norb;
+    C* s_coef = reinterpret_cast<C*>(smem_raw);                  // 2*TK*norb*NB
+    C* s_h = s_coef + static_cast<size_t>(2 * TK) * norb * NB;   // ROWS x ROWS, row-major [r][c]
+    C* s_x = s_h + ROWS * ROWS;                                  // ROWS x kVecChunk
+    const int64_t N = static_cast<int64_t>(nk) * NCV;
+    const int Kr = blockIdx.x;
+    const int nT = (nk + TK - 1) / TK;
+    const int t = threadIdx.x;
+    const int tr = t % TK, tc = t / TK;
+    const int nchunk = (nvec + kVecChunk - 1) / kVecChunk;
+
+    C acc[NCHUNK][PER];
+#pragma unroll
+    for (int q = 0; q < NCHUNK; ++q)
+#pragma unroll
+        for (int m = 0; m < PER; ++m) acc[q][m] = mk(R(0), R(0));
+
+    for (int Kc = 0; Kc < nT; ++Kc) {
+        __syncthreads();
+        stage_coefficients<R, NC, NV, TK>(s_coef, norb, nk, Kr, Kc, evk, evkq, NT);
+        for (int e = t; e < ROWS * ROWS; e += NT) s_h[e] = mk(R(0), R(0));
+        __syncthreads();
+        const int kr = Kr * TK + tr, kc = Kc * TK + tc;
+        if (kr < nk && kc < nk) {
+            const bool sw = kr > kc;
+            const int ka = sw ? kc : kr;
+            const int kb = sw ? kr : kc;
+            const C* ca_ = s_coef + static_cast<size_t>(sw ? (TK + tc) : tr) * norb * NB;
+            const C* cb_ = s_coef + static_cast<size_t>(sw ? tr : (TK + tc)) * norb * NB;
+            const bool diagblk = (ka == kb);
+            pair_block<R, NA, NC, NV>(lat, ca_, cb_, ka, kb, ft, kcoord, e1, e2, exchange,
+                                      [&](int cA, int vA, int cB, int vB, C DX) {
+                                          const int ia = cA * NV + vA, ib = cB * NV + vB;
+                                          if (diagblk) {
+                                              const int base = tr * NCV;
+                                              if (ia < ib) {
+                                                  s_h[(base + ia) * ROWS + tc * NCV + ib] = mk(-DX.x, -DX.y);
+                                                  s_h[(base + ib) * ROWS + tc * NCV + ia] = mk(-DX.x, DX.y);
+                                              } else if (ia == ib) {
+                                                  const double h = diag_value(eval_k, eval_kq, NB, NV, ka, cA, vA,
+                                                                              scissor, static_cast<double>(DX.x));
+                                                  s_h[(base + ia) * ROWS + tc * NCV + ia] = mk(R(h), R(0));
+                                              }
+                                          } else if (!sw) {
+                                              s_h[(tr * NCV + ia) * ROWS + tc * NCV + ib] = mk(-DX.x, -DX.y);
+                                          } else {
+                                              s_h[(tr * NCV + ib) * ROWS + tc * NCV + ia] = mk(-DX.x, DX.y);
+                                          }
+                                      });
+        }
+#pragma unroll
+        for (int q = 0; q < NCHUNK; ++q) {
+            if (q < nchunk) {
+                __syncthreads();
+                for (int e = t; e < ROWS * kVecChunk; e += NT) {
+                    const int r = e % ROWS, j = e / ROWS;
+                    const int64_t row = static_cast<int64_t>(Kc) * ROWS + r;
+                    const int vj = q * kVecChunk + j;
+                    s_x[r * kVecChunk + j] = (row < N && vj < nvec) ? X[row + static_cast<int64_t>(vj) * N]
+                                                                    : mk(R(0), R(0));
+                }
+                __syncthreads();
+#pragma unroll
+                for (int m = 0; m < PER; ++m) {
+                    const int e = t + NT * m;
+                    if (e < ROWS * kVecChunk) {
+                        const int r = e % ROWS, j = e / ROWS;
+                        C sum = acc[q][m];
+                        for (int cidx = 0; cidx < ROWS; ++cidx) cfma(sum, s_h[r * ROWS + cidx], s_x[cidx * kVecChunk + j]);
+                        acc[q][m] = sum;
                     }
                 }
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NCHUNK; ++q) {
+#pragma unroll
+        for (int m = 0; m < PER; ++m) {
+            const int e = t + NT * m;
+            if (q < nchunk && e < ROWS * kVecChunk) {
+                const int r = e % ROWS, j = e / ROWS;
+                const int64_t row = static_cast<int64_t>(Kr) * ROWS + r;
+                const int vj = q * kVecChunk + j;
+                if (row < N && vj < nvec) Y[row + static_cast<int64_t>(vj) * N] = acc[q][m];
+            }
         }
     }
 }
@@ -324,39 +506,104 @@ __global__ void compare_kernel(const double2* __restrict__ H64, const float2* __
     if (threadIdx.x == 0) out[bi + static_cast<size_t>(nblk) * bj] = s[0];
 }
 
+template <typename R, int TK>
+size_t coef_bytes(const Context& c, int nb) {
+    return static_cast<size_t>(2 * TK) * c.lat.norb * nb * sizeof(cx_t<R>);
+}
+
+// Launchers for one (natoms, nc, nv) shape.
 template <typename R, int NA, int NC, int NV>
-bool try_launch(const Context& c, cx_t<R>* H, hipStream_t s) {
-    if (c.lat.natoms != NA || c.nc != NC || c.nv != NV) return false;
-    constexpr int NB = NC + NV;
-    const size_t bytes16 = static_cast<size_t>(2 * 16) * c.lat.norb * NB * sizeof(cx_t<R>);
-    if (bytes16 <= 64 * 1024) {
-        constexpr int TK = 16;
+struct Shape {
+    static constexpr int NB = NC + NV;
+    static constexpr int NCV = NC * NV;
+    static bool matches(const Context& c) { return c.lat.natoms == NA && c.nc == NC && c.nv == NV; }
+
+    template <int TK>
+    static void build_tk(const Context& c, cx_t<R>* H, hipStream_t s) {
         const int nT = (c.nk + TK - 1) / TK;
-        hipLaunchKernelGGL((bse_tile_kernel<R, NA, NC, NV, TK>), dim3(nT, nT), dim3(TK * TK), bytes16, s, c.lat, c.nk,
-                           c.d_evec_k.p, c.d_evec_kq.p, c.d_eval_k.p, c.d_eval_kq.p, c.d_kcoord.p, c.d_ft.p,
-                           c.d_e1.p, c.d_e2.p, c.inter.exchange, c.scissor, H);
-    } else {
-        constexpr int TK = 8;
-        const size_t bytes8 = static_cast<size_t>(2 * TK) * c.lat.norb * NB * sizeof(cx_t<R>);
-        if (bytes8 > 160 * 1024) throw Error(XATU_ERR_INVALID_ARGUMENT, "too many orbitals for the tile kernel");
-        const int nT = (c.nk + TK - 1) / TK;
-        hipLaunchKernelGGL((bse_tile_kernel<R, NA, NC, NV, TK>), dim3(nT, nT), dim3(TK * TK), bytes8, s, c.lat, c.nk,
-                           c.d_evec_k.p, c.d_evec_kq.p, c.d_eval_k.p, c.d_eval_kq.p, c.d_kcoord.p, c.d_ft.p,
-                           c.d_e1.p, c.d_e2.p, c.inter.exchange, c.scissor, H);
+        const size_t lds = coef_bytes<R, TK>(c, NB);
+        hipLaunchKernelGGL((bse_tile_kernel<R, NA, NC, NV, TK>), dim3(nT, nT), dim3(TK * TK), lds, s, c.lat, c.nk, c.d_evec_k.p, c.d_evec_kq.p, c.d_eval_k.p, c.d_eval_kq.p, c.d_kcoord.p,
+                           c.d_ft.p, c.d_e1.p, c.d_e2.p, c.inter.exchange, c.scissor, H);
     }
-    XATU_HIP(hipGetLastError());
-    return true;
-}
+    static void build(const Context& c, cx_t<R>* H, hipStream_t s) {
+        if (coef_bytes<R, 16>(c, NB) <= 64 * 1024) {
+            build_tk<16>(c, H, s);
+        } else if (coef_bytes<R, 8>(c, NB) <= 160 * 1024) {
+            build_tk<8>(c, H, s);
+        } else {
+            throw Error(XATU_ERR_INVALID_ARGUMENT, "too many orbitals for the tile kernel");
+        }
+        XATU_HIP(hipGetLastError());
+    }
 
-template <typename R, int NA>
-bool try_atoms(const Context& c, cx_t<R>* H, hipStream_t s) {
-    return try_launch<R, NA, 1, 1>(c, H, s) || try_launch<R, NA, 2, 2>(c, H, s) || try_launch<R, NA, 3, 3>(c, H, s);
-}
+    static void diag(const Context& c, double* d, hipStream_t s) {
+        constexpr int TK = 8;
+        if (coef_bytes<R, TK>(c, NB) > 160 * 1024) throw Error(XATU_ERR_INVALID_ARGUMENT, "too many orbitals");
+        const int nT = (c.nk + TK - 1) / TK;
+        const size_t lds = coef_bytes<R, TK>(c, NB);
+        hipLaunchKernelGGL((bse_diag_kernel<R, NA, NC, NV, TK>), dim3(nT), dim3(TK * TK), lds, s, c.lat, c.nk, c.d_evec_k.p, c.d_evec_kq.p, c.d_eval_k.p, c.d_eval_kq.p, c.d_kcoord.p,
+                           c.d_ft.p, c.d_e1.p, c.d_e2.p, c.inter.exchange, c.scissor, d);
+        XATU_HIP(hipGetLastError());
+    }
 
-template <typename R>
-bool dispatch_tiles(const Context& c, cx_t<R>* H, hipStream_t s) {
-    return try_atoms<R, 1>(c, H, s) || try_atoms<R, 2>(c, H, s) || try_atoms<R, 3>(c, H, s) ||
-           try_atoms<R, 4>(c, H, s);
+    template <int TK>
+    static size_t matfree_bytes(const Context& c) {
+        constexpr int ROWS = TK * NCV;
+        return coef_bytes<R, TK>(c, NB) + static_cast<size_t>(ROWS) * ROWS * sizeof(cx_t<R>) +
+               static_cast<size_t>(ROWS) * kVecChunk * sizeof(cx_t<R>);
+    }
+    template <int TK>
+    static void matfree_tk(const Context& c, const cx_t<R>* X, cx_t<R>* Y, int nvec, hipStream_t s) {
+        const int nT = (c.nk + TK - 1) / TK;
+        const size_t lds = matfree_bytes<TK>(c);
+        for (int v0 = 0; v0 < nvec; v0 += kMaxVec) {
+            const int nv = std::min(kMaxVec, nvec - v0);
+            matfree_launch<TK>(c, X + static_cast<int64_t>(v0) * c.N, Y + static_cast<int64_t>(v0) * c.N, nv, nT,
+                               lds, s);
+        }
+    }
+    template <int TK>
+    static void matfree_launch(const Context& c, const cx_t<R>* X, cx_t<R>* Y, int nvec, int nT, size_t lds,
+                               hipStream_t s) {
+        hipLaunchKernelGGL((bse_matfree_kernel<R, NA, NC, NV, TK>), dim3(nT), dim3(TK * TK), lds, s, c.lat, c.nk, c.d_evec_k.p, c.d_evec_kq.p, c.d_eval_k.p, c.d_eval_kq.p, c.d_kcoord.p,
+                           c.d_ft.p, c.d_e1.p, c.d_e2.p, c.inter.exchange, c.scissor, X, Y, nvec);
+    }
+    static void matfree(const Context& c, const cx_t<R>* X, cx_t<R>* Y, int nvec, hipStream_t s) {
+        if (matfree_bytes<16>(c) <= 160 * 1024) {
+            matfree_tk<16>(c, X, Y, nvec, s);
+        } else if (matfree_bytes<8>(c) <= 160 * 1024) {
+            matfree_tk<8>(c, X, Y, nvec, s);
+        } else if (matfree_bytes<4>(c) <= 160 * 1024) {
+            matfree_tk<4>(c, X, Y, nvec, s);
+        } else {
+            throw Error(XATU_ERR_INVALID_ARGUMENT, "too many orbitals for the matrix-free kernel");
+        }
+        XATU_HIP(hipGetLastError());
+    }
+};
+
+// Call fn(Shape<R, NA, NC, NV>{}) for the context's shape; false if none matches.
+template <typename R, typename Fn>
+bool for_shape(const Context& c, Fn&& fn) {
+#define XATU_SHAPE(NA, NC, NV)                 \
+    if (Shape<R, NA, NC, NV>::matches(c)) {    \
+        fn(Shape<R, NA, NC, NV>{});            \
+        return true;                           \
+    }
+    XATU_SHAPE(1, 1, 1)
+    XATU_SHAPE(1, 2, 2)
+    XATU_SHAPE(1, 3, 3)
+    XATU_SHAPE(2, 1, 1)
+    XATU_SHAPE(2, 2, 2)
+    XATU_SHAPE(2, 3, 3)
+    XATU_SHAPE(3, 1, 1)
+    XATU_SHAPE(3, 2, 2)
+    XATU_SHAPE(3, 3, 3)
+    XATU_SHAPE(4, 1, 1)
+    XATU_SHAPE(4, 2, 2)
+    XATU_SHAPE(4, 3, 3)
+#undef XATU_SHAPE
+    return false;
 }
 
 }  // namespace
@@ -391,7 +638,7 @@ void launch_build_basis(const Context& c, int nbasis, const int* d_basis, double
 }
 
 void launch_build_tiles64(const Context& c, double2* H, hipStream_t s) {
-    if (dispatch_tiles<double>(c, H, s)) return;
+    if (for_shape<double>(c, [&](auto sh) { decltype(sh)::build(c, H, s); })) return;
     // Shapes without a specialised tile kernel use the generic element kernel.
     std::vector<int> basis;
     full_basis(c, basis);
@@ -403,10 +650,31 @@ void launch_build_tiles64(const Context& c, double2* H, hipStream_t s) {
     XATU_HIP(hipStreamSynchronize(s));
 }
 
+static const char* kShapeMsg =
+    "this operation supports natoms <= 4 and nv == nc <= 3 (the FP64 build covers other shapes)";
+
 void launch_build_tiles32(const Context& c, float2* H, hipStream_t s) {
-    if (dispatch_tiles<float>(c, H, s)) return;
-    throw Error(XATU_ERR_INVALID_ARGUMENT,
-                "FP32 assembly supports natoms <= 4 and nv == nc <= 3 (use the FP64 build for other shapes)");
+    if (!for_shape<float>(c, [&](auto sh) { decltype(sh)::build(c, H, s); })) {
+        throw Error(XATU_ERR_INVALID_ARGUMENT, kShapeMsg);
+    }
+}
+
+void launch_diag_direct(const Context& c, double* diag, hipStream_t s) {
+    if (!for_shape<double>(c, [&](auto sh) { decltype(sh)::diag(c, diag, s); })) {
+        throw Error(XATU_ERR_INVALID_ARGUMENT, kShapeMsg);
+    }
+}
+
+void launch_matfree64(const Context& c, const double2* X, double2* Y, int nvec, hipStream_t s) {
+    if (!for_shape<double>(c, [&](auto sh) { decltype(sh)::matfree(c, X, Y, nvec, s); })) {
+        throw Error(XATU_ERR_INVALID_ARGUMENT, kShapeMsg);
+    }
+}
+
+void launch_matfree32(const Context& c, const float2* X, float2* Y, int nvec, hipStream_t s) {
+    if (!for_shape<float>(c, [&](auto sh) { decltype(sh)::matfree(c, X, Y, nvec, s); })) {
+        throw Error(XATU_ERR_INVALID_ARGUMENT, kShapeMsg);
+    }
 }
 
 void launch_diag_extract(const Context& c, const double2* H, double* diag, hipStream_t s) {

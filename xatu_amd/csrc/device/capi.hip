@@ -78,25 +78,24 @@ PotDesc make_pot(const Context& c, int kind) {
     return p;
 }
 
-template <typename T>
-__global__ void diag_from_kernel(const T* __restrict__ H, int64_t N, double* __restrict__ diag) {
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < N) diag[i] = static_cast<double>(H[i + i * N].x);
-}
-
-void refresh_diag(Context& c, int precision) {
-    c.d_diag.alloc(c.N);
-    if (precision == XATU_FP64) {
-        hipLaunchKernelGGL(diag_from_kernel<double2>, dim3((c.N + 255) / 256), dim3(256), 0, c.stream, c.d_H64.p,
-                           c.N, c.d_diag.p);
+// Exchange factors and the FP64 diagonal (Davidson preconditioner), needed by
+// every product whether or not a matrix is resident.
+void prepare_operator(Context& c) {
+    if (c.inter.exchange) {
+        const size_t ne = static_cast<size_t>(c.nk) * c.lat.natoms * c.nc * c.nv;
+        c.d_e1.alloc(ne);
+        c.d_e2.alloc(ne);
+        launch_exchange_factors(c, c.stream);
     } else {
-        hipLaunchKernelGGL(diag_from_kernel<float2>, dim3((c.N + 255) / 256), dim3(256), 0, c.stream, c.d_H32.p,
-                           c.N, c.d_diag.p);
+        c.d_e1.alloc(1);
+        c.d_e2.alloc(1);
     }
-    XATU_HIP(hipGetLastError());
+    c.d_diag.alloc(c.N);
+    launch_diag_direct(c, c.d_diag.p, c.stream);
     c.h_diag.resize(c.N);
     XATU_HIP(hipMemcpyAsync(c.h_diag.data(), c.d_diag.p, c.N * sizeof(double), hipMemcpyDeviceToHost, c.stream));
     XATU_HIP(hipStreamSynchronize(c.stream));
+    c.prepared = true;
 }
 
 }  // namespace
@@ -108,9 +107,21 @@ void apply_operator(Context& c, int precision, int matrix_free, const double2* X
                     double* ms_kernel) {
     EventTimer ev(c.stream);
     if (matrix_free) {
-        ev.start();
-        launch_matfree64(c, X, Y, nvec, c.stream);
-        ev.stop();
+        if (!c.prepared) throw Error(XATU_ERR_LOGIC, "operator not prepared (call xatu_build_bse)");
+        if (precision == XATU_FP64) {
+            ev.start();
+            launch_matfree64(c, X, Y, nvec, c.stream);
+            ev.stop();
+        } else {
+            const size_t n = static_cast<size_t>(c.N) * nvec;
+            c.s_x32.alloc(std::max(c.s_x32.n, n));
+            c.s_y32.alloc(std::max(c.s_y32.n, n));
+            launch_convert_64_to_32(X, c.s_x32.p, n, c.stream);
+            ev.start();
+            launch_matfree32(c, c.s_x32.p, c.s_y32.p, nvec, c.stream);
+            ev.stop();
+            launch_convert_32_to_64(c.s_y32.p, Y, n, c.stream);
+        }
     } else if (precision == XATU_FP64) {
         if (!c.built64) throw Error(XATU_ERR_LOGIC, "FP64 BSE matrix is not resident");
         ev.start();
@@ -118,20 +129,16 @@ void apply_operator(Context& c, int precision, int matrix_free, const double2* X
         ev.stop();
     } else {
         if (!c.built32) throw Error(XATU_ERR_LOGIC, "FP32 BSE matrix is not resident");
-        const size_t n = static_cast<size_t>(c.N) * nvec;
-        c.s_x32.alloc(std::max(c.s_x32.n, n));
-        c.s_y32.alloc(std::max(c.s_y32.n, n));
-        launch_convert_64_to_32(X, c.s_x32.p, n, c.stream);
         ev.start();
-        launch_matvec32(c, c.s_x32.p, c.s_y32.p, nvec, c.stream);
+        launch_matvec32(c, X, Y, nvec, c.stream);
         ev.stop();
-        launch_convert_32_to_64(c.s_y32.p, Y, n, c.stream);
     }
     const double ms = ev.ms();
     if (ms_kernel) *ms_kernel = ms;
     if (!matrix_free) {
         c.tim.ms_matvec_kernel += ms;
         c.tim.n_matvec_kernel += 1;
+        c.tim.n_matvec_vectors += nvec;
     }
 }
 
@@ -202,7 +209,7 @@ int xatu_set_lattice(xatu_ctx* ctx, int ndim, const double* bravais, int natoms,
         c.lat = L;
         c.a1norm = std::sqrt(bravais[0] * bravais[0] + bravais[1] * bravais[1] + bravais[2] * bravais[2]);
         c.have_lattice = true;
-        c.have_bands = c.have_ft = c.built64 = c.built32 = false;
+        c.have_bands = c.have_ft = c.built64 = c.built32 = c.prepared = false;
     });
 }
 
@@ -245,7 +252,7 @@ int xatu_set_bands(xatu_ctx* ctx, int nk, int nv, int nc, const double* eigval_k
                                 c.stream));
         XATU_HIP(hipStreamSynchronize(c.stream));
         c.have_bands = true;
-        c.built64 = c.built32 = false;
+        c.built64 = c.built32 = c.prepared = false;
     });
 }
 
@@ -289,7 +296,7 @@ int xatu_fourier_sums(xatu_ctx* ctx, const xatu_interaction* inter) {
         ev.stop();
         c.tim.ms_fourier = ev.ms();
         c.have_ft = true;
-        c.built64 = c.built32 = false;
+        c.built64 = c.built32 = c.prepared = false;
     });
 }
 
@@ -298,42 +305,37 @@ int xatu_build_bse(xatu_ctx* ctx, int precision, double scissor, void* hbs_out) 
         c.require_lattice();
         c.require_bands();
         c.require_ft();
-        if (precision != XATU_FP64 && precision != XATU_FP32) throw Error(XATU_ERR_INVALID_ARGUMENT, "bad precision");
+        if (precision != XATU_FP64 && precision != XATU_FP32 && precision != XATU_MATRIX_FREE) {
+            throw Error(XATU_ERR_INVALID_ARGUMENT, "bad precision");
+        }
+        if (precision == XATU_MATRIX_FREE && hbs_out) {
+            throw Error(XATU_ERR_INVALID_ARGUMENT, "a matrix-free build has no matrix to copy out");
+        }
+        if (c.prepared && c.scissor != scissor) {
+            // a resident matrix built with another scissor no longer matches the operator
+            c.built64 = c.built32 = false;
+        }
         c.scissor = scissor;
         const size_t nn = static_cast<size_t>(c.N) * static_cast<size_t>(c.N);
         EventTimer ev(c.stream), evk(c.stream);
         ev.start();
-        if (c.inter.exchange) {
-            const size_t ne = static_cast<size_t>(c.nk) * c.lat.natoms * c.nc * c.nv;
-            c.d_e1.alloc(ne);
-            c.d_e2.alloc(ne);
-            launch_exchange_factors(c, c.stream);
-        } else {
-            c.d_e1.alloc(1);
-            c.d_e2.alloc(1);
-        }
+        prepare_operator(c);
+        evk.start();
         if (precision == XATU_FP64) {
             c.built64 = false;
             c.d_H64.alloc(nn);
-            evk.start();
             launch_build_tiles64(c, c.d_H64.p, c.stream);
-            evk.stop();
-        } else {
+        } else if (precision == XATU_FP32) {
             c.built32 = false;
             c.d_H32.alloc(nn);
-            evk.start();
             launch_build_tiles32(c, c.d_H32.p, c.stream);
-            evk.stop();
         }
+        evk.stop();
         ev.stop();
         c.tim.ms_build = ev.ms();
         c.tim.ms_build_kernel = evk.ms();
-        if (precision == XATU_FP64) {
-            c.built64 = true;
-        } else {
-            c.built32 = true;
-        }
-        refresh_diag(c, precision);
+        if (precision == XATU_FP64) c.built64 = true;
+        if (precision == XATU_FP32) c.built32 = true;
         if (hbs_out) {
             const size_t bytes = nn * (precision == XATU_FP64 ? sizeof(double2) : sizeof(float2));
             const void* src = precision == XATU_FP64 ? static_cast<const void*>(c.d_H64.p)
@@ -402,6 +404,7 @@ int xatu_solve(xatu_ctx* ctx, const xatu_solve_options* opt, double* eigval_out,
         xatu_solve_report rep{};
         c.tim.ms_matvec_kernel = 0.0;
         c.tim.n_matvec_kernel = 0;
+        c.tim.n_matvec_vectors = 0;
         if (opt->method == XATU_SOLVE_DIAG) {
             solve_dense_diag(c, opt->nstates, eigval_out, eigvec_out);
             rep.converged = 1;
@@ -414,9 +417,7 @@ int xatu_solve(xatu_ctx* ctx, const xatu_solve_options* opt, double* eigval_out,
                     throw Error(XATU_ERR_LOGIC, "diagonalizeRaw(): BSE Hamiltonian is not initialized.");
                 }
             }
-            if (c.h_diag.size() != static_cast<size_t>(c.N)) {
-                throw Error(XATU_ERR_LOGIC, "diagonalizeRaw(): BSE Hamiltonian is not initialized.");
-            }
+            if (!c.prepared) throw Error(XATU_ERR_LOGIC, "diagonalizeRaw(): BSE Hamiltonian is not initialized.");
             solve_davidson(c, *opt, eigval_out, eigvec_out, &rep);
         } else {
             throw Error(XATU_ERR_INVALID_ARGUMENT, "unknown solve method");

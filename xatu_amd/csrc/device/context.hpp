@@ -124,13 +124,17 @@ struct Context {
     // resident matrices
     double scissor = 0.0;
     bool built64 = false, built32 = false;
+    bool prepared = false;  // exchange factors and diagonal ready for products
     DevBuf<double2> d_H64;
     DevBuf<float2> d_H32;
     std::vector<double> h_diag;  // diagonal of H (FP64), N
     DevBuf<double> d_diag;
 
-    // scratch for FP32 products
-    DevBuf<float2> s_x32, s_y32;
+    // scratch for products and projections
+    DevBuf<float2> s_x32, s_y32;      // FP32 copies of vectors (matrix-free FP32 products)
+    DevBuf<double2> s_xt64, s_yp64;   // staged vectors / split partial sums, FP64 dense product
+    DevBuf<float2> s_xt32, s_yp32;    // same, FP32 dense product
+    DevBuf<double2> s_gram;           // split partial sums of the Gram kernel
 
     // last solve
     int last_nstates = 0;
@@ -183,11 +187,16 @@ void launch_build_tiles64(const Context& c, double2* H, hipStream_t s);
 void launch_build_tiles32(const Context& c, float2* H, hipStream_t s);
 void launch_build_basis(const Context& c, int nbasis, const int* d_basis, double2* H, hipStream_t s);
 void launch_diag_extract(const Context& c, const double2* H, double* diag, hipStream_t s);
+void launch_diag_direct(const Context& c, double* diag, hipStream_t s);
 void launch_compare(const Context& c, int kblock, double* d_blockerr, hipStream_t s);
 
-// Dense products Y = H X (N x nvec, complex) for the resident matrices.
-void launch_matvec64(const Context& c, const double2* X, double2* Y, int nvec, hipStream_t s);
-void launch_matvec32(const Context& c, const float2* X, float2* Y, int nvec, hipStream_t s);
+// Dense products Y = H X (N x nvec, complex FP64 vectors) with the resident FP64 or
+// FP32 matrix (the FP32 product stages and accumulates in FP32).
+void launch_matvec64(Context& c, const double2* X, double2* Y, int nvec, hipStream_t s);
+void launch_matvec32(Context& c, const double2* X, double2* Y, int nvec, hipStream_t s);
+// C (m x n, leading dimension ldc) = A^H B over the N rows of A (lda) and B (ldb), FP64.
+void launch_gram(Context& c, const double2* A, int lda, const double2* B, int ldb, int m, int n, double2* C, int ldc,
+                 hipStream_t s);
 // Matrix-free product in FP64 / FP32 arithmetic (tiles recomputed).
 void launch_matfree64(const Context& c, const double2* X, double2* Y, int nvec, hipStream_t s);
 void launch_matfree32(const Context& c, const float2* X, float2* Y, int nvec, hipStream_t s);

@@ -23,6 +23,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <memory>
 #include <cmath>
 #include <complex>
 #include <numeric>
@@ -99,9 +100,9 @@ public:
         XATU_HIP(hipStreamSynchronize(c_.stream));
     }
 
-    // Start from an orthonormal block already in X (N x b): V = X.
-    void init_from_X() {
-        XATU_HIP(hipMemcpyAsync(ws_.V.p, ws_.X.p, static_cast<size_t>(N_) * b_ * sizeof(double2),
+    // Start from b orthonormal vectors given on the device (N x b): V = X0.
+    void init_from(const double2* X0) {
+        XATU_HIP(hipMemcpyAsync(ws_.V.p, X0, static_cast<size_t>(N_) * b_ * sizeof(double2),
                                 hipMemcpyDeviceToDevice, c_.stream));
         m_ = 0;
         append_block(b_);
@@ -133,6 +134,7 @@ public:
     const double2* ritz_vectors() const { return ws_.X.p; }
     double products_ms() const { return prod_ms_; }
     double rr_ms() const { return rr_ms_; }
+    double projections_ms() const { return proj_ms_; }
 
 private:
     // W[:, m:m+nb] = H V[:, m:m+nb], then extend the projected matrix.
@@ -145,9 +147,13 @@ private:
         rep_->products += nb;
         const int mn = m_ + nb;
         // P = V[:, 0:mn]^H W_new  (mn x nb)
-        XATU_BLAS(rocblas_zgemm(c_.blas, rocblas_operation_conjugate_transpose, rocblas_operation_none, mn, nb,
-                                static_cast<int>(N_), &kOne, bz(ws_.V.p), static_cast<int>(N_), bz(Wn),
-                                static_cast<int>(N_), &kZero, bz(ws_.P.p), mn));
+        {
+            EventTimer ev(c_.stream);
+            ev.start();
+            launch_gram(c_, ws_.V.p, static_cast<int>(N_), Wn, static_cast<int>(N_), mn, nb, ws_.P.p, mn, c_.stream);
+            ev.stop();
+            proj_ms_ += ev.ms();
+        }
         std::vector<cd> P(static_cast<size_t>(mn) * nb);
         XATU_HIP(hipMemcpyAsync(P.data(), ws_.P.p, P.size() * sizeof(cd), hipMemcpyDeviceToHost, c_.stream));
         XATU_HIP(hipStreamSynchronize(c_.stream));
@@ -175,10 +181,9 @@ private:
     void rayleigh_ritz() {
         const double t0 = now_ms();
         std::vector<double> w(m_);
-        std::vector<cd> Z(static_cast<size_t>(m_) * m_);
-        xatu_linalg::heev(m_, T_.data(), mmax_, w.data(), Z.data());
+        Yb_.assign(static_cast<size_t>(m_) * b_, cd(0.0, 0.0));
+        xatu_linalg::heev_lowest(m_, T_.data(), mmax_, b_, w.data(), Yb_.data());
         theta_.assign(w.begin(), w.begin() + b_);
-        Yb_.assign(Z.begin(), Z.begin() + static_cast<size_t>(m_) * b_);
         rr_ms_ += now_ms() - t0;
     }
 
@@ -186,16 +191,20 @@ private:
         XATU_HIP(hipMemcpyAsync(ws_.Y.p, Yb_.data(), Yb_.size() * sizeof(cd), hipMemcpyHostToDevice, c_.stream));
         XATU_HIP(hipMemcpyAsync(ws_.theta.p, theta_.data(), b_ * sizeof(double), hipMemcpyHostToDevice, c_.stream));
         const int n = static_cast<int>(N_);
+        EventTimer ev(c_.stream);
+        ev.start();
         XATU_BLAS(rocblas_zgemm(c_.blas, rocblas_operation_none, rocblas_operation_none, n, b_, m_, &kOne,
                                 bz(ws_.V.p), n, bz(ws_.Y.p), m_, &kZero, bz(ws_.X.p), n));
         XATU_BLAS(rocblas_zgemm(c_.blas, rocblas_operation_none, rocblas_operation_none, n, b_, m_, &kOne,
                                 bz(ws_.W.p), n, bz(ws_.Y.p), m_, &kZero, bz(ws_.AX.p), n));
+        ev.stop();
         int nblk = 0;
         launch_residual(ws_.X.p, ws_.AX.p, ws_.theta.p, N_, b_, ws_.R.p, ws_.partial.p, &nblk, c_.stream);
         std::vector<double> part(static_cast<size_t>(nblk) * b_);
         XATU_HIP(hipMemcpyAsync(part.data(), ws_.partial.p, part.size() * sizeof(double), hipMemcpyDeviceToHost,
                                 c_.stream));
         XATU_HIP(hipStreamSynchronize(c_.stream));
+        proj_ms_ += ev.ms();
         res_.assign(b_, 0.0);
         for (int j = 0; j < b_; ++j) {
             double s = 0.0;
@@ -217,12 +226,15 @@ private:
     // Orthogonalise the block Tb (N x nb) against V[:, 0:m] (two passes).
     void project_out(double2* Tb, int nb) {
         const int n = static_cast<int>(N_);
+        EventTimer ev(c_.stream);
+        ev.start();
         for (int pass = 0; pass < 2; ++pass) {
-            XATU_BLAS(rocblas_zgemm(c_.blas, rocblas_operation_conjugate_transpose, rocblas_operation_none, m_, nb,
-                                    n, &kOne, bz(ws_.V.p), n, bz(Tb), n, &kZero, bz(ws_.P.p), m_));
+            launch_gram(c_, ws_.V.p, n, Tb, n, m_, nb, ws_.P.p, m_, c_.stream);
             XATU_BLAS(rocblas_zgemm(c_.blas, rocblas_operation_none, rocblas_operation_none, n, nb, m_, &kMinusOne,
                                     bz(ws_.V.p), n, bz(ws_.P.p), m_, &kOne, bz(Tb), n));
         }
+        ev.stop();
+        proj_ms_ += ev.ms();
     }
 
     // Orthonormalise the block Tb in place via its Gram matrix; returns the kept count
@@ -230,8 +242,7 @@ private:
     int orthonormalise(double2* Tb, int nb) {
         const int n = static_cast<int>(N_);
         std::vector<cd> G(static_cast<size_t>(nb) * nb);
-        XATU_BLAS(rocblas_zgemm(c_.blas, rocblas_operation_conjugate_transpose, rocblas_operation_none, nb, nb, n,
-                                &kOne, bz(Tb), n, bz(Tb), n, &kZero, bz(ws_.P.p), nb));
+        launch_gram(c_, Tb, n, Tb, n, nb, nb, ws_.P.p, nb, c_.stream);
         XATU_HIP(hipMemcpyAsync(G.data(), ws_.P.p, G.size() * sizeof(cd), hipMemcpyDeviceToHost, c_.stream));
         XATU_HIP(hipStreamSynchronize(c_.stream));
         std::vector<double> lam(nb);
@@ -282,7 +293,7 @@ private:
     std::vector<cd> T_;
     std::vector<double> theta_, res_;
     std::vector<cd> Yb_;
-    double prod_ms_ = 0.0, rr_ms_ = 0.0, last_worst_ = 0.0;
+    double prod_ms_ = 0.0, rr_ms_ = 0.0, proj_ms_ = 0.0, last_worst_ = 0.0;
 };
 
 }  // namespace
@@ -330,38 +341,50 @@ void solve_davidson(Context& c, const xatu_solve_options& opt, double* eigval_ou
     mmax = std::max(mmax, std::min<int>(2 * b, static_cast<int>(N)));
     if (mmax < b + 1 && N > b) mmax = b + 1;
     const int maxit = opt.max_iter > 0 ? opt.max_iter : 400;
-    const double tol64 = opt.tol > 0 ? opt.tol : 1e-6;
+    const double tol = opt.tol > 0 ? opt.tol : 1e-5;
 
     const double t0 = now_ms();
-    Davidson dav(c, opt.precision, opt.matrix_free, nev, b, mmax, rep);
-    dav.init_unit();
+    std::unique_ptr<Davidson> d32;
+    std::unique_ptr<Davidson> dav;
     bool ok;
     if (opt.precision == XATU_FP32) {
-        // inner FP32 phase to a loose tolerance, then FP64 refinement
-        const double tol32 = std::max(tol64, 1e-4);
-        dav.run(tol32, maxit);
+        // FP32 products to a tolerance above the FP32 noise floor ...
+        d32 = std::make_unique<Davidson>(c, XATU_FP32, opt.matrix_free, nev, b, mmax, rep);
+        d32->init_unit();
+        d32->run(std::max(0.5 * tol, 2e-5), maxit);
         const int it32 = rep->iterations;
-        dav.set_precision(XATU_FP64, opt.matrix_free || !c.built64);
-        dav.init_from_X();
-        ok = dav.run(tol64, maxit);
+        // ... then FP64 refinement from the lowest nev Ritz vectors, with FP64 products
+        // (matrix-free when no FP64 matrix is resident), until the FP64 residuals meet tol.
+        const int bref = nev;
+        const int mref = static_cast<int>(std::min<int64_t>(N, std::max(4 * bref, bref + 1)));
+        dav = std::make_unique<Davidson>(c, XATU_FP64, opt.matrix_free || !c.built64, nev, bref, mref, rep);
+        dav->init_from(d32->ritz_vectors());
+        ok = dav->run(tol, maxit);
         rep->refine_iterations = rep->iterations - it32;
+        rep->ms_products = d32->products_ms();
+        rep->ms_rayleigh_ritz = d32->rr_ms();
+        rep->ms_projections = d32->projections_ms();
     } else {
-        ok = dav.run(tol64, maxit);
+        dav = std::make_unique<Davidson>(c, opt.precision, opt.matrix_free, nev, b, mmax, rep);
+        dav->init_unit();
+        ok = dav->run(tol, maxit);
     }
     rep->converged = ok ? 1 : 0;
-    rep->max_residual = dav.worst();
-    rep->ms_products = dav.products_ms();
-    rep->ms_rayleigh_ritz = dav.rr_ms();
+    rep->max_residual = dav->worst();
+    rep->ms_products += dav->products_ms();
+    rep->ms_rayleigh_ritz += dav->rr_ms();
+    rep->ms_projections += dav->projections_ms();
+    const Davidson& final_d = *dav;
 
-    const std::vector<double>& th = dav.theta();
+    const std::vector<double>& th = final_d.theta();
     std::copy(th.begin(), th.begin() + nev, eigval_out);
     c.last_nstates = nev;
     c.last_val.assign(th.begin(), th.begin() + nev);
     c.d_last_vec.alloc(static_cast<size_t>(N) * nev);
-    XATU_HIP(hipMemcpyAsync(c.d_last_vec.p, dav.ritz_vectors(), static_cast<size_t>(N) * nev * sizeof(double2),
+    XATU_HIP(hipMemcpyAsync(c.d_last_vec.p, final_d.ritz_vectors(), static_cast<size_t>(N) * nev * sizeof(double2),
                             hipMemcpyDeviceToDevice, c.stream));
     if (eigvec_out) {
-        XATU_HIP(hipMemcpyAsync(eigvec_out, dav.ritz_vectors(), static_cast<size_t>(N) * nev * sizeof(double2),
+        XATU_HIP(hipMemcpyAsync(eigvec_out, final_d.ritz_vectors(), static_cast<size_t>(N) * nev * sizeof(double2),
                                 hipMemcpyDeviceToHost, c.stream));
     }
     XATU_HIP(hipStreamSynchronize(c.stream));

@@ -152,7 +152,8 @@ PYBIND11_MODULE(_xatu, m) {
         .def_readonly("ms_build", &xatu_timings::ms_build)
         .def_readonly("ms_build_kernel", &xatu_timings::ms_build_kernel)
         .def_readonly("ms_matvec_kernel", &xatu_timings::ms_matvec_kernel)
-        .def_readonly("n_matvec_kernel", &xatu_timings::n_matvec_kernel);
+        .def_readonly("n_matvec_kernel", &xatu_timings::n_matvec_kernel)
+        .def_readonly("n_matvec_vectors", &xatu_timings::n_matvec_vectors);
 
     py::class_<xatu_solve_report>(m, "SolveReport")
         .def_readonly("iterations", &xatu_solve_report::iterations)
@@ -163,7 +164,8 @@ PYBIND11_MODULE(_xatu, m) {
         .def_readonly("max_residual", &xatu_solve_report::max_residual)
         .def_readonly("ms_total", &xatu_solve_report::ms_total)
         .def_readonly("ms_products", &xatu_solve_report::ms_products)
-        .def_readonly("ms_rayleigh_ritz", &xatu_solve_report::ms_rayleigh_ritz);
+        .def_readonly("ms_rayleigh_ritz", &xatu_solve_report::ms_rayleigh_ritz)
+        .def_readonly("ms_projections", &xatu_solve_report::ms_projections);
 
     py::class_<ResultTB>(m, "ResultTB")
         .def_property_readonly("eigval", [](const ResultTB& r) { return py::array_t<double>(r.eigval.size(), r.eigval.data()); })
