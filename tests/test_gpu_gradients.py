@@ -1,0 +1,153 @@
+"""Hellmann-Feynman gradients through the C-ABI (xatu_hf_gradients).
+
+Checks: (i) the oracle's explicit FP64 dH (analytic dV tables pushed through the
+restated per-element BSE loop with band energies and scissor set to zero),
+(ii) the exact identity dE/deps_m = dE/deps_s = (<K>_n - E_n) / (2 eps_bar) for a
+Keldysh interaction (K the non-interacting diagonal), (iii) central finite
+differences of the oracle spectrum (step 1e-5 relative), and (iv) a
+torch.autograd.Function wrapper whose backward uses the adjoint gradients."""
+import numpy as np
+import pytest
+
+import oracle as O
+from cases import MODELS, gpu_context, oracle_case
+
+pytestmark = pytest.mark.gpu
+
+R0, EPS_M, EPS_S = 0, 1, 2
+
+
+def case(name, request):
+    if name == "random18":
+        return oracle_case(request.getfixturevalue("synthetic18"), 10, 2, 0, (1.0, 1.0, 40.0), cutoff=25,
+                           exchange=True, Q=(0.02, 0.0, 0.0))
+    if name == "mos2_q_x":
+        return oracle_case(MODELS / "MoS2.model", 6, 2, 0, (1.0, 4.0, 13.55), Q=(0, 0.1, 0), exchange=True)
+    if name == "hbn_coulomb":
+        return oracle_case(MODELS / "hBN.model", 8, 1, 0, (1, 1, 10), potential="coulomb", exchange=True,
+                           exchange_potential="coulomb")
+    raise KeyError(name)
+
+
+def explicit(ex, psi, kind):
+    dft = ex.ft_stack(kind)
+    dftq = ex.ft_at(ex.Q, kind) if ex.exchange else np.zeros_like(ex.ftq)
+    dH = ex.bse(ft=dft, ftq=dftq, eigval_k=np.zeros_like(ex.eigval_k), eigval_kq=np.zeros_like(ex.eigval_kq),
+                scissor=0.0)
+    return np.real(np.einsum("in,ij,jn->n", psi.conj(), dH, psi))
+
+
+@pytest.mark.parametrize("name", ["random18", "mos2_q_x"])
+def test_gradients_match_explicit_dH(name, request):
+    ex = case(name, request)
+    H = ex.bse()
+    w, V = np.linalg.eigh(H)
+    ns = 8
+    psi = V[:, :ns]
+    ctx = gpu_context(ex)
+    ctx.build(2)
+    g = ctx.hf_gradients(ns, [R0, EPS_M, EPS_S], eigvec=psi)
+    ref_r0 = explicit(ex, psi, O.POT_KELDYSH_DR0)
+    ref_eps = explicit(ex, psi, O.POT_KELDYSH_DEPS)
+    print(f"[{name}] dE/dr0 {g[:, 0]}")
+    assert np.abs(g[:, 0] - ref_r0).max() <= 1e-9 * np.abs(ref_r0).max()
+    assert np.abs(g[:, 1] - ref_eps).max() <= 1e-9 * np.abs(ref_eps).max()
+    assert np.array_equal(g[:, 1], g[:, 2])
+    # closed form for the dielectric constants
+    H0 = ex.bse(ft=np.zeros_like(ex.ft), ftq=np.zeros_like(ex.ftq))
+    K = np.real(np.einsum("in,i,in->n", psi.conj(), np.real(np.diag(H0)), psi))
+    ebar = (ex.eps_m + ex.eps_s) / 2
+    assert np.abs(g[:, 1] - (K - w[:ns]) / (2 * ebar)).max() < 1e-9
+    ctx.close()
+
+
+def test_gradients_match_finite_differences(request):
+    ex = case("random18", request)
+    H = ex.bse()
+    w, V = np.linalg.eigh(H)
+    ns = 8
+    assert np.min(np.diff(w[:ns + 1])) > 1e-4  # non-degenerate states
+    ctx = gpu_context(ex)
+    ctx.build(2)
+    g = ctx.hf_gradients(ns, [R0, EPS_S], eigvec=V[:, :ns])
+    ctx.close()
+
+    def energies(r0, eps_s):
+        e = oracle_case(request.getfixturevalue("synthetic18"), 10, 2, 0, (1.0, eps_s, r0), cutoff=25, exchange=True,
+                        Q=(0.02, 0.0, 0.0))
+        return np.linalg.eigvalsh(e.bse())[:ns]
+
+    h = 1e-5 * 40.0
+    fd_r0 = (energies(40.0 + h, 1.0) - energies(40.0 - h, 1.0)) / (2 * h)
+    he = 1e-5 * 1.0
+    fd_eps = (energies(40.0, 1.0 + he) - energies(40.0, 1.0 - he)) / (2 * he)
+    print("dE/dr0 adjoint", g[:, 0], "fd", fd_r0)
+    assert np.abs(g[:, 0] - fd_r0).max() <= 1e-4 * np.abs(fd_r0).max()
+    assert np.abs(g[:, 1] - fd_eps).max() <= 1e-4 * np.abs(fd_eps).max()
+
+
+def test_coulomb_has_no_screening_gradient(request):
+    ex = case("hbn_coulomb", request)
+    w, V = np.linalg.eigh(ex.bse())
+    ctx = gpu_context(ex)
+    ctx.build(2)
+    g = ctx.hf_gradients(4, [R0, EPS_M, EPS_S], eigvec=V[:, :4])
+    assert np.all(g == 0.0)
+    ctx.close()
+
+
+def test_gradients_of_last_solve_through_facade():
+    import xatu_amd as X
+    ex = X.ExcitonTB(X.SystemConfiguration(str(MODELS / "MoS2.model")), 6, 2, 0, [1., 4., 13.55], [0, 0.1, 0])
+    ex.setExchange(True)
+    ex.brillouinZoneMesh(6)
+    ex.initializeHamiltonian()
+    ex.BShamiltonian()
+    res = ex.diagonalize("davidson", 4)
+    g_last = ex.hfGradients(None, 4, ["r0", "eps_m"])
+    g_given = ex.hfGradients(res.eigvec, 4, ["r0", "eps_m"])
+    assert np.allclose(g_last, g_given, rtol=1e-12, atol=0)
+
+
+def test_autograd_function_wrapper(request):
+    print("[autograd] importing torch", flush=True)
+    import torch
+
+    from xatu_amd import capi
+    ex = case("random18", request)
+    base = gpu_context(ex)
+    ns = 4
+
+    class ExcitonEnergies(torch.autograd.Function):
+        """E_n(r0, eps_m, eps_s) from the GPU pipeline; backward from the adjoint kernel."""
+
+        @staticmethod
+        def forward(ctx_, params):
+            r0, em, es = (float(x) for x in params)
+            base.fourier_sums(em, es, r0, ex.cutoff, ex.ncell, ex.total_cells, regularization=ex.regularization,
+                              exchange=True, Q=tuple(ex.Q))
+            base.build(capi.XATU_FP64)
+            w, v, _ = base.solve(method=capi.XATU_SOLVE_DIAG, nstates=ex.N)
+            g = base.hf_gradients(ns, [R0, EPS_M, EPS_S], eigvec=v[:, :ns])
+            ctx_.save_for_backward(torch.from_numpy(g))
+            return torch.from_numpy(w[:ns].copy())
+
+        @staticmethod
+        def backward(ctx_, grad_e):
+            (g,) = ctx_.saved_tensors
+            return g.T @ grad_e
+
+    p = torch.tensor([40.0, 1.0, 1.0], dtype=torch.float64, requires_grad=True)
+    e = ExcitonEnergies.apply(p)
+    e.sum().backward()
+    # compare with a central difference of the summed energies
+    h = torch.tensor([1e-5 * 40.0, 1e-5, 1e-5], dtype=torch.float64)
+    fd = []
+    for i in range(3):
+        dp = torch.zeros(3, dtype=torch.float64)
+        dp[i] = h[i]
+        fd.append(float((ExcitonEnergies.apply(p.detach() + dp).sum() - ExcitonEnergies.apply(p.detach() - dp).sum())
+                        / (2 * h[i])))
+    fd = np.array(fd)
+    assert np.abs(p.grad.numpy() - fd).max() <= 1e-4 * np.abs(fd).max()
+    base.close()

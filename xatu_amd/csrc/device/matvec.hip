@@ -27,10 +27,14 @@ constexpr int kBlockRows = 256;  // rows per workgroup (4 waves)
 
 template <typename R>
 struct Mfma;
+// Both take A[i][k] from lane i + 16 k and B[k][j] from lane j + 16 k. Output
+// register v of lane l holds C[row][l % 16] with row = 4 (l / 16) + v for the FP32
+// instruction and row = l / 16 + 4 v for the FP64 one (checked on gfx950).
 template <>
 struct Mfma<float> {
     using acc_t = f32x4;
     __device__ static acc_t mma(float a, float b, acc_t c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+    __device__ static int out_row(int lk, int v) { return 4 * lk + v; }
 };
 template <>
 struct Mfma<double> {
@@ -38,6 +42,7 @@ struct Mfma<double> {
     __device__ static acc_t mma(double a, double b, acc_t c) {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
+    __device__ static int out_row(int lk, int v) { return lk + 4 * v; }
 };
 
 // Xt[j][n] = X[j + n N] for n < nvec, 0 otherwise (j < N, n < 32).
@@ -104,13 +109,12 @@ __global__ __launch_bounds__(256) void dense_product_kernel(const cx_t<R>* __res
                 accI[t][u] = M::mma(h[t].y, x[u].x, accI[t][u]);
             }
     }
-    // acc[v] holds C[4*(lane/16) + v][lane%16]
     cx_t<R>* out = Yp + static_cast<int64_t>(blockIdx.y) * N * kVec;
 #pragma unroll
     for (int t = 0; t < 4; ++t)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-            const int64_t r = row0 + 16 * t + 4 * lk + v;
+            const int64_t r = row0 + 16 * t + M::out_row(lk, v);
             if (r < N) {
 #pragma unroll
                 for (int u = 0; u < 2; ++u) out[r * kVec + 16 * u + li] = mk(accR[t][u][v], accI[t][u][v]);
