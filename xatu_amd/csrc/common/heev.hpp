@@ -95,7 +95,6 @@ inline void tridiagonalize(int n, const cd* Ain, int lda, Tridiag& t) {
         t.vused[k] = 1;
         // p = B v with B = A[k+1:, k+1:]: p_i = sum_j conj(B_ji) v_j (column i of B is contiguous)
         const int base = k + 1;
-#pragma omp parallel for schedule(static) if (m >= 96)
         for (int i = 0; i < m; ++i) {
             const double* br = ar.data() + static_cast<size_t>(base + i) * n + base;
             const double* bi = ai.data() + static_cast<size_t>(base + i) * n + base;
@@ -115,7 +114,6 @@ inline void tridiagonalize(int n, const cd* Ain, int lda, Tridiag& t) {
             wi[i] = pi[i] - cr * vi[i];
         }
         // B -= 2 (v w^H + w v^H)
-#pragma omp parallel for schedule(static) if (m >= 96)
         for (int j = 0; j < m; ++j) {
             double* br = ar.data() + static_cast<size_t>(base + j) * n + base;
             double* bi = ai.data() + static_cast<size_t>(base + j) * n + base;

@@ -322,11 +322,12 @@ __global__ __launch_bounds__(TK* TK) void bse_matfree_kernel(LatticeDesc lat, in
     constexpr int NT = TK * TK;
     constexpr int NCHUNK = kMaxVec / kVecChunk;
     constexpr int PER = (ROWS * kVecChunk + NT - 1) / NT;
+    constexpr int LDH = ROWS + 1;  // column-major H tile, one element of padding
     extern __shared__ unsigned char smem_raw[];
     const int norb = lat.norb;
     C* s_coef = reinterpret_cast<C*>(smem_raw);                  // 2*TK*norb*NB
-    C* s_h = s_coef + static_cast<size_t>(2 * TK) * norb * NB;   // ROWS x ROWS, row-major [r][c]
-    C* s_x = s_h + ROWS * ROWS;                                  // ROWS x kVecChunk
+    C* s_h = s_coef + static_cast<size_t>(2 * TK) * norb * NB;   // ROWS x ROWS, column-major, ld LDH
+    C* s_x = s_h + ROWS * LDH;                                  // kVecChunk x ROWS, [j][r]
     const int64_t N = static_cast<int64_t>(nk) * NCV;
     const int Kr = blockIdx.x;
     const int nT = (nk + TK - 1) / TK;
@@ -343,7 +344,7 @@ __global__ __launch_bounds__(TK* TK) void bse_matfree_kernel(LatticeDesc lat, in
     for (int Kc = 0; Kc < nT; ++Kc) {
         __syncthreads();
         stage_coefficients<R, NC, NV, TK>(s_coef, norb, nk, Kr, Kc, evk, evkq, NT);
-        for (int e = t; e < ROWS * ROWS; e += NT) s_h[e] = mk(R(0), R(0));
+        for (int e = t; e < ROWS * LDH; e += NT) s_h[e] = mk(R(0), R(0));
         __syncthreads();
         const int kr = Kr * TK + tr, kc = Kc * TK + tc;
         if (kr < nk && kc < nk) {
@@ -359,17 +360,17 @@ __global__ __launch_bounds__(TK* TK) void bse_matfree_kernel(LatticeDesc lat, in
                                           if (diagblk) {
                                               const int base = tr * NCV;
                                               if (ia < ib) {
-                                                  s_h[(base + ia) * ROWS + tc * NCV + ib] = mk(-DX.x, -DX.y);
-                                                  s_h[(base + ib) * ROWS + tc * NCV + ia] = mk(-DX.x, DX.y);
+                                                  s_h[(tc * NCV + ib) * LDH + base + ia] = mk(-DX.x, -DX.y);
+                                                  s_h[(tc * NCV + ia) * LDH + base + ib] = mk(-DX.x, DX.y);
                                               } else if (ia == ib) {
                                                   const double h = diag_value(eval_k, eval_kq, NB, NV, ka, cA, vA,
                                                                               scissor, static_cast<double>(DX.x));
-                                                  s_h[(base + ia) * ROWS + tc * NCV + ia] = mk(R(h), R(0));
+                                                  s_h[(tc * NCV + ia) * LDH + base + ia] = mk(R(h), R(0));
                                               }
                                           } else if (!sw) {
-                                              s_h[(tr * NCV + ia) * ROWS + tc * NCV + ib] = mk(-DX.x, -DX.y);
+                                              s_h[(tc * NCV + ib) * LDH + tr * NCV + ia] = mk(-DX.x, -DX.y);
                                           } else {
-                                              s_h[(tr * NCV + ib) * ROWS + tc * NCV + ia] = mk(-DX.x, DX.y);
+                                              s_h[(tc * NCV + ia) * LDH + tr * NCV + ib] = mk(-DX.x, DX.y);
                                           }
                                       });
         }
@@ -381,7 +382,7 @@ __global__ __launch_bounds__(TK* TK) void bse_matfree_kernel(LatticeDesc lat, in
                     const int r = e % ROWS, j = e / ROWS;
                     const int64_t row = static_cast<int64_t>(Kc) * ROWS + r;
                     const int vj = q * kVecChunk + j;
-                    s_x[r * kVecChunk + j] = (row < N && vj < nvec) ? X[row + static_cast<int64_t>(vj) * N]
+                    s_x[j * ROWS + r] = (row < N && vj < nvec) ? X[row + static_cast<int64_t>(vj) * N]
                                                                     : mk(R(0), R(0));
                 }
                 __syncthreads();
@@ -391,7 +392,7 @@ __global__ __launch_bounds__(TK* TK) void bse_matfree_kernel(LatticeDesc lat, in
                     if (e < ROWS * kVecChunk) {
                         const int r = e % ROWS, j = e / ROWS;
                         C sum = acc[q][m];
-                        for (int cidx = 0; cidx < ROWS; ++cidx) cfma(sum, s_h[r * ROWS + cidx], s_x[cidx * kVecChunk + j]);
+                        for (int cidx = 0; cidx < ROWS; ++cidx) cfma(sum, s_h[cidx * LDH + r], s_x[j * ROWS + cidx]);
                         acc[q][m] = sum;
                     }
                 }
@@ -549,7 +550,7 @@ struct Shape {
     template <int TK>
     static size_t matfree_bytes(const Context& c) {
         constexpr int ROWS = TK * NCV;
-        return coef_bytes<R, TK>(c, NB) + static_cast<size_t>(ROWS) * ROWS * sizeof(cx_t<R>) +
+        return coef_bytes<R, TK>(c, NB) + static_cast<size_t>(ROWS) * (ROWS + 1) * sizeof(cx_t<R>) +
                static_cast<size_t>(ROWS) * kVecChunk * sizeof(cx_t<R>);
     }
     template <int TK>
