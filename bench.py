@@ -42,6 +42,19 @@ PEAK_HBM_GBS = 8000.0
 TOL = 1e-5  # FP64 residual tolerance (eV) of both forms
 
 
+def measured_traffic(config, kind):
+    """HBM bytes per dense-product launch from the committed rocprofv3 PMC summary
+    (FETCH_SIZE + WRITE_SIZE, copy-calibrated), or None if that summary is absent."""
+    path = ROOT / "profiles" / f"dense_product_{config}_traffic.md"
+    if not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            vals = dict(kv.split("=") for kv in line.split(":", 1)[1].split())
+            return float(vals[kind]) if kind in vals else None
+    return None
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -196,10 +209,11 @@ def main():
         t = tim.ms_matvec_kernel / 1e3
         ach = flops / t / 1e12
         out["roofline"] = {"bound": "mfma", "achieved": round(ach, 2), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                           "frac": round(ach / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                           "frac": round(ach / PEAK_FP32_TFLOPS, 4), "traffic": measured_traffic(args.config, "fp32"),
                            "kernel": "FP32 dense product H X (N x N by N x 32)",
                            "avg_ms": round(tim.ms_matvec_kernel, 3), "launches": tim.n_matvec_kernel,
-                           "avg_vectors": round(nvec_avg, 2)}
+                           "avg_vectors": round(nvec_avg, 2), "algorithmic_bytes": 8.0 * N * N,
+                           "hbm_GBs": round(8.0 * N * N / t / 1e9, 1)}
     if not args.no_cpu:
         n_products = results["fp64"]["rep"].products if "fp64" in results else rep.products
         log("cpu baseline")
