@@ -167,6 +167,29 @@ int xatu_bse_matvec(xatu_ctx* ctx, int precision, int matrix_free, int nvec, con
 int xatu_hf_gradients(xatu_ctx* ctx, int nstates, const double* eigvec, int nparams, const int* params,
                       double* grad_out);
 
+/* Sensitivities of exciton energies to the band data, the adjoint of the BSE assembly
+ * (no reference counterpart; used for gradients with respect to the tight-binding
+ * hopping amplitudes together with first-order band perturbation theory on the host).
+ * For each state n (eigvec: N x nstates complex128, null: the last solve's vectors):
+ *   w_out: dE_n/d eps_b(k), nstates x 2 x nk x nb doubles (state slowest; stack 0 is
+ *          eigvalKStack, stack 1 eigvalKQStack, band fastest),
+ *   g_out: dE_n/d conj(C_b(k)) (Wirtinger derivative of the real E_n with respect to
+ *          the eigenvector coefficients), nstates x 2 stacks of norb x nb x nk complex128
+ *          (the eigvecKStack / eigvecKQStack layout),
+ * so that a change of the band data changes E_n by sum [w d(eps) + 2 Re(conj(g) dC)].
+ * Only the columns the BSE uses are non-zero: valence columns of the k stack and
+ * conduction columns of the k+Q stack. */
+int xatu_band_gradients(xatu_ctx* ctx, int nstates, const double* eigvec, double* w_out, double* g_out);
+
+/* Exciton oscillator strengths (the exciton sum of ResultTB::excitonOscillatorStrength,
+ * src/ResultTB.cpp:242-298 / src/skubo_w.f90:300-323): out[j + 3 n] = sum_i X_n(i) v_j(i)
+ * for j = x, y, z, with the pair velocity matrix elements velocity[i + N j] (complex128,
+ * atomic units, computed on the host from the band stacks) and the state vectors X
+ * (eigvec: N x nstates complex128, null: the last solve's vectors); no conjugation of X,
+ * as in the reference. out: 3 x nstates complex128. */
+int xatu_oscillator_strengths(xatu_ctx* ctx, int nstates, const double* eigvec, const double* velocity,
+                              double* out);
+
 /* Read-backs for parity checks.
  * ft_out: natoms x natoms x nmesh complex128 (ftMotifStack); ftq_out: natoms x natoms (ftMotifQ).
  * cells: number of lattice vectors nR and (if coefs non-null) nR x ndim integer coefficients. */

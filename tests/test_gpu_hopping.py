@@ -1,0 +1,195 @@
+"""Gradients of exciton energies with respect to the tight-binding hopping amplitudes.
+
+The device computes the band-data sensitivities (xatu_band_gradients, the adjoint of
+the BSE assembly) and the facade turns them into dE/dH(R) by first-order band
+perturbation theory (ExcitonTB.hoppingGradients). Checks:
+  (i)   gauge invariance: E does not change when one band vector is multiplied by a
+        phase, so Im(g_b(k)^H C_b(k)) = 0 for every band the BSE uses (exact identity
+        of the device adjoint alone);
+  (ii)  the band-energy part alone: with the interaction switched off (eps huge) E is
+        sum |psi|^2 (E_c - E_v) and the gradient reduces to band Hellmann-Feynman terms;
+  (iii) central finite differences (step 1e-5) of the oracle spectrum with the hopping
+        amplitude and its Hermitian partner H(-R)_ba = conj H(R)_ab perturbed together;
+  (iv)  the oracle's explicit dH: central differences of the full FP64 BSE matrix
+        contracted with the unperturbed eigenvectors.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from cases import gpu_context, oracle_case
+
+pytestmark = pytest.mark.gpu
+
+NCELL = 6
+Q = (0.02, 0.0, 0.0)
+PARAMS = (1.0, 1.0, 40.0)
+
+
+def facade(path, params=PARAMS, Q=Q, exchange=True):
+    import xatu_amd as X
+    ex = X.ExcitonTB(X.SystemConfiguration(path), NCELL, 2, 0, list(params), list(Q))
+    ex.setCutoff(25)
+    ex.setExchange(exchange)
+    ex.brillouinZoneMesh(NCELL)
+    ex.initializeHamiltonian()
+    ex.BShamiltonian()
+    return ex
+
+
+def partner(cells, r):
+    d = np.linalg.norm(cells + cells[r], axis=1)
+    rp = int(np.argmin(d))
+    assert d[rp] < 1e-9
+    return rp
+
+
+def perturbed_energies(path, r, a, b, delta, ns, params=PARAMS, want_matrix=False):
+    s = O.System(path)
+    s.brillouin_zone_mesh(NCELL)
+    rp = partner(s.cells, r)
+    if r == rp and a == b:
+        s.H[r, a, a] += delta.real
+    else:
+        s.H[r, a, b] += delta
+        s.H[rp, b, a] += np.conj(delta)
+    ex = O.Exciton(s, NCELL, 2, 0, params, Q=Q)
+    ex.cutoff = 25
+    ex.exchange = True
+    ex.initialize()
+    H = ex.bse()
+    return H if want_matrix else np.linalg.eigvalsh(H)[:ns]
+
+
+def analytic(G, cells, r, a, b):
+    """(dE/dRe t, dE/dIm t) for t = H(R)_ab moved together with H(-R)_ba = conj t."""
+    rp = partner(cells, r)
+    if r == rp and a == b:
+        return np.real(G[:, r, a, a]), None
+    d_re = np.real(G[:, r, a, b] + G[:, rp, b, a])
+    d_im = np.real(1j * G[:, r, a, b] - 1j * G[:, rp, b, a])
+    return d_re, d_im
+
+
+def test_band_sensitivities_are_gauge_invariant(synthetic18):
+    ex = oracle_case(synthetic18, NCELL, 2, 0, PARAMS, Q=Q, exchange=True, cutoff=25)
+    ctx = gpu_context(ex)
+    ctx.build()
+    w, V = np.linalg.eigh(ex.bse())
+    ns = 6
+    wb, g = ctx.band_gradients(ns, eigvec=V[:, :ns])
+    ctx.close()
+    nv = ex.nv
+    for st, C in ((0, ex.eigvec_k), (1, ex.eigvec_kq)):
+        bands = range(nv) if st == 0 else range(nv, nv + ex.nc)
+        other = range(nv, nv + ex.nc) if st == 0 else range(nv)
+        for b in bands:
+            ip = np.einsum("skn,nk->sk", g[:, st, :, b, :].conj(), C[:, b, :])
+            scale = np.abs(g[:, st, :, b, :]).max()
+            assert scale > 0
+            assert np.abs(ip.imag).max() <= 1e-12 * scale, (st, b)
+        for b in other:  # columns the BSE does not use in this stack
+            assert np.all(g[:, st, :, b, :] == 0)
+    # band-energy sensitivities: weights of the state on its conduction / valence bands
+    P = np.abs(V[:, :ns]) ** 2
+    P = P.reshape(ex.nk, ex.nc, ex.nv, ns)
+    assert np.allclose(wb[:, 1, :, nv:], np.transpose(P.sum(axis=2), (2, 0, 1)), atol=1e-15)
+    assert np.allclose(wb[:, 0, :, :nv], -np.transpose(P.sum(axis=1), (2, 0, 1)), atol=1e-15)
+
+
+def test_band_energy_part_without_interaction(synthetic18):
+    """eps = 1e12: the interaction vanishes and dE/dH(R) is the band Hellmann-Feynman term."""
+    params = (1e12, 1e12, 40.0)
+    ex = facade(synthetic18, params=params, exchange=False)
+    ns = 4
+    res = ex.diagonalize("diag", 0)
+    w = np.asarray(res.eigval)
+    assert np.min(np.diff(w[:ns + 1])) > 1e-6
+    V = np.asarray(res.eigvec)[:, :ns]
+    G = ex.hoppingGradients(V, ns)
+    s = O.System(synthetic18)
+    s.brillouin_zone_mesh(NCELL)
+    oc = oracle_case(synthetic18, NCELL, 2, 0, params, Q=Q, exchange=False, cutoff=25)
+    # explicit: dE = sum_i |psi_i|^2 (C_c^H dh(k+Q) C_c - C_v^H dh(k) C_v) for dh = dH(R) e^{ik.R}
+    P = (np.abs(V) ** 2).reshape(oc.nk, oc.nc, oc.nv, ns)
+    rng = np.random.default_rng(3)
+    for _ in range(4):
+        r = int(rng.integers(len(s.cells)))
+        a, b = (int(x) for x in rng.integers(s.basisdim, size=2))
+        # perturb H(R)_ab only (the formula is linear; use the Hermitian pair as in analytic())
+        rp = partner(s.cells, r)
+        ref = np.zeros(ns)
+        for kk in range(oc.nk):
+            for st, kv, C in ((0, s.kpoints[kk], oc.eigvec_k), (1, s.kpoints[kk] + np.asarray(Q), oc.eigvec_kq)):
+                dh = np.zeros((s.basisdim, s.basisdim), complex)
+                dh[a, b] += np.exp(1j * kv @ s.cells[r])
+                if not (r == rp and a == b):
+                    dh[b, a] += np.exp(1j * kv @ s.cells[rp])
+                else:
+                    dh[a, a] = np.cos(kv @ s.cells[r])
+                for n in range(ns):
+                    if st == 1:
+                        for ic in range(oc.nc):
+                            c = C[:, oc.nv + ic, kk]
+                            ref[n] += P[kk, ic, :, n].sum() * np.real(c.conj() @ dh @ c)
+                    else:
+                        for iv in range(oc.nv):
+                            v = C[:, iv, kk]
+                            ref[n] -= P[kk, :, iv, n].sum() * np.real(v.conj() @ dh @ v)
+        d_re, _ = analytic(G, s.cells, r, a, b)
+        assert np.abs(d_re - ref).max() <= 1e-9 * max(np.abs(ref).max(), 1e-3), (r, a, b)
+
+
+def test_hopping_gradients_match_finite_differences(synthetic18):
+    ex = facade(synthetic18)
+    ns = 4
+    res = ex.diagonalize("diag", 0)
+    w = np.asarray(res.eigval)
+    assert np.min(np.diff(w[:ns + 1])) > 1e-5
+    V = np.asarray(res.eigvec)[:, :ns]
+    G = ex.hoppingGradients(V, ns)
+    cells = np.asarray(ex.system.unitCellList).reshape(-1, 3)
+    s = O.System(synthetic18)
+    assert np.allclose(cells, s.cells)
+    rng = np.random.default_rng(11)
+    zero = int(np.argmin(np.linalg.norm(s.cells, axis=1)))
+    picks = [(zero, 0, 0), (zero, 1, 7)]
+    while len(picks) < 5:
+        r = int(rng.integers(len(s.cells)))
+        a, b = (int(x) for x in rng.integers(s.basisdim, size=2))
+        if np.abs(s.H[r, a, b]) > 1e-3:
+            picks.append((r, a, b))
+    h = 1e-5
+    for r, a, b in picks:
+        d_re, d_im = analytic(G, s.cells, r, a, b)
+        fd = (perturbed_energies(synthetic18, r, a, b, h, ns) - perturbed_energies(synthetic18, r, a, b, -h, ns)) / (2 * h)
+        print(f"[hopping] R={s.cells[r]} ({a},{b}) d/dRe adjoint {d_re} fd {fd}")
+        assert np.abs(d_re - fd).max() <= 1e-4 * max(np.abs(fd).max(), 1e-3), (r, a, b)
+        if d_im is not None:
+            fdi = (perturbed_energies(synthetic18, r, a, b, 1j * h, ns)
+                   - perturbed_energies(synthetic18, r, a, b, -1j * h, ns)) / (2 * h)
+            assert np.abs(d_im - fdi).max() <= 1e-4 * max(np.abs(fdi).max(), 1e-3), (r, a, b)
+
+
+def test_hopping_gradients_match_explicit_dH(synthetic18):
+    ex = facade(synthetic18)
+    ns = 4
+    res = ex.diagonalize("diag", 0)
+    V = np.asarray(res.eigvec)[:, :ns]
+    G = ex.hoppingGradients(V, ns)
+    s = O.System(synthetic18)
+    # the unperturbed oracle eigenvectors (same matrix to 1e-10: the facade's is compared elsewhere)
+    H0 = perturbed_energies(synthetic18, 0, 0, 0, 0.0, ns, want_matrix=True)
+    _, U = np.linalg.eigh(H0)
+    U = U[:, :ns]
+    rng = np.random.default_rng(5)
+    h = 1e-4
+    for _ in range(3):
+        r = int(rng.integers(len(s.cells)))
+        a, b = (int(x) for x in rng.integers(s.basisdim, size=2))
+        d_re, d_im = analytic(G, s.cells, r, a, b)
+        dH = (perturbed_energies(synthetic18, r, a, b, h, ns, want_matrix=True)
+              - perturbed_energies(synthetic18, r, a, b, -h, ns, want_matrix=True)) / (2 * h)
+        ref = np.real(np.einsum("in,ij,jn->n", U.conj(), dH, U))
+        print(f"[hopping dH] R={s.cells[r]} ({a},{b}) adjoint {d_re} explicit {ref}")
+        assert np.abs(d_re - ref).max() <= 1e-6 * max(np.abs(ref).max(), 1e-3), (r, a, b)

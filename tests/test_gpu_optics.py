@@ -1,0 +1,73 @@
+"""Exciton oscillator strengths and absorption through the facade (ResultTB) and the C-ABI
+(xatu_oscillator_strengths), against the oracle restatement of src/skubo_w.f90 and the
+reference's regression values (test/tests.cpp:370-395, 776-800)."""
+import numpy as np
+import pytest
+
+import oracle as O
+from cases import MODELS, gpu_context, oracle_case
+
+pytestmark = pytest.mark.gpu
+
+
+def facade(model, n, nb, params, Q=(0, 0, 0), exchange=False):
+    import xatu_amd as X
+    ex = X.ExcitonTB(X.SystemConfiguration(str(model)), n, nb, 0, list(params), list(Q))
+    ex.setExchange(exchange)
+    ex.brillouinZoneMesh(n)
+    ex.initializeHamiltonian()
+    ex.BShamiltonian()
+    return ex
+
+
+@pytest.mark.parametrize("model,n,nb,params,ref", [
+    ("hBN.model", 20, 1, (1, 1, 10), 47.4140063784),
+    ("MoS2.model", 12, 2, (1.0, 4.0, 13.55), 37.2792093358),
+])
+def test_oscillator_strength_reference_values(model, n, nb, params, ref):
+    print(f"[optics] {model} {n}x{n}", flush=True)
+    ex = facade(MODELS / model, n, nb, params)
+    res = ex.diagonalize("diag", 2)
+    vme = np.asarray(res.excitonOscillatorStrength())
+    assert vme.shape == (3, ex.excitonbasisdim)
+    assert abs(np.sum(np.abs(vme) ** 2) - ref) <= 1e-7
+
+
+def test_oscillator_strengths_match_oracle_per_state():
+    oc = oracle_case(MODELS / "MoS2.model", 12, 2, 0, (1.0, 4.0, 13.55))
+    ex = facade(MODELS / "MoS2.model", 12, 2, (1.0, 4.0, 13.55))
+    res = ex.diagonalize("diag", 2)
+    X = np.asarray(res.eigvec)
+    vme = np.asarray(res.excitonOscillatorStrength())
+    ref = O.exciton_oscillator_strength(oc, X)  # same eigenvectors: the contraction is checked per state
+    assert np.abs(vme - ref).max() <= 1e-12 * np.abs(ref).max()
+    # spectrum from the lowest states (cut between two distinct levels)
+    e = np.asarray(res.eigval)
+    ns = next(n for n in range(16, 40) if e[n] - e[n - 1] > 1e-6)
+    w, sig = O.exciton_spectrum(ref[:, :ns], e[:ns], oc.nk,
+                                float(np.linalg.norm(np.cross(oc.sys.bravais[0], oc.sys.bravais[1]))),
+                                w0=0.0, wrange=6.0, nw=2000, eta=0.05)
+    ex.setDavidsonTolerance(1e-10)
+    spec = np.asarray(ex.diagonalize("davidson", ns).absorptionSpectrum(0.0, 6.0, 2000, 0.05))
+    assert np.allclose(spec[:, 0], w)
+    got = spec[:, 1:].reshape(-1, 2, 2)
+    assert np.abs(got - sig).max() <= 1e-6 * np.abs(sig).max()
+
+
+def test_capi_oscillator_uses_last_solve():
+    from xatu_amd import capi
+    oc = oracle_case(MODELS / "MoS2.model", 12, 2, 0, (1.0, 4.0, 13.55), Q=(0, 0.05, 0), exchange=True)
+    ctx = gpu_context(oc)
+    ctx.build()
+    ns = 6
+    w, v, _ = ctx.solve(method=capi.XATU_SOLVE_DAVIDSON, nstates=ns)
+    V = O.velocity_matrix_elements(oc)
+    vel = np.zeros((oc.nk, oc.nc, oc.nv, 3), complex)
+    for ic in range(oc.nc):
+        for iv in range(oc.nv):
+            vel[:, ic, iv, :] = V[:, :, iv, oc.nv + ic]
+    vel = vel.reshape(oc.N, 3)
+    got = ctx.oscillator_strengths(vel, ns)           # device vectors of the last solve
+    ref = vel.T @ v
+    assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+    ctx.close()

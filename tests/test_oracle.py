@@ -281,3 +281,37 @@ def test_bse_exchange_is_rank_natoms():
     Hx = ex.bse() - ex.bse(ftq=np.zeros_like(ex.ftq))
     s = np.linalg.svd(Hx, compute_uv=False)
     assert s[ex.sys.natoms] < 1e-10 * s[0]
+
+
+# ---------------------------------------------------------------------------- optics
+
+@pytest.mark.parametrize("model,n,nb,params,ref", [
+    ("hBN.model", 20, 1, (1, 1, 10), 47.4140063784),        # test/tests.cpp:370-395
+    ("MoS2.model", 12, 2, (1.0, 4.0, 13.55), 37.2792093358),  # test/tests.cpp:776-800
+])
+def test_oscillator_strength_sum_matches_reference(model, n, nb, params, ref):
+    """Sum over all exciton states of |vme_ex|^2: by unitarity of the eigenvectors this
+    is the sum of |V(k, j, v, c)|^2 over all pairs, so it pins the velocity restatement."""
+    ex = oracle_case(MODELS / model, n, nb, 0, params)
+    _, X = np.linalg.eigh(ex.bse())
+    vme = O.exciton_oscillator_strength(ex, X)
+    assert abs(np.sum(np.abs(vme) ** 2) - ref) <= 1e-7
+    # the same sum without the eigenvectors (unitarity)
+    V = O.velocity_matrix_elements(ex)
+    pair = sum(np.sum(np.abs(V[:, :, iv, ex.nv + ic]) ** 2) for ic in range(ex.nc) for iv in range(ex.nv))
+    assert abs(pair - ref) <= 1e-7
+
+
+def test_exciton_spectrum_is_a_sum_of_unit_lorentzians():
+    """One state of energy e and strength v: the peak is at e, its height pi/(nk vcell)
+    |v|^2/e / (pi eta) and its area (in a.u. of frequency) pi/(nk vcell) |v|^2/e."""
+    e, eta, nk, vcell = 2.0, 0.05, 100, 5.4
+    vme = np.array([[1.0 + 0.5j], [0.2], [0.0]])
+    w, sig = O.exciton_spectrum(vme, [e], nk, vcell, w0=0.0, wrange=6.0, nw=6000, eta=eta)
+    amp = np.pi / (nk * vcell / O.BOHR ** 2) * abs(vme[0, 0]) ** 2 / (e / O.HARTREE)
+    i = int(np.argmax(sig[:, 0, 0]))
+    assert abs(w[i] - e) < 2e-3
+    assert abs(sig[i, 0, 0] - amp / (np.pi * eta / O.HARTREE)) < 1e-3 * sig[i, 0, 0]
+    area = np.sum(sig[:, 0, 0]) * (w[1] - w[0]) / O.HARTREE
+    assert abs(area - amp) < 0.02 * amp  # tails beyond the window
+    assert np.allclose(sig[:, 0, 1], sig[:, 1, 0])

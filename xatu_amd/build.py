@@ -36,6 +36,7 @@ DEVICE_SOURCES = [
     "device/matvec.hip",
     "device/vec_kernels.hip",
     "device/hf.hip",
+    "device/band_adjoint.hip",
     "device/capi.hip",
     "device/davidson.cpp",
 ]
@@ -44,6 +45,7 @@ HOST_SOURCES = [
     "host/configuration.cpp",
     "host/system.cpp",
     "host/exciton.cpp",
+    "host/optics.cpp",
     "host/pybind.cpp",
 ]
 

@@ -30,7 +30,8 @@ EXPORTED = [
     "xatu_create", "xatu_destroy", "xatu_last_error", "xatu_set_lattice", "xatu_set_bands",
     "xatu_fourier_sums", "xatu_build_bse", "xatu_build_bse_basis", "xatu_get_bse", "xatu_free_bse",
     "xatu_solve", "xatu_bse_matvec", "xatu_hf_gradients", "xatu_get_fourier", "xatu_get_cells",
-    "xatu_compare_builds", "xatu_get_timings",
+    "xatu_compare_builds", "xatu_get_timings", "xatu_band_gradients",
+    "xatu_oscillator_strengths",
 ]
 
 
@@ -88,6 +89,8 @@ def load(path: Path | str | None = None) -> ctypes.CDLL:
         L.xatu_solve.argtypes = [_vp, ctypes.POINTER(SolveOptions), _vp, _vp, ctypes.POINTER(SolveReport)]
         L.xatu_bse_matvec.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _vp]
         L.xatu_hf_gradients.argtypes = [_vp, ctypes.c_int, _vp, ctypes.c_int, _vp, _vp]
+        L.xatu_band_gradients.argtypes = [_vp, ctypes.c_int, _vp, _vp, _vp]
+        L.xatu_oscillator_strengths.argtypes = [_vp, ctypes.c_int, _vp, _vp, _vp]
         L.xatu_get_fourier.argtypes = [_vp, _vp, _vp]
         L.xatu_get_cells.argtypes = [_vp, ctypes.POINTER(ctypes.c_int), _vp]
         L.xatu_compare_builds.argtypes = [_vp, ctypes.c_int, _vp, ctypes.POINTER(ctypes.c_double)]
@@ -141,6 +144,7 @@ class Context:
         o = np.ascontiguousarray(orb_per_atom, dtype=np.int32)
         self.ndim = b.shape[0]
         self.natoms = m.shape[0]
+        self.norb = int(o.sum())
         self.nmesh = mesh_n ** self.ndim
         self._check(self.L.xatu_set_lattice(self._h, self.ndim, _ptr(b), self.natoms, _ptr(m), _ptr(o), mesh_n))
 
@@ -153,6 +157,7 @@ class Context:
         nk = ek.shape[1]
         self.N = nk * nv * nc
         self.nv, self.nc = nv, nc
+        self.nk, self.nb = nk, nv + nc
         self._keep = (ek, vk, ekq, vkq, off)
         self._check(self.L.xatu_set_bands(self._h, nk, nv, nc, _ptr(ek), _ptr(ekq), _ptr(vk), _ptr(vkq),
                                           _ptr(off)))
@@ -213,6 +218,23 @@ class Context:
         g = np.zeros((nstates, len(p)), order="F")
         self._check(self.L.xatu_hf_gradients(self._h, nstates, _ptr(v), len(p), _ptr(p), _ptr(g)))
         return g
+
+    def band_gradients(self, nstates, eigvec=None):
+        """(w, g): w[n, stack, k, b] = dE_n/d eps_b(k); g[n, stack, k, b, orb] = dE_n/d conj C_b(k)[orb]
+        (stack 0: k stack, 1: k+Q stack)."""
+        v = None if eigvec is None else np.asfortranarray(eigvec, dtype=np.complex128)
+        w = np.zeros((nstates, 2, self.nk, self.nb))
+        g = np.zeros((nstates, 2, self.nk, self.nb, self.norb), dtype=np.complex128)
+        self._check(self.L.xatu_band_gradients(self._h, nstates, _ptr(v), _ptr(w), _ptr(g)))
+        return w, g
+
+    def oscillator_strengths(self, velocity, nstates, eigvec=None):
+        """out[j, n] = sum_i X_n(i) velocity[i, j] (velocity: N x 3 complex)."""
+        vel = np.asfortranarray(velocity, dtype=np.complex128)
+        v = None if eigvec is None else np.asfortranarray(eigvec, dtype=np.complex128)
+        out = np.zeros((3, nstates), dtype=np.complex128, order="F")
+        self._check(self.L.xatu_oscillator_strengths(self._h, nstates, _ptr(v), _ptr(vel), _ptr(out)))
+        return out
 
     def get_fourier(self):
         ft = np.zeros((self.natoms, self.natoms, self.nmesh), dtype=np.complex128, order="F")

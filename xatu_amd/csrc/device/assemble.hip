@@ -16,23 +16,12 @@
 // is exactly Hermitian while every store stays coalesced along the rows.
 #include "context.hpp"
 #include "cplx.hpp"
+#include "kmesh.hpp"
+#include "mfma.hpp"
 
 namespace xatu_dev {
 
 namespace {
-
-__device__ inline int mesh_index(const int* __restrict__ kcoord, int ka, int kb, int ndim, int n) {
-    int q = 0, mul = 1;
-    const int half = n / 2;
-    for (int j = 0; j < ndim; ++j) {
-        int d = kcoord[3 * kb + j] - kcoord[3 * ka + j] + half;
-        d %= n;
-        if (d < 0) d += n;
-        q += d * mul;
-        mul *= n;
-    }
-    return q;
-}
 
 // E1_k[a](c,v) = sum_{alpha in a} conj(CKQ_k[alpha,c]) VK_k[alpha,v]
 // E2_k[a](c,v) = sum_b MQ_ab sum_{beta in b} conj(VK_k[beta,v]) CKQ_k[beta,c]
@@ -298,117 +287,158 @@ __global__ __launch_bounds__(TK* TK) void bse_diag_kernel(LatticeDesc lat, int n
                               });
 }
 
-// Matrix-free product Y = H X: one workgroup owns TK row k-points and sweeps every
-// column tile; each tile of H (TK*NCV x TK*NCV) is recomputed once into LDS and
-// applied to up to kMaxVec vectors, staged kVecChunk at a time.
-constexpr int kVecChunk = 8;
-constexpr int kMaxVec = 32;
+// Matrix-free product Y = H X on the matrix cores. The grid is (row tiles) x (column
+// splits) in one dimension: consecutive workgroup ids go round-robin over the XCDs,
+// so split = id % splits keeps a column range, and its slice of the staged vectors,
+// on one XCD's L2. A workgroup stages the coefficients of its TK row k-points once;
+// for every column tile of its range it stages the TK column k-points, recomputes the
+// (TK*NCV) x (TK*NCV) tile of H into LDS (column-major, one k-point pair per thread,
+// the instruction sequence of the build kernel, so the tile equals the stored matrix
+// bit for bit), and multiplies it with the vectors Xt[col][32] on 16x16x4 MFMAs (four
+// real products per complex tile pair). Each wave owns a fixed set of (16-row tile,
+// 16-vector tile) outputs. Partial sums per split go to Yp[split][row][32] and are
+// added in a fixed order by launch_reduce_partials.
+constexpr int matfree_threads(int tk) { return tk * tk >= 256 ? 256 : 64; }
+
+// Stage the coefficient vectors of `count` k-points from kbase into slots of stride
+// `ss` elements, per slot [norb][NB]: conduction columns (k+Q stack), then valence (k).
+template <typename R, int NC, int NV>
+__device__ inline void stage_slots(cx_t<R>* dst, int ss, int norb, int nk, int kbase, int count,
+                                   const double2* __restrict__ evk, const double2* __restrict__ evkq, int nthreads) {
+    constexpr int NB = NC + NV;
+    const int total = count * norb * NB;
+    for (int e = threadIdx.x; e < total; e += nthreads) {
+        const int al = e % norb;
+        const int rest = e / norb;
+        const int band = rest % NB;
+        const int slot = rest / NB;
+        const int k = kbase + slot;
+        cx_t<R> val = mk(R(0), R(0));
+        if (k < nk) {
+            const double2 g = (band < NC) ? evkq[(static_cast<size_t>(k) * NB + (NV + band)) * norb + al]
+                                          : evk[(static_cast<size_t>(k) * NB + (band - NC)) * norb + al];
+            val = to_cx<R>(g);
+        }
+        dst[static_cast<size_t>(slot) * ss + al * NB + band] = val;
+    }
+}
 
 template <typename R, int NA, int NC, int NV, int TK>
-__global__ __launch_bounds__(TK* TK) void bse_matfree_kernel(LatticeDesc lat, int nk, const double2* __restrict__ evk,
-                                                             const double2* __restrict__ evkq,
-                                                             const double* __restrict__ eval_k,
-                                                             const double* __restrict__ eval_kq,
-                                                             const int* __restrict__ kcoord,
-                                                             const double2* __restrict__ ft,
-                                                             const double2* __restrict__ e1,
-                                                             const double2* __restrict__ e2, int exchange,
-                                                             double scissor, const cx_t<R>* __restrict__ X,
-                                                             cx_t<R>* __restrict__ Y, int nvec) {
+__global__ __launch_bounds__(matfree_threads(TK)) void bse_matfree_kernel(
+    LatticeDesc lat, int nk, const double2* __restrict__ evk, const double2* __restrict__ evkq,
+    const double* __restrict__ eval_k, const double* __restrict__ eval_kq, const int* __restrict__ kcoord,
+    const double2* __restrict__ ft, const double2* __restrict__ e1, const double2* __restrict__ e2, int exchange,
+    double scissor, const cx_t<R>* __restrict__ Xt, int nvec, int splits, int tiles_per_split,
+    cx_t<R>* __restrict__ Yp) {
     using C = cx_t<R>;
+    using M = Mfma<R>;
+    using acc_t = typename M::acc_t;
     constexpr int NB = NC + NV;
     constexpr int NCV = NC * NV;
     constexpr int ROWS = TK * NCV;
-    constexpr int NT = TK * TK;
-    constexpr int NCHUNK = kMaxVec / kVecChunk;
-    constexpr int PER = (ROWS * kVecChunk + NT - 1) / NT;
-    constexpr int LDH = ROWS + 1;  // column-major H tile, one element of padding
+    constexpr int ROWS16 = (ROWS + 15) / 16 * 16;  // MFMA-padded tile size
+    constexpr int LDH = ROWS16 + 1;                // column-major tile, one element of padding
+    constexpr int NT = matfree_threads(TK);
+    constexpr int WPB = NT / 64;
+    constexpr int RT = ROWS16 / 16;
+    constexpr int MAXC = (2 * RT + WPB - 1) / WPB;  // output tiles per wave (up to 2 vector tiles)
     extern __shared__ unsigned char smem_raw[];
     const int norb = lat.norb;
-    C* s_coef = reinterpret_cast<C*>(smem_raw);                  // 2*TK*norb*NB
-    C* s_h = s_coef + static_cast<size_t>(2 * TK) * norb * NB;   // ROWS x ROWS, column-major, ld LDH
-    C* s_x = s_h + ROWS * LDH;                                  // kVecChunk x ROWS, [j][r]
-    const int64_t N = static_cast<int64_t>(nk) * NCV;
-    const int Kr = blockIdx.x;
+    const int ss = norb * NB + 1;  // slot stride, padded against LDS bank conflicts
+    C* s_row = reinterpret_cast<C*>(smem_raw);
+    C* s_col = s_row + static_cast<size_t>(TK) * ss;
+    C* s_h = s_col + static_cast<size_t>(TK) * ss;
+
+    const int split = blockIdx.x % splits;
+    const int Kr = blockIdx.x / splits;
     const int nT = (nk + TK - 1) / TK;
+    const int kc0 = split * tiles_per_split;
+    const int kc1 = min(nT, kc0 + tiles_per_split);
+    const int64_t N = static_cast<int64_t>(nk) * NCV;
     const int t = threadIdx.x;
-    const int tr = t % TK, tc = t / TK;
-    const int nchunk = (nvec + kVecChunk - 1) / kVecChunk;
+    const int lane = t & 63, wave = t >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const int ntiles = RT * ((nvec + 15) / 16);
 
-    C acc[NCHUNK][PER];
+    stage_slots<R, NC, NV>(s_row, ss, norb, nk, Kr * TK, TK, evk, evkq, NT);
+    // padding rows/columns are never written by the build step: start from zero
+    for (int e = t; e < ROWS16 * LDH; e += NT) s_h[e] = mk(R(0), R(0));
+    acc_t accR[MAXC], accI[MAXC];
 #pragma unroll
-    for (int q = 0; q < NCHUNK; ++q)
-#pragma unroll
-        for (int m = 0; m < PER; ++m) acc[q][m] = mk(R(0), R(0));
+    for (int i = 0; i < MAXC; ++i) accR[i] = accI[i] = acc_t{0, 0, 0, 0};
 
-    for (int Kc = 0; Kc < nT; ++Kc) {
+    for (int Kc = kc0; Kc < kc1; ++Kc) {
+        __syncthreads();  // the previous tile has been consumed
+        stage_slots<R, NC, NV>(s_col, ss, norb, nk, Kc * TK, TK, evk, evkq, NT);
         __syncthreads();
-        stage_coefficients<R, NC, NV, TK>(s_coef, norb, nk, Kr, Kc, evk, evkq, NT);
-        for (int e = t; e < ROWS * LDH; e += NT) s_h[e] = mk(R(0), R(0));
-        __syncthreads();
-        const int kr = Kr * TK + tr, kc = Kc * TK + tc;
-        if (kr < nk && kc < nk) {
-            const bool sw = kr > kc;
-            const int ka = sw ? kc : kr;
-            const int kb = sw ? kr : kc;
-            const C* ca_ = s_coef + static_cast<size_t>(sw ? (TK + tc) : tr) * norb * NB;
-            const C* cb_ = s_coef + static_cast<size_t>(sw ? tr : (TK + tc)) * norb * NB;
-            const bool diagblk = (ka == kb);
-            pair_block<R, NA, NC, NV>(lat, ca_, cb_, ka, kb, ft, kcoord, e1, e2, exchange,
-                                      [&](int cA, int vA, int cB, int vB, C DX) {
-                                          const int ia = cA * NV + vA, ib = cB * NV + vB;
-                                          if (diagblk) {
-                                              const int base = tr * NCV;
-                                              if (ia < ib) {
-                                                  s_h[(tc * NCV + ib) * LDH + base + ia] = mk(-DX.x, -DX.y);
-                                                  s_h[(tc * NCV + ia) * LDH + base + ib] = mk(-DX.x, DX.y);
-                                              } else if (ia == ib) {
-                                                  const double h = diag_value(eval_k, eval_kq, NB, NV, ka, cA, vA,
-                                                                              scissor, static_cast<double>(DX.x));
-                                                  s_h[(tc * NCV + ia) * LDH + base + ia] = mk(R(h), R(0));
+        for (int p = t; p < TK * TK; p += NT) {
+            const int tr = p % TK, tc = p / TK;
+            const int kr = Kr * TK + tr, kc = Kc * TK + tc;
+            if (kr < nk && kc < nk) {
+                const bool sw = kr > kc;
+                const int ka = sw ? kc : kr;
+                const int kb = sw ? kr : kc;
+                const C* ca_ = sw ? s_col + static_cast<size_t>(tc) * ss : s_row + static_cast<size_t>(tr) * ss;
+                const C* cb_ = sw ? s_row + static_cast<size_t>(tr) * ss : s_col + static_cast<size_t>(tc) * ss;
+                const bool diagblk = (ka == kb);
+                pair_block<R, NA, NC, NV>(lat, ca_, cb_, ka, kb, ft, kcoord, e1, e2, exchange,
+                                          [&](int cA, int vA, int cB, int vB, C DX) {
+                                              const int ia = cA * NV + vA, ib = cB * NV + vB;
+                                              // element (row, col) of the tile at s_h[col * LDH + row]
+                                              const int r0 = tr * NCV, c0 = tc * NCV;
+                                              if (diagblk) {
+                                                  if (ia < ib) {
+                                                      s_h[(c0 + ib) * LDH + r0 + ia] = mk(-DX.x, -DX.y);
+                                                      s_h[(c0 + ia) * LDH + r0 + ib] = mk(-DX.x, DX.y);
+                                                  } else if (ia == ib) {
+                                                      const double h = diag_value(eval_k, eval_kq, NB, NV, ka, cA, vA,
+                                                                                  scissor, static_cast<double>(DX.x));
+                                                      s_h[(c0 + ia) * LDH + r0 + ia] = mk(R(h), R(0));
+                                                  }
+                                              } else if (!sw) {
+                                                  s_h[(c0 + ib) * LDH + r0 + ia] = mk(-DX.x, -DX.y);
+                                              } else {
+                                                  s_h[(c0 + ia) * LDH + r0 + ib] = mk(-DX.x, DX.y);
                                               }
-                                          } else if (!sw) {
-                                              s_h[(tc * NCV + ib) * LDH + tr * NCV + ia] = mk(-DX.x, -DX.y);
-                                          } else {
-                                              s_h[(tc * NCV + ia) * LDH + tr * NCV + ib] = mk(-DX.x, DX.y);
-                                          }
-                                      });
+                                          });
+            }
         }
+        __syncthreads();
+        const int64_t cbase = static_cast<int64_t>(Kc) * ROWS;
 #pragma unroll
-        for (int q = 0; q < NCHUNK; ++q) {
-            if (q < nchunk) {
-                __syncthreads();
-                for (int e = t; e < ROWS * kVecChunk; e += NT) {
-                    const int r = e % ROWS, j = e / ROWS;
-                    const int64_t row = static_cast<int64_t>(Kc) * ROWS + r;
-                    const int vj = q * kVecChunk + j;
-                    s_x[j * ROWS + r] = (row < N && vj < nvec) ? X[row + static_cast<int64_t>(vj) * N]
-                                                                    : mk(R(0), R(0));
-                }
-                __syncthreads();
+        for (int i = 0; i < MAXC; ++i) {
+            const int tile = wave + WPB * i;
+            if (tile < ntiles) {
+                const int rt = tile % RT, vt = tile / RT;
+                acc_t aR = accR[i], aI = accI[i];
 #pragma unroll
-                for (int m = 0; m < PER; ++m) {
-                    const int e = t + NT * m;
-                    if (e < ROWS * kVecChunk) {
-                        const int r = e % ROWS, j = e / ROWS;
-                        C sum = acc[q][m];
-                        for (int cidx = 0; cidx < ROWS; ++cidx) cfma(sum, s_h[cidx * LDH + r], s_x[j * ROWS + cidx]);
-                        acc[q][m] = sum;
-                    }
+                for (int kk = 0; kk < ROWS16 / 4; ++kk) {
+                    const int cl = 4 * kk + lk;
+                    const C a = s_h[cl * LDH + 16 * rt + li];
+                    const int64_t col = cbase + cl;
+                    // columns past the tile or the matrix meet zero vectors (tile entries there are finite)
+                    const C b = (cl < ROWS && col < N) ? Xt[col * kBlockVectors + 16 * vt + li] : mk(R(0), R(0));
+                    aR = M::mma(a.x, b.x, aR);
+                    aR = M::mma(a.y, -b.y, aR);
+                    aI = M::mma(a.x, b.y, aI);
+                    aI = M::mma(a.y, b.x, aI);
                 }
+                accR[i] = aR;
+                accI[i] = aI;
             }
         }
     }
+    C* out = Yp + static_cast<int64_t>(split) * N * kBlockVectors;
 #pragma unroll
-    for (int q = 0; q < NCHUNK; ++q) {
+    for (int i = 0; i < MAXC; ++i) {
+        const int tile = wave + WPB * i;
+        if (tile < ntiles) {
+            const int rt = tile % RT, vt = tile / RT;
 #pragma unroll
-        for (int m = 0; m < PER; ++m) {
-            const int e = t + NT * m;
-            if (q < nchunk && e < ROWS * kVecChunk) {
-                const int r = e % ROWS, j = e / ROWS;
-                const int64_t row = static_cast<int64_t>(Kr) * ROWS + r;
-                const int vj = q * kVecChunk + j;
-                if (row < N && vj < nvec) Y[row + static_cast<int64_t>(vj) * N] = acc[q][m];
+            for (int v = 0; v < 4; ++v) {
+                const int rl = 16 * rt + M::out_row(lk, v);
+                const int64_t row = static_cast<int64_t>(Kr) * ROWS + rl;
+                if (rl < ROWS && row < N) out[row * kBlockVectors + 16 * vt + li] = mk(accR[i][v], accI[i][v]);
             }
         }
     }
@@ -549,33 +579,40 @@ struct Shape {
 
     template <int TK>
     static size_t matfree_bytes(const Context& c) {
-        constexpr int ROWS = TK * NCV;
-        return coef_bytes<R, TK>(c, NB) + static_cast<size_t>(ROWS) * (ROWS + 1) * sizeof(cx_t<R>) +
-               static_cast<size_t>(ROWS) * kVecChunk * sizeof(cx_t<R>);
+        constexpr int ROWS16 = (TK * NCV + 15) / 16 * 16;
+        return (static_cast<size_t>(2 * TK) * (c.lat.norb * NB + 1) + static_cast<size_t>(ROWS16) * (ROWS16 + 1)) *
+               sizeof(cx_t<R>);
     }
     template <int TK>
-    static void matfree_tk(const Context& c, const cx_t<R>* X, cx_t<R>* Y, int nvec, hipStream_t s) {
+    static void matfree_tk(Context& c, const double2* X, double2* Y, int nvec, hipStream_t s,
+                           DevBuf<cx_t<R>>& xt, DevBuf<cx_t<R>>& yp) {
         const int nT = (c.nk + TK - 1) / TK;
         const size_t lds = matfree_bytes<TK>(c);
-        for (int v0 = 0; v0 < nvec; v0 += kMaxVec) {
-            const int nv = std::min(kMaxVec, nvec - v0);
-            matfree_launch<TK>(c, X + static_cast<int64_t>(v0) * c.N, Y + static_cast<int64_t>(v0) * c.N, nv, nT,
-                               lds, s);
+        // about 1024 workgroups or more; splits a multiple of the XCD count
+        int splits = std::max(1, (1024 + nT - 1) / nT);
+        splits = std::min((splits + kXcds - 1) / kXcds * kXcds, std::max(kXcds, nT));
+        const int tiles_per_split = (nT + splits - 1) / splits;
+        xt.alloc(std::max<size_t>(xt.n, static_cast<size_t>(c.N) * kBlockVectors));
+        yp.alloc(std::max<size_t>(yp.n, static_cast<size_t>(splits) * c.N * kBlockVectors));
+        for (int v0 = 0; v0 < nvec; v0 += kBlockVectors) {
+            const int nv = std::min(kBlockVectors, nvec - v0);
+            launch_stage_vectors(X + static_cast<int64_t>(v0) * c.N, c.N, nv, xt.p, s);
+            hipLaunchKernelGGL((bse_matfree_kernel<R, NA, NC, NV, TK>), dim3(nT * splits), dim3(matfree_threads(TK)),
+                               lds, s, c.lat, c.nk, c.d_evec_k.p, c.d_evec_kq.p, c.d_eval_k.p, c.d_eval_kq.p,
+                               c.d_kcoord.p, c.d_ft.p, c.d_e1.p, c.d_e2.p, c.inter.exchange, c.scissor, xt.p, nv,
+                               splits, tiles_per_split, yp.p);
+            launch_reduce_partials(yp.p, c.N, nv, splits, Y + static_cast<int64_t>(v0) * c.N, s);
         }
     }
-    template <int TK>
-    static void matfree_launch(const Context& c, const cx_t<R>* X, cx_t<R>* Y, int nvec, int nT, size_t lds,
-                               hipStream_t s) {
-        hipLaunchKernelGGL((bse_matfree_kernel<R, NA, NC, NV, TK>), dim3(nT), dim3(TK * TK), lds, s, c.lat, c.nk, c.d_evec_k.p, c.d_evec_kq.p, c.d_eval_k.p, c.d_eval_kq.p, c.d_kcoord.p,
-                           c.d_ft.p, c.d_e1.p, c.d_e2.p, c.inter.exchange, c.scissor, X, Y, nvec);
-    }
-    static void matfree(const Context& c, const cx_t<R>* X, cx_t<R>* Y, int nvec, hipStream_t s) {
-        if (matfree_bytes<16>(c) <= 160 * 1024) {
-            matfree_tk<16>(c, X, Y, nvec, s);
-        } else if (matfree_bytes<8>(c) <= 160 * 1024) {
-            matfree_tk<8>(c, X, Y, nvec, s);
-        } else if (matfree_bytes<4>(c) <= 160 * 1024) {
-            matfree_tk<4>(c, X, Y, nvec, s);
+    static void matfree(Context& c, const double2* X, double2* Y, int nvec, hipStream_t s,
+                        DevBuf<cx_t<R>>& xt, DevBuf<cx_t<R>>& yp) {
+        constexpr size_t kLds = 160 * 1024;
+        if (matfree_bytes<16>(c) <= kLds) {
+            matfree_tk<16>(c, X, Y, nvec, s, xt, yp);
+        } else if (matfree_bytes<8>(c) <= kLds) {
+            matfree_tk<8>(c, X, Y, nvec, s, xt, yp);
+        } else if (matfree_bytes<4>(c) <= kLds) {
+            matfree_tk<4>(c, X, Y, nvec, s, xt, yp);
         } else {
             throw Error(XATU_ERR_INVALID_ARGUMENT, "too many orbitals for the matrix-free kernel");
         }
@@ -666,14 +703,14 @@ void launch_diag_direct(const Context& c, double* diag, hipStream_t s) {
     }
 }
 
-void launch_matfree64(const Context& c, const double2* X, double2* Y, int nvec, hipStream_t s) {
-    if (!for_shape<double>(c, [&](auto sh) { decltype(sh)::matfree(c, X, Y, nvec, s); })) {
+void launch_matfree64(Context& c, const double2* X, double2* Y, int nvec, hipStream_t s) {
+    if (!for_shape<double>(c, [&](auto sh) { decltype(sh)::matfree(c, X, Y, nvec, s, c.s_xt64, c.s_yp64); })) {
         throw Error(XATU_ERR_INVALID_ARGUMENT, kShapeMsg);
     }
 }
 
-void launch_matfree32(const Context& c, const float2* X, float2* Y, int nvec, hipStream_t s) {
-    if (!for_shape<float>(c, [&](auto sh) { decltype(sh)::matfree(c, X, Y, nvec, s); })) {
+void launch_matfree32(Context& c, const double2* X, double2* Y, int nvec, hipStream_t s) {
+    if (!for_shape<float>(c, [&](auto sh) { decltype(sh)::matfree(c, X, Y, nvec, s, c.s_xt32, c.s_yp32); })) {
         throw Error(XATU_ERR_INVALID_ARGUMENT, kShapeMsg);
     }
 }

@@ -113,14 +113,9 @@ void apply_operator(Context& c, int precision, int matrix_free, const double2* X
             launch_matfree64(c, X, Y, nvec, c.stream);
             ev.stop();
         } else {
-            const size_t n = static_cast<size_t>(c.N) * nvec;
-            c.s_x32.alloc(std::max(c.s_x32.n, n));
-            c.s_y32.alloc(std::max(c.s_y32.n, n));
-            launch_convert_64_to_32(X, c.s_x32.p, n, c.stream);
             ev.start();
-            launch_matfree32(c, c.s_x32.p, c.s_y32.p, nvec, c.stream);
+            launch_matfree32(c, X, Y, nvec, c.stream);
             ev.stop();
-            launch_convert_32_to_64(c.s_y32.p, Y, n, c.stream);
         }
     } else if (precision == XATU_FP64) {
         if (!c.built64) throw Error(XATU_ERR_LOGIC, "FP64 BSE matrix is not resident");
@@ -463,6 +458,48 @@ int xatu_hf_gradients(xatu_ctx* ctx, int nstates, const double* eigvec, int npar
             dv = c.d_last_vec.p;
         }
         hf_gradients(c, nstates, dv, nparams, params, grad_out);
+    });
+}
+
+int xatu_band_gradients(xatu_ctx* ctx, int nstates, const double* eigvec, double* w_out, double* g_out) {
+    return guarded(ctx, [&](Context& c) {
+        c.require_ft();
+        if (nstates < 1 || !w_out || !g_out) throw Error(XATU_ERR_INVALID_ARGUMENT, "bad args");
+        DevBuf<double2> V;
+        const double2* dv;
+        if (eigvec) {
+            V.alloc(static_cast<size_t>(c.N) * nstates);
+            XATU_HIP(hipMemcpyAsync(V.p, eigvec, V.bytes(), hipMemcpyHostToDevice, c.stream));
+            dv = V.p;
+        } else {
+            if (c.last_nstates < nstates) throw Error(XATU_ERR_LOGIC, "no eigenvectors from a previous solve");
+            dv = c.d_last_vec.p;
+        }
+        band_gradients(c, nstates, dv, w_out, g_out);
+    });
+}
+
+int xatu_oscillator_strengths(xatu_ctx* ctx, int nstates, const double* eigvec, const double* velocity,
+                              double* out) {
+    return guarded(ctx, [&](Context& c) {
+        c.require_bands();
+        if (nstates < 1 || !velocity || !out) throw Error(XATU_ERR_INVALID_ARGUMENT, "bad args");
+        DevBuf<double2> V, vel, o;
+        const double2* dv;
+        if (eigvec) {
+            V.alloc(static_cast<size_t>(c.N) * nstates);
+            XATU_HIP(hipMemcpyAsync(V.p, eigvec, V.bytes(), hipMemcpyHostToDevice, c.stream));
+            dv = V.p;
+        } else {
+            if (c.last_nstates < nstates) throw Error(XATU_ERR_LOGIC, "no eigenvectors from a previous solve");
+            dv = c.d_last_vec.p;
+        }
+        vel.alloc(static_cast<size_t>(c.N) * 3);
+        o.alloc(static_cast<size_t>(3) * nstates);
+        XATU_HIP(hipMemcpyAsync(vel.p, velocity, vel.bytes(), hipMemcpyHostToDevice, c.stream));
+        launch_oscillator(dv, c.N, nstates, vel.p, o.p, c.stream);
+        XATU_HIP(hipMemcpyAsync(out, o.p, o.bytes(), hipMemcpyDeviceToHost, c.stream));
+        XATU_HIP(hipStreamSynchronize(c.stream));
     });
 }
 

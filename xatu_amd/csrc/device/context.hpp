@@ -131,7 +131,6 @@ struct Context {
     DevBuf<double> d_diag;
 
     // scratch for products and projections
-    DevBuf<float2> s_x32, s_y32;      // FP32 copies of vectors (matrix-free FP32 products)
     DevBuf<double2> s_xt64, s_yp64;   // staged vectors / split partial sums, FP64 dense product
     DevBuf<float2> s_xt32, s_yp32;    // same, FP32 dense product
     DevBuf<double2> s_gram;           // split partial sums of the Gram kernel
@@ -197,13 +196,16 @@ void launch_matvec32(Context& c, const double2* X, double2* Y, int nvec, hipStre
 // C (m x n, leading dimension ldc) = A^H B over the N rows of A (lda) and B (ldb), FP64.
 void launch_gram(Context& c, const double2* A, int lda, const double2* B, int ldb, int m, int n, double2* C, int ldc,
                  hipStream_t s);
-// Matrix-free product in FP64 / FP32 arithmetic (tiles recomputed).
-void launch_matfree64(const Context& c, const double2* X, double2* Y, int nvec, hipStream_t s);
-void launch_matfree32(const Context& c, const float2* X, float2* Y, int nvec, hipStream_t s);
-
-// Small vector kernels used by the solver.
-void launch_convert_64_to_32(const double2* x, float2* y, size_t n, hipStream_t s);
-void launch_convert_32_to_64(const float2* x, double2* y, size_t n, hipStream_t s);
+// Matrix-free products Y = H X (complex FP64 vectors) with tiles of H recomputed in
+// FP64 / FP32 arithmetic inside the product (FP32: FP32 tiles, FP32 accumulation).
+void launch_matfree64(Context& c, const double2* X, double2* Y, int nvec, hipStream_t s);
+void launch_matfree32(Context& c, const double2* X, double2* Y, int nvec, hipStream_t s);
+// Vector blocks of the products: Xt[row][32] (zero padded beyond nvec <= 32) in the
+// product's precision, and the fixed-order sum Y[row + n N] = sum_s Yp[s][row][n].
+void launch_stage_vectors(const double2* X, int64_t N, int nvec, double2* Xt, hipStream_t s);
+void launch_stage_vectors(const double2* X, int64_t N, int nvec, float2* Xt, hipStream_t s);
+void launch_reduce_partials(const double2* Yp, int64_t N, int nvec, int splits, double2* Y, hipStream_t s);
+void launch_reduce_partials(const float2* Yp, int64_t N, int nvec, int splits, double2* Y, hipStream_t s);
 
 // Solver entry points (davidson.cpp).
 void solve_dense_diag(Context& c, int nstates, double* eigval_out, double* eigvec_out);
@@ -215,5 +217,9 @@ void apply_operator(Context& c, int precision, int matrix_free, const double2* X
 // Hellmann-Feynman gradients (hf.hip / capi.cpp).
 void hf_gradients(Context& c, int nstates, const double2* d_vec, int nparams, const int* params,
                   double* grad_out);
+// out[j + 3 n] = sum_i X[i + n N] vel[i + j N] (oscillator strengths, vec_kernels.hip).
+void launch_oscillator(const double2* X, int64_t N, int nstates, const double2* vel, double2* out, hipStream_t s);
+// Band-data adjoint of the BSE assembly (band_adjoint.hip).
+void band_gradients(Context& c, int nstates, const double2* d_vec, double* w_out, double* g_out);
 
 }  // namespace xatu_dev

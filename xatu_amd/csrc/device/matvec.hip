@@ -3,47 +3,24 @@
 //
 // Product: H is column-major complex (FP32: float2, FP64: double2), N x N. The
 // vectors are staged transposed, Xt[j][n] (32 vectors, zero padded). Each wave owns
-// 64 rows (four 16-row tiles) and all 32 vectors (two 16-column tiles) and walks the
+// 32 rows (two 16-row tiles) and all 32 vectors (two 16-column tiles) and walks the
 // columns of H four at a time: lane l loads the complex H(row = l%16 + 16 t, col =
-// j + l/16), so the 16x16x4 MFMA sees A[i][k] = Re H(i, j+k) in one product and
-// Im H(i, j+k) in the other. With B[k][n] built from X(j+k, n) the complex product
-// is four real MFMA products per tile pair (Re Re, Im (-Im), Re Im, Im Re), exact in
-// the sense of the usual 4-real-product complex multiply. Column ranges are split
-// across workgroups (split-K) to fill all 256 CUs; partial results go to a workspace
-// and a reduction kernel sums them and writes Y in the caller's column-major layout.
+// j + l/16), so the 16x16x4 MFMA sees A[i][k] = Re H(i, j+k) (or Im, or Re + Im) and
+// B[k][n] built from X(j+k, n). FP64 uses four real MFMA products per tile pair
+// (Re Re, Im (-Im), Re Im, Im Re); FP32 uses the three-product form. The loads of
+// several 4-column steps are issued before their MFMAs to keep more bytes in flight.
+// Column ranges are split across workgroups (split-K, XCD-aware) to fill all 256 CUs;
+// partial results go to a workspace and a reduction kernel sums them in a fixed order
+// and writes Y in the caller's column-major layout.
 #include "context.hpp"
 #include "cplx.hpp"
+#include "mfma.hpp"
 
 namespace xatu_dev {
 
 namespace {
 
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-typedef double f64x4 __attribute__((ext_vector_type(4)));
-
-constexpr int kVec = 32;         // vectors per product launch
-constexpr int kWaveRows = 64;    // rows per wave
-constexpr int kBlockRows = 256;  // rows per workgroup (4 waves)
-
-template <typename R>
-struct Mfma;
-// Both take A[i][k] from lane i + 16 k and B[k][j] from lane j + 16 k. Output
-// register v of lane l holds C[row][l % 16] with row = 4 (l / 16) + v for the FP32
-// instruction and row = l / 16 + 4 v for the FP64 one (checked on gfx950).
-template <>
-struct Mfma<float> {
-    using acc_t = f32x4;
-    __device__ static acc_t mma(float a, float b, acc_t c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
-    __device__ static int out_row(int lk, int v) { return 4 * lk + v; }
-};
-template <>
-struct Mfma<double> {
-    using acc_t = f64x4;
-    __device__ static acc_t mma(double a, double b, acc_t c) {
-        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-    }
-    __device__ static int out_row(int lk, int v) { return lk + 4 * v; }
-};
+constexpr int kVec = kBlockVectors;
 
 // Xt[j][n] = X[j + n N] for n < nvec, 0 otherwise (j < N, n < 32).
 template <typename Rin, typename R>
@@ -60,64 +37,110 @@ __global__ void stage_vectors_kernel(const cx_t<Rin>* __restrict__ X, int64_t N,
     Xt[idx] = v;
 }
 
-// grid: (ceil(N / 256), splits); block 256. Partial sums to Yp[split][row][32].
+// Tile parameters of the product: TW 16-row tiles per wave (4 waves per workgroup, so
+// 64 TW rows per workgroup) and U 4-column steps whose loads are issued together.
+// FP32 uses the 3-real-product complex multiply (Gauss): with H = a + ib and x = c + id,
+// k1 = (a + b) c, k2 = a (d - c), k3 = b (c + d), Re = k1 - k3, Im = k1 + k2.
+template <typename R>
+struct ProductTiling;
+template <>
+struct ProductTiling<float> {
+    static constexpr int TW = 2, U = 4;
+    static constexpr bool G3M = true;
+};
+template <>
+struct ProductTiling<double> {
+    static constexpr int TW = 2, U = 2;
+    static constexpr bool G3M = false;
+};
+
+// 1-D grid of row_blocks * splits workgroups, block 256. Workgroup id -> (split = id %
+// splits, row block = id / splits): with splits a multiple of 8 every split, and with
+// it its slice of Xt, lives on one XCD and its L2. Partial sums to Yp[split][row][32].
 template <typename R>
 __global__ __launch_bounds__(256) void dense_product_kernel(const cx_t<R>* __restrict__ H, const cx_t<R>* __restrict__ Xt,
-                                                            int64_t N, int64_t kchunk, cx_t<R>* __restrict__ Yp) {
+                                                            int64_t N, int64_t kchunk, int splits,
+                                                            cx_t<R>* __restrict__ Yp) {
     using M = Mfma<R>;
     using acc_t = typename M::acc_t;
+    using T = ProductTiling<R>;
+    constexpr int TW = T::TW, U = T::U;
+    constexpr int NACC = T::G3M ? 3 : 2;
+    const int sp = blockIdx.x % splits;
+    const int rb = blockIdx.x / splits;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int64_t row0 = static_cast<int64_t>(blockIdx.x) * kBlockRows + wave * kWaveRows;
-    const int64_t kbeg = static_cast<int64_t>(blockIdx.y) * kchunk;
+    const int64_t row0 = static_cast<int64_t>(rb) * (64 * TW) + wave * (16 * TW);
+    const int64_t kbeg = static_cast<int64_t>(sp) * kchunk;
     const int64_t kend = min(N, kbeg + kchunk);
     const int li = lane & 15;  // row within a 16-row tile / vector within a 16-vector tile
-    const int lk = lane >> 4;  // column offset 0..3 within the 4-column step
+    const int lk = lane >> 4;  // column offset 0..3 within a 4-column step
 
-    acc_t accR[4][2], accI[4][2];
+    acc_t acc[TW][2][NACC];
 #pragma unroll
-    for (int t = 0; t < 4; ++t)
+    for (int t = 0; t < TW; ++t)
 #pragma unroll
-        for (int u = 0; u < 2; ++u) {
-            accR[t][u] = acc_t{0, 0, 0, 0};
-            accI[t][u] = acc_t{0, 0, 0, 0};
-        }
+        for (int u = 0; u < 2; ++u)
+#pragma unroll
+            for (int a = 0; a < NACC; ++a) acc[t][u][a] = acc_t{0, 0, 0, 0};
 
-    int64_t rows[4];
-    bool rok[4];
+    int64_t rows[TW];
+    bool rok[TW];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
+    for (int t = 0; t < TW; ++t) {
         rows[t] = row0 + 16 * t + li;
         rok[t] = rows[t] < N;
     }
 
-    for (int64_t j = kbeg; j < kend; j += 4) {
-        const int64_t col = j + lk;
-        const bool cok = col < kend;
-        cx_t<R> h[4], x[2];
+    for (int64_t j = kbeg; j < kend; j += 4 * U) {
+        cx_t<R> h[U][TW], x[U][2];
 #pragma unroll
-        for (int t = 0; t < 4; ++t) h[t] = (cok && rok[t]) ? H[rows[t] + col * N] : mk(R(0), R(0));
+        for (int s = 0; s < U; ++s) {
+            const int64_t col = j + 4 * s + lk;
+            const bool cok = col < kend;
 #pragma unroll
-        for (int u = 0; u < 2; ++u) x[u] = cok ? Xt[col * kVec + 16 * u + li] : mk(R(0), R(0));
+            for (int t = 0; t < TW; ++t) h[s][t] = (cok && rok[t]) ? H[rows[t] + col * N] : mk(R(0), R(0));
 #pragma unroll
-        for (int t = 0; t < 4; ++t)
+            for (int u = 0; u < 2; ++u) x[s][u] = cok ? Xt[col * kVec + 16 * u + li] : mk(R(0), R(0));
+        }
 #pragma unroll
-            for (int u = 0; u < 2; ++u) {
-                accR[t][u] = M::mma(h[t].x, x[u].x, accR[t][u]);
-                accR[t][u] = M::mma(h[t].y, -x[u].y, accR[t][u]);
-                accI[t][u] = M::mma(h[t].x, x[u].y, accI[t][u]);
-                accI[t][u] = M::mma(h[t].y, x[u].x, accI[t][u]);
-            }
+        for (int s = 0; s < U; ++s)
+#pragma unroll
+            for (int t = 0; t < TW; ++t)
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    const cx_t<R> hv = h[s][t], xv = x[s][u];
+                    if (T::G3M) {
+                        acc[t][u][0] = M::mma(hv.x + hv.y, xv.x, acc[t][u][0]);
+                        acc[t][u][1] = M::mma(hv.x, xv.y - xv.x, acc[t][u][1]);
+                        acc[t][u][2] = M::mma(hv.y, xv.x + xv.y, acc[t][u][2]);
+                    } else {
+                        acc[t][u][0] = M::mma(hv.x, xv.x, acc[t][u][0]);
+                        acc[t][u][0] = M::mma(hv.y, -xv.y, acc[t][u][0]);
+                        acc[t][u][1] = M::mma(hv.x, xv.y, acc[t][u][1]);
+                        acc[t][u][1] = M::mma(hv.y, xv.x, acc[t][u][1]);
+                    }
+                }
     }
-    cx_t<R>* out = Yp + static_cast<int64_t>(blockIdx.y) * N * kVec;
+    cx_t<R>* out = Yp + static_cast<int64_t>(sp) * N * kVec;
 #pragma unroll
-    for (int t = 0; t < 4; ++t)
+    for (int t = 0; t < TW; ++t)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int64_t r = row0 + 16 * t + M::out_row(lk, v);
             if (r < N) {
 #pragma unroll
-                for (int u = 0; u < 2; ++u) out[r * kVec + 16 * u + li] = mk(accR[t][u][v], accI[t][u][v]);
+                for (int u = 0; u < 2; ++u) {
+                    R re, im;
+                    if (T::G3M) {
+                        re = acc[t][u][0][v] - acc[t][u][2][v];
+                        im = acc[t][u][0][v] + acc[t][u][1][v];
+                    } else {
+                        re = acc[t][u][0][v];
+                        im = acc[t][u][1][v];
+                    }
+                    out[r * kVec + 16 * u + li] = mk(re, im);
+                }
             }
         }
 }
@@ -194,28 +217,51 @@ __global__ __launch_bounds__(256) void gram_partial_kernel(const double2* __rest
         }
 }
 
-__global__ void gram_reduce_kernel(const double2* __restrict__ P, int m, int n, int chunks, double2* __restrict__ C,
-                                   int ldc) {
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= m * n) return;
-    const int i = idx % m, j = idx / m;
+// C[i + j ldc] = sum over chunks of P[chunk][i + j m]. One workgroup of 1024 threads
+// per 64 outputs: 16 slices each sum every 16th chunk, then the slices are added in
+// a fixed order, so the result does not depend on scheduling.
+constexpr int kRedOut = 64, kRedSlices = 16;
+__global__ __launch_bounds__(kRedOut* kRedSlices) void gram_reduce_kernel(const double2* __restrict__ P, int m, int n,
+                                                                          int chunks, double2* __restrict__ C,
+                                                                          int ldc) {
+    __shared__ double2 part[kRedSlices][kRedOut];
+    const int o = threadIdx.x % kRedOut, sl = threadIdx.x / kRedOut;
+    const int idx = blockIdx.x * kRedOut + o;
+    const int64_t mn = static_cast<int64_t>(m) * n;
     double sr = 0.0, si = 0.0;
-    for (int c = 0; c < chunks; ++c) {
-        const double2 v = P[static_cast<int64_t>(c) * m * n + idx];
-        sr += v.x;
-        si += v.y;
+    if (idx < mn) {
+#pragma unroll 4
+        for (int c = sl; c < chunks; c += kRedSlices) {
+            const double2 v = P[static_cast<int64_t>(c) * mn + idx];
+            sr += v.x;
+            si += v.y;
+        }
     }
-    C[i + static_cast<int64_t>(j) * ldc] = mk(sr, si);
+    part[sl][o] = mk(sr, si);
+    __syncthreads();
+    if (sl == 0 && idx < mn) {
+        for (int k = 1; k < kRedSlices; ++k) {
+            sr += part[k][o].x;
+            si += part[k][o].y;
+        }
+        const int i = idx % m, j = idx / m;
+        C[i + static_cast<int64_t>(j) * ldc] = mk(sr, si);
+    }
 }
 
 template <typename Rin, typename R, typename Rout>
 void dense_product(const Context& c, const cx_t<R>* H, const cx_t<Rin>* X, cx_t<Rout>* Y, int nvec, hipStream_t s,
                    DevBuf<cx_t<R>>& xt, DevBuf<cx_t<R>>& yp) {
+    using T = ProductTiling<R>;
     const int64_t N = c.N;
-    const int row_blocks = static_cast<int>((N + kBlockRows - 1) / kBlockRows);
-    int splits = std::max(1, std::min(64, (1024 + row_blocks - 1) / row_blocks));
-    int64_t kchunk = ((N + splits - 1) / splits + 3) / 4 * 4;
-    splits = static_cast<int>((N + kchunk - 1) / kchunk);
+    const int rows_wg = 64 * T::TW;
+    const int row_blocks = static_cast<int>((N + rows_wg - 1) / rows_wg);
+    // enough workgroups to fill 256 CUs a few times over; splits a multiple of the XCD count
+    int splits = std::max(1, std::min(64, (2048 + row_blocks - 1) / row_blocks));
+    splits = (splits + kXcds - 1) / kXcds * kXcds;
+    const int step = 4 * T::U;
+    int64_t kchunk = ((N + splits - 1) / splits + step - 1) / step * step;
+    // every split owns a (possibly empty) column range and writes its partial rows
     xt.alloc(std::max<size_t>(xt.n, static_cast<size_t>(N) * kVec));
     yp.alloc(std::max<size_t>(yp.n, static_cast<size_t>(splits) * N * kVec));
     for (int v0 = 0; v0 < nvec; v0 += kVec) {
@@ -223,8 +269,8 @@ void dense_product(const Context& c, const cx_t<R>* H, const cx_t<Rin>* X, cx_t<
         const int64_t tot = N * kVec;
         hipLaunchKernelGGL((stage_vectors_kernel<Rin, R>), dim3((tot + 255) / 256), dim3(256), 0, s, X + v0 * N, N, nv,
                            xt.p);
-        hipLaunchKernelGGL((dense_product_kernel<R>), dim3(row_blocks, splits), dim3(256), 0, s, H, xt.p, N, kchunk,
-                           yp.p);
+        hipLaunchKernelGGL((dense_product_kernel<R>), dim3(row_blocks * splits), dim3(256), 0, s, H, xt.p, N, kchunk,
+                           splits, yp.p);
         hipLaunchKernelGGL((reduce_product_kernel<R, Rout>), dim3((N + 255) / 256, nv), dim3(256), 0, s, yp.p, N, nv,
                            splits, Y + v0 * N);
     }
@@ -242,17 +288,35 @@ void launch_matvec32(Context& c, const double2* X, double2* Y, int nvec, hipStre
     dense_product<double, float, double>(c, c.d_H32.p, X, Y, nvec, s, c.s_xt32, c.s_yp32);
 }
 
+void launch_stage_vectors(const double2* X, int64_t N, int nvec, double2* Xt, hipStream_t s) {
+    hipLaunchKernelGGL((stage_vectors_kernel<double, double>), dim3((N * kVec + 255) / 256), dim3(256), 0, s, X, N,
+                       nvec, Xt);
+}
+void launch_stage_vectors(const double2* X, int64_t N, int nvec, float2* Xt, hipStream_t s) {
+    hipLaunchKernelGGL((stage_vectors_kernel<double, float>), dim3((N * kVec + 255) / 256), dim3(256), 0, s, X, N,
+                       nvec, Xt);
+}
+void launch_reduce_partials(const double2* Yp, int64_t N, int nvec, int splits, double2* Y, hipStream_t s) {
+    hipLaunchKernelGGL((reduce_product_kernel<double, double>), dim3((N + 255) / 256, nvec), dim3(256), 0, s, Yp, N,
+                       nvec, splits, Y);
+}
+void launch_reduce_partials(const float2* Yp, int64_t N, int nvec, int splits, double2* Y, hipStream_t s) {
+    hipLaunchKernelGGL((reduce_product_kernel<float, double>), dim3((N + 255) / 256, nvec), dim3(256), 0, s, Yp, N,
+                       nvec, splits, Y);
+}
+
 void launch_gram(Context& c, const double2* A, int lda, const double2* B, int ldb, int m, int n, double2* C, int ldc,
                  hipStream_t s) {
     const int64_t N = c.N;
     const int tiles = ((m + kGM - 1) / kGM) * ((n + kGN - 1) / kGN);
-    int chunks = std::max(1, std::min<int>(static_cast<int>((N + kGR - 1) / kGR), 1024 / tiles));
+    int chunks = std::max(1, std::min<int>(static_cast<int>((N + kGR - 1) / kGR), 512 / tiles));
     int64_t rows = ((N + chunks - 1) / chunks + kGR - 1) / kGR * kGR;
     chunks = static_cast<int>((N + rows - 1) / rows);
     c.s_gram.alloc(std::max<size_t>(c.s_gram.n, static_cast<size_t>(chunks) * m * n));
     hipLaunchKernelGGL(gram_partial_kernel, dim3((m + kGM - 1) / kGM, (n + kGN - 1) / kGN, chunks), dim3(256), 0, s, A,
                        lda, B, ldb, N, m, n, rows, c.s_gram.p);
-    hipLaunchKernelGGL(gram_reduce_kernel, dim3((m * n + 255) / 256), dim3(256), 0, s, c.s_gram.p, m, n, chunks, C, ldc);
+    hipLaunchKernelGGL(gram_reduce_kernel, dim3((m * n + kRedOut - 1) / kRedOut), dim3(kRedOut * kRedSlices), 0, s,
+                       c.s_gram.p, m, n, chunks, C, ldc);
     XATU_HIP(hipGetLastError());
 }
 

@@ -6,16 +6,6 @@ namespace xatu_dev {
 
 namespace {
 
-__global__ void conv64to32_kernel(const double2* __restrict__ x, float2* __restrict__ y, size_t n) {
-    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n) y[i] = make_float2(static_cast<float>(x[i].x), static_cast<float>(x[i].y));
-}
-
-__global__ void conv32to64_kernel(const float2* __restrict__ x, double2* __restrict__ y, size_t n) {
-    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n) y[i] = make_double2(static_cast<double>(x[i].x), static_cast<double>(x[i].y));
-}
-
 // R[:, j] = AX[:, j] - theta_j X[:, j]; norms2[j] += |R[:, j]|^2 via block partials.
 // grid: (ceil(N/256), nvec)
 __global__ void residual_kernel(const double2* __restrict__ X, const double2* __restrict__ AX,
@@ -61,15 +51,27 @@ __global__ void unit_vectors_kernel(const int* __restrict__ idx, int64_t N, int 
     if (j < nvec) V[idx[j] + static_cast<int64_t>(j) * N] = make_double2(1.0, 0.0);
 }
 
-}  // namespace
-
-void launch_convert_64_to_32(const double2* x, float2* y, size_t n, hipStream_t s) {
-    hipLaunchKernelGGL(conv64to32_kernel, dim3((n + 255) / 256), dim3(256), 0, s, x, y, n);
-    XATU_HIP(hipGetLastError());
+// out[j + 3 n] = sum_i X[i + n N] vel[i + j N] (no conjugation, as the reference's
+// oscillator sum): one workgroup per (j, n), fixed-order tree reduction.
+__global__ __launch_bounds__(256) void oscillator_kernel(const double2* __restrict__ X, int64_t N,
+                                                         const double2* __restrict__ vel, double2* __restrict__ out) {
+    __shared__ double2 s[256];
+    const int j = blockIdx.x % 3, n = blockIdx.x / 3;
+    double2 acc = make_double2(0.0, 0.0);
+    for (int64_t i = threadIdx.x; i < N; i += 256) cfma(acc, X[i + n * N], vel[i + j * N]);
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st) s[threadIdx.x] = cadd(s[threadIdx.x], s[threadIdx.x + st]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = s[0];
 }
 
-void launch_convert_32_to_64(const float2* x, double2* y, size_t n, hipStream_t s) {
-    hipLaunchKernelGGL(conv32to64_kernel, dim3((n + 255) / 256), dim3(256), 0, s, x, y, n);
+}  // namespace
+
+void launch_oscillator(const double2* X, int64_t N, int nstates, const double2* vel, double2* out, hipStream_t s) {
+    hipLaunchKernelGGL(oscillator_kernel, dim3(3 * nstates), dim3(256), 0, s, X, N, vel, out);
     XATU_HIP(hipGetLastError());
 }
 

@@ -480,6 +480,106 @@ std::vector<double> ExcitonTB::hfGradients(const std::vector<cdouble>& eigvec, i
     return g;
 }
 
+// Gradients of exciton energies with respect to the tight-binding matrices H(R)
+// (no reference counterpart). The device returns, per state, the sensitivities to the
+// band data (w = dE/d eps_b(k), g = dE/d conj C_b(k)); first-order perturbation theory
+// of the Bloch Hamiltonian, d eps_n = C_n^H dh C_n and dC_n = sum_{m != n} C_m
+// (C_m^H dh C_n) / (eps_n - eps_m), turns them into dE = Re Tr(A(k) dh(k)) with
+//   A(k) = sum_n [ w_n C_n C_n^H + 2 sum_{m != n} (g_n^H C_m) / (eps_n - eps_m) C_n C_m^H ],
+// n over the bands the BSE uses at that k (valence of the k stack, conduction of the
+// k+Q stack), m over all bands of h(k). The phase of C_n cancels from E, so the gauge
+// term of dC_n is left out. With h(k) = sum_R H(R) e^{i k.R}, G(R)_ab = sum_k A(k)_ba
+// e^{i k.R} gives dE = Re sum_R sum_ab G(R)_ab dH(R)_ab for any change of the H(R)
+// that keeps h(k) Hermitian. Output: nstates x ncells x basisdim x basisdim complex,
+// each block column-major like hamiltonianMatrices.
+std::vector<cdouble> ExcitonTB::hoppingGradients(const std::vector<cdouble>& eigvec, int nstates) const {
+    if (!system->overlapMatrices.empty() || system->isTriangular) {
+        throw std::invalid_argument(
+            "hoppingGradients(): needs an orthonormal basis with complete (not triangular) H(R) matrices");
+    }
+    const int nk = system->nk;
+    const int nb = static_cast<int>(bandList.size());
+    const int nv = static_cast<int>(valenceBands.size());
+    const int bd = system->basisdim;
+    const int nR = system->ncells;
+    std::vector<double> w(static_cast<size_t>(nstates) * 2 * nb * nk);
+    std::vector<cdouble> g(static_cast<size_t>(nstates) * 2 * bd * nb * nk);
+    const double* v = eigvec.empty() ? nullptr : reinterpret_cast<const double*>(eigvec.data());
+    check(xatu_band_gradients(ctx_, nstates, v, w.data(), reinterpret_cast<double*>(g.data())));
+
+    const size_t bd2 = static_cast<size_t>(bd) * bd;
+    // A[st][s][k] (bd x bd, column-major)
+    std::vector<cdouble> A(static_cast<size_t>(2) * nstates * nk * bd2, cdouble(0.0, 0.0));
+    std::string err;
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int k = 0; k < nk; ++k) {
+        std::vector<double> ev;
+        std::vector<cdouble> W;
+        for (int st = 0; st < 2; ++st) {
+            double kv[3];
+            for (int x = 0; x < 3; ++x) kv[x] = system->kpoints[3 * static_cast<size_t>(k) + x] + (st ? Q[x] : 0.0);
+            system->solveBands(kv, ev, W);
+            const std::vector<cdouble>& stackV = st ? eigvecKQStack : eigvecKStack;
+            const std::vector<double>& stackE = st ? eigvalKQStack : eigvalKStack;
+            const int b0 = st ? nv : 0, b1 = st ? nb : nv;
+            for (int s = 0; s < nstates; ++s) {
+                cdouble* Ak = A.data() + ((static_cast<size_t>(st) * nstates + s) * nk + k) * bd2;
+                for (int b = b0; b < b1; ++b) {
+                    const cdouble* Cn = stackV.data() + (static_cast<size_t>(k) * nb + b) * bd;
+                    const double en = stackE[static_cast<size_t>(k) * nb + b];
+                    const long long nabs = bandList[b];
+                    const double wn = w[((static_cast<size_t>(s) * 2 + st) * nk + k) * nb + b];
+                    const cdouble* gn = g.data() + (((static_cast<size_t>(s) * 2 + st) * nk + k) * nb + b) * bd;
+                    // B = w_n C_n + 2 sum_{m != n} coef_m C_m, so that A += C_n B^H ... written as
+                    // A_ab += C_n[a] conj(Bv[b]) with Bv = w_n C_n + 2 sum_m conj(coef_m) C_m
+                    std::vector<cdouble> Bv(bd);
+                    for (int a = 0; a < bd; ++a) Bv[a] = wn * Cn[a];
+                    for (int m = 0; m < bd; ++m) {
+                        if (m == nabs) continue;
+                        const double de = en - ev[m];
+                        if (std::abs(de) < 1e-9) {
+#pragma omp critical
+                            err = "hoppingGradients(): degenerate bands at a k-point; the gradient is not defined";
+                            continue;
+                        }
+                        const cdouble* Cm = W.data() + static_cast<size_t>(m) * bd;
+                        cdouble gc(0.0, 0.0);
+                        for (int a = 0; a < bd; ++a) gc += std::conj(gn[a]) * Cm[a];
+                        const cdouble coef = 2.0 * gc / de;
+                        for (int a = 0; a < bd; ++a) Bv[a] += std::conj(coef) * Cm[a];
+                    }
+                    for (int bcol = 0; bcol < bd; ++bcol) {
+                        const cdouble cb = std::conj(Bv[bcol]);
+                        for (int a = 0; a < bd; ++a) Ak[a + static_cast<size_t>(bcol) * bd] += Cn[a] * cb;
+                    }
+                }
+            }
+        }
+    }
+    if (!err.empty()) throw std::invalid_argument(err);
+
+    // G_s(R)_ab = sum_st sum_k A_st,s(k)_ba exp(i (k + st Q).R)
+    std::vector<cdouble> G(static_cast<size_t>(nstates) * nR * bd2, cdouble(0.0, 0.0));
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int s = 0; s < nstates; ++s) {
+        for (int r = 0; r < nR; ++r) {
+            cdouble* Gr = G.data() + (static_cast<size_t>(s) * nR + r) * bd2;
+            const double* R = system->unitCellList.data() + 3 * static_cast<size_t>(r);
+            for (int st = 0; st < 2; ++st) {
+                for (int k = 0; k < nk; ++k) {
+                    double ph = 0.0;
+                    for (int x = 0; x < 3; ++x) ph += (system->kpoints[3 * static_cast<size_t>(k) + x] + (st ? Q[x] : 0.0)) * R[x];
+                    const cdouble e(std::cos(ph), std::sin(ph));
+                    const cdouble* Ak = A.data() + ((static_cast<size_t>(st) * nstates + s) * nk + k) * bd2;
+                    for (int b = 0; b < bd; ++b)
+                        for (int a = 0; a < bd; ++a) Gr[a + static_cast<size_t>(b) * bd] += Ak[b + static_cast<size_t>(a) * bd] * e;
+                }
+            }
+        }
+    }
+    return G;
+}
+
 xatu_timings ExcitonTB::timings() const {
     xatu_timings t{};
     if (ctx_) xatu_get_timings(ctx_, &t);

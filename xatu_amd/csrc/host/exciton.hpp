@@ -102,6 +102,9 @@ public:
     // Hellmann-Feynman gradients of the given states with respect to "r0", "eps_m", "eps_s".
     std::vector<double> hfGradients(const std::vector<cdouble>& eigvec, int nstates,
                                     const std::vector<std::string>& params) const;
+    // dE_n/dH(R) for every cell and orbital pair (nstates x ncells x basisdim x basisdim,
+    // blocks column-major); eigvec empty: the last solve's vectors.
+    std::vector<cdouble> hoppingGradients(const std::vector<cdouble>& eigvec, int nstates) const;
     xatu_timings timings() const;
     xatu_solve_report lastSolveReport() const { return lastReport_; }
     xatu_ctx* context() const { return ctx_; }
@@ -114,6 +117,7 @@ public:
     static void fixGlobalPhase(std::vector<cdouble>& coefs, int rows, int cols);
 
 private:
+    friend class ResultTB;
     void initializeExcitonAttributes(int ncell, const std::vector<long long>& bands,
                                      const std::vector<double>& parameters, const std::vector<double>& Q);
     void initializeExcitonAttributes(const ExcitonConfiguration& cfg);
@@ -142,10 +146,17 @@ public:
     // Observables (Result.tpp:16-60)
     double kineticEnergy(int stateindex) const;
     double potentialEnergy(int stateindex) const;
+    // Optics (src/ResultTB.cpp:242-298, 412-460; src/skubo_w.f90): oscillator strengths
+    // 3 x ncols (a.u.), and the Lorentzian-broadened exciton conductivity, nw x {w, xx, xy, yx, yy}.
+    std::vector<cdouble> excitonOscillatorStrength() const;
+    std::vector<double> absorptionSpectrum(double w0, double wrange, int nw, double eta) const;
     // Output formats (Result.tpp:404-448)
     void writeEigenvalues(FILE* textfile, int n = 0) const;
     void writeStates(FILE* textfile, int n = 0) const;
 };
+
+// Single-particle velocity matrix elements of the k stack, V[((k*3 + j)*nb + n)*nb + m] (a.u.).
+std::vector<cdouble> velocityMatrixElements(const ExcitonTB& ex);
 
 // Degenerate-level grouping of sorted energies (src/utils.cpp:121-152).
 std::vector<std::vector<double>> detectDegeneracies(const std::vector<double>& eigval, int n, int precision);

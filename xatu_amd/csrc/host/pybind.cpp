@@ -170,6 +170,16 @@ PYBIND11_MODULE(_xatu, m) {
     py::class_<ResultTB>(m, "ResultTB")
         .def_property_readonly("eigval", [](const ResultTB& r) { return py::array_t<double>(r.eigval.size(), r.eigval.data()); })
         .def_property_readonly("eigvec", [](const ResultTB& r) { return fortran(r.eigvec, {r.nrows, r.ncols}); })
+        .def("excitonOscillatorStrength",
+             [](const ResultTB& r) { return fortran(r.excitonOscillatorStrength(), {3, r.ncols}); })
+        .def("absorptionSpectrum",
+             [](const ResultTB& r, double w0, double wrange, int nw, double eta) {
+                 std::vector<double> a = r.absorptionSpectrum(w0, wrange, nw, eta);
+                 py::array_t<double> out({static_cast<py::ssize_t>(nw), static_cast<py::ssize_t>(5)});
+                 std::copy(a.begin(), a.end(), out.mutable_data());
+                 return out;
+             },
+             py::arg("w0") = 0.0, py::arg("wrange") = 6.0, py::arg("nw") = 2000, py::arg("eta") = 0.05)
         .def("kineticEnergy", &ResultTB::kineticEnergy)
         .def("potentialEnergy", &ResultTB::potentialEnergy)
         .def("writeEigenvalues",
@@ -313,6 +323,22 @@ PYBIND11_MODULE(_xatu, m) {
                  return fortran(g, {nstates, static_cast<py::ssize_t>(params.size())});
              },
              py::arg("eigvec"), py::arg("nstates"), py::arg("params"))
+        .def("hoppingGradients",
+             [](const ExcitonTB& e, py::object eigvec, int nstates) {
+                 std::vector<cdouble> v;
+                 if (!eigvec.is_none()) v = to_vec(eigvec.cast<py::array_t<cdouble, py::array::f_style | py::array::forcecast>>());
+                 std::vector<cdouble> G = e.hoppingGradients(v, nstates);
+                 const py::ssize_t bd = e.system->basisdim, nR = e.system->ncells;
+                 // [state][cell][b][a] in memory -> array indexed [state, cell, a, b]
+                 py::array_t<cdouble> out({static_cast<py::ssize_t>(nstates), nR, bd, bd},
+                                          {static_cast<py::ssize_t>(nR * bd * bd * sizeof(cdouble)),
+                                           static_cast<py::ssize_t>(bd * bd * sizeof(cdouble)),
+                                           static_cast<py::ssize_t>(sizeof(cdouble)),
+                                           static_cast<py::ssize_t>(bd * sizeof(cdouble))});
+                 std::copy(G.begin(), G.end(), out.mutable_data());
+                 return out;
+             },
+             py::arg("eigvec"), py::arg("nstates"))
         .def("timings", &ExcitonTB::timings)
         .def("lastSolveReport", &ExcitonTB::lastSolveReport)
         .def("contextAddress", [](const ExcitonTB& e) { return reinterpret_cast<std::uintptr_t>(e.context()); });
