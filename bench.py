@@ -102,6 +102,43 @@ def cpu_baseline(model_path, cfg, n_products_fp64, nstates, block):
     }
 
 
+def extras(ex, config, nstates, mf):
+    """Workload-specific figures measured through the ExcitonTB facade after the timed steps:
+    c3 - Hellmann-Feynman gradients of the lowest 8 energies w.r.t. r0, eps_s and every
+         hopping amplitude (FP64 adjoint kernels);
+    c5 - matrix-free mixed solve, absorption spectrum along x (0.05 eV Lorentzians, 2000
+         points over 0-6 eV) and dE/d(r0, eps_s) of the lowest exciton."""
+    ex.setPrecision("fp32" if config == "c5" else "fp64")
+    ex.setMatrixFree(bool(mf))
+    ex.setDavidsonTolerance(TOL)
+    ex.BShamiltonian()
+    t0 = time.perf_counter()
+    res = ex.diagonalize("davidson", nstates)
+    t_solve = time.perf_counter() - t0
+    out = {"solve_s": round(t_solve, 4), "energies_eV": [round(float(e), 6) for e in np.asarray(res.eigval)]}
+    if config == "c3":
+        t0 = time.perf_counter()
+        g = np.asarray(ex.hfGradients(None, nstates, ["r0", "eps_s"]))
+        t_hf = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        G = np.asarray(ex.hoppingGradients(None, nstates))
+        t_hop = time.perf_counter() - t0
+        out.update({"hf_gradients_s": round(t_hf, 4), "hopping_gradients_s": round(t_hop, 4),
+                    "n_hopping_entries": int(G[0].size), "dE_dr0": [float(f"{x:.6g}") for x in g[:, 0]],
+                    "dE_deps_s": [float(f"{x:.6g}") for x in g[:, 1]]})
+    else:
+        t0 = time.perf_counter()
+        spec = np.asarray(res.absorptionSpectrum(0.0, 6.0, 2000, 0.05))
+        t_abs = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        g = np.asarray(ex.hfGradients(None, 1, ["r0", "eps_s"]))
+        t_hf = time.perf_counter() - t0
+        i = int(np.argmax(spec[:, 1]))
+        out.update({"absorption_s": round(t_abs, 4), "absorption_peak_eV": float(spec[i, 0]),
+                    "hf_gradients_s": round(t_hf, 4), "dE0_dr0": float(g[0, 0]), "dE0_deps_s": float(g[0, 1])})
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -153,8 +190,9 @@ def main():
         return w, rep, t_build
 
     results = {}
+    skip_plain = args.no_plain or mf  # the FP64 form of a matrix-free workload is not a product path worth timing
     for name, prec in (("fp64", capi.XATU_FP64), ("mixed", capi.XATU_FP32)):
-        if name == "fp64" and args.no_plain:
+        if name == "fp64" and skip_plain:
             continue
         for i in range(args.warmup):
             log(f"{name} warmup {i}")
@@ -214,11 +252,16 @@ def main():
                            "avg_ms": round(tim.ms_matvec_kernel, 3), "launches": tim.n_matvec_kernel,
                            "avg_vectors": round(nvec_avg, 2), "algorithmic_bytes": 8.0 * N * N,
                            "hbm_GBs": round(8.0 * N * N / t / 1e9, 1)}
+    if args.config in ("c3", "c5"):
+        ctx.close()
+        out["extras"] = extras(ex, args.config, nstates, mf)
+        ctx = None
     if not args.no_cpu:
         n_products = results["fp64"]["rep"].products if "fp64" in results else rep.products
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(model, cfg, n_products, nstates, block)
-    ctx.close()
+    if ctx is not None:
+        ctx.close()
     print(json.dumps(out), flush=True)
 
 

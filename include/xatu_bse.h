@@ -41,6 +41,9 @@ typedef struct xatu_ctx xatu_ctx;
 #define XATU_FP32 1
 /* xatu_build_bse mode that prepares matrix-free products without storing the matrix */
 #define XATU_MATRIX_FREE 2
+/* Matrix-free product forms (xatu_solve_options.matrix_free, xatu_bse_matvec) */
+#define XATU_MATRIX_FREE_CONV 1
+#define XATU_MATRIX_FREE_TILES 2
 
 /* Interaction potentials (src/ExcitonTB.cpp:347-399) */
 #define XATU_POTENTIAL_KELDYSH 0
@@ -75,7 +78,10 @@ typedef struct xatu_solve_options {
     int method;      /* XATU_SOLVE_DIAG or XATU_SOLVE_DAVIDSON */
     int precision;   /* XATU_FP64: products with the FP64 matrix.
                         XATU_FP32: products with the FP32 matrix, then FP64 refinement */
-    int matrix_free; /* non-zero: no resident matrix; tiles recomputed inside each product */
+    int matrix_free; /* 0: resident matrix. XATU_MATRIX_FREE_CONV (1): matrix-free, the direct term as a
+                        k-space convolution (FFT) and the exchange term along the upper triangle;
+                        XATU_MATRIX_FREE_TILES (2): matrix-free, tiles of H recomputed inside each
+                        product on the matrix cores */
     int nstates;     /* number of lowest eigenpairs wanted */
     int block;       /* Davidson block size; 0 selects max(2*nstates, 8) */
     int max_basis;   /* Davidson basis size before restart; 0 selects a default */
@@ -158,7 +164,7 @@ int xatu_solve(xatu_ctx* ctx, const xatu_solve_options* opt, double* eigval_out,
                xatu_solve_report* report);
 
 /* y = H x for nvec vectors (complex128, N x nvec column-major) with the resident matrix of
- * the given precision, or matrix-free when matrix_free != 0. */
+ * the given precision, or matrix-free (matrix_free: XATU_MATRIX_FREE_CONV or _TILES). */
 int xatu_bse_matvec(xatu_ctx* ctx, int precision, int matrix_free, int nvec, const double* x, double* y);
 
 /* Hellmann-Feynman gradients dE_n/dtheta = psi_n^H (dH/dtheta) psi_n for the screening

@@ -39,7 +39,11 @@ def test_oscillator_strengths_match_oracle_per_state():
     res = ex.diagonalize("diag", 2)
     X = np.asarray(res.eigvec)
     vme = np.asarray(res.excitonOscillatorStrength())
-    ref = O.exciton_oscillator_strength(oc, X)  # same eigenvectors: the contraction is checked per state
+    # per-state values depend on the gauge inside degenerate band subspaces: give the oracle
+    # the facade's band stacks so that both use the same band vectors
+    oc.eigvec_k = np.asfortranarray(np.asarray(ex.eigvecKStack).reshape(oc.eigvec_k.shape, order="F"))
+    oc.eigval_k = np.asfortranarray(np.asarray(ex.eigvalKStack).reshape(oc.eigval_k.shape, order="F"))
+    ref = O.exciton_oscillator_strength(oc, X)
     assert np.abs(vme - ref).max() <= 1e-12 * np.abs(ref).max()
     # spectrum from the lowest states (cut between two distinct levels)
     e = np.asarray(res.eigval)

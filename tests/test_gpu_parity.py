@@ -143,28 +143,35 @@ def test_matvec(name, request):
     ctx.close()
 
 
-@pytest.mark.parametrize("name", ["hbn20", "mos2_q_x", "mos2_reduced", "random18"])
-def test_matrix_free_products(name, request):
+MF_FORMS = [1, 2]  # XATU_MATRIX_FREE_CONV (k-space convolution), XATU_MATRIX_FREE_TILES (recomputed tiles)
+
+
+@pytest.mark.parametrize("form", MF_FORMS)
+@pytest.mark.parametrize("name", ["hbn20", "mos2_q_x", "mos2_reduced", "mos2_odd", "random18"])
+def test_matrix_free_products(name, form, request):
+    print(f"[matrix-free products] {name} form {form}", flush=True)
     ex, H = get_case(name, request)
     ctx = gpu_context(ex)
     ctx.build(2, scissor=ex.scissor)  # prepare only: nothing resident
     rng = np.random.default_rng(11)
-    x = rng.normal(size=(ex.N, 11)) + 1j * rng.normal(size=(ex.N, 11))
+    x = rng.normal(size=(ex.N, 37)) + 1j * rng.normal(size=(ex.N, 37))  # two vector blocks
     y = H @ x
-    assert rel_err(ctx.matvec(x, precision=0, matrix_free=True), y) < 1e-12
-    assert rel_err(ctx.matvec(x, precision=1, matrix_free=True), y) < 1e-5
+    assert rel_err(ctx.matvec(x, precision=0, matrix_free=form), y) < 1e-12
+    assert rel_err(ctx.matvec(x, precision=1, matrix_free=form), y) < 1e-5
     ctx.close()
 
 
+@pytest.mark.parametrize("form", MF_FORMS)
 @pytest.mark.parametrize("name,nstates", [("hbn30_cut20_x", 8), ("random18", 16)])
-def test_matrix_free_davidson(name, nstates, request):
+def test_matrix_free_davidson(name, nstates, form, request):
+    print(f"[matrix-free davidson] {name} form {form}", flush=True)
     ex, H = get_case(name, request)
     wr = np.linalg.eigvalsh(H)[:nstates]
     ctx = gpu_context(ex)
     ctx.build(2, scissor=ex.scissor)
-    w, v, rep = ctx.solve(method=1, nstates=nstates, precision=0, matrix_free=True, tol=1e-7)
+    w, v, rep = ctx.solve(method=1, nstates=nstates, precision=0, matrix_free=form, tol=1e-7)
     assert rep.converged == 1 and np.abs(w - wr).max() < 1e-6
-    w, v, rep = ctx.solve(method=1, nstates=nstates, precision=1, matrix_free=True, tol=1e-6)
+    w, v, rep = ctx.solve(method=1, nstates=nstates, precision=1, matrix_free=form, tol=1e-6)
     assert rep.converged == 1 and np.abs(w - wr).max() < 1e-6
     ctx.close()
 

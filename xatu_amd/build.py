@@ -37,6 +37,7 @@ DEVICE_SOURCES = [
     "device/vec_kernels.hip",
     "device/hf.hip",
     "device/band_adjoint.hip",
+    "device/convolution.hip",
     "device/capi.hip",
     "device/davidson.cpp",
 ]
@@ -76,7 +77,7 @@ def build_device(jobs: int = 8) -> Path:
     out = LIBDIR / "libxatu_bse.so"
     tmp = LIBDIR / "libxatu_bse.so.tmp"
     _run([str(ROCM / "bin" / "hipcc"), "--offload-arch=gfx950", "-shared", "-fPIC", *map(str, objs),
-          "-o", str(tmp), f"-L{ROCM / 'lib'}", "-lrocsolver", "-lrocblas",
+          "-o", str(tmp), f"-L{ROCM / 'lib'}", "-lrocsolver", "-lrocblas", "-lrocfft",
           f"-Wl,-rpath,{ROCM / 'lib'}"])
     os.replace(tmp, out)
     return out

@@ -21,6 +21,8 @@ XATU_ERR_NOT_CONVERGED = 5
 XATU_ERR_NOT_HERMITIAN = 6
 XATU_FP64, XATU_FP32 = 0, 1
 XATU_MATRIX_FREE = 2
+XATU_MATRIX_FREE_CONV = 1    # matrix-free product forms (solve / matvec)
+XATU_MATRIX_FREE_TILES = 2
 XATU_POTENTIAL_KELDYSH, XATU_POTENTIAL_COULOMB = 0, 1
 XATU_SOLVE_DIAG, XATU_SOLVE_DAVIDSON = 0, 1
 XATU_PARAM_R0, XATU_PARAM_EPS_M, XATU_PARAM_EPS_S = 0, 1, 2
@@ -195,7 +197,7 @@ class Context:
 
     def solve(self, method=XATU_SOLVE_DAVIDSON, nstates=8, precision=XATU_FP64, matrix_free=False, block=0,
               max_basis=0, max_iter=0, tol=0.0, want_vectors=True):
-        opt = SolveOptions(method, precision, int(bool(matrix_free)), nstates, block, max_basis, max_iter, tol)
+        opt = SolveOptions(method, precision, int(matrix_free), nstates, block, max_basis, max_iter, tol)
         n_out = self.N if method == XATU_SOLVE_DIAG and nstates <= 0 else nstates
         w = np.zeros(max(n_out, 1))
         v = np.zeros((self.N, n_out), dtype=np.complex128, order="F") if want_vectors else None
@@ -208,7 +210,7 @@ class Context:
         if x.ndim == 1:
             x = x[:, None]
         y = np.zeros_like(x, order="F")
-        self._check(self.L.xatu_bse_matvec(self._h, precision, int(bool(matrix_free)), x.shape[1], _ptr(x),
+        self._check(self.L.xatu_bse_matvec(self._h, precision, int(matrix_free), x.shape[1], _ptr(x),
                                            _ptr(y)))
         return y
 

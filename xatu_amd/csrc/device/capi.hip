@@ -96,6 +96,7 @@ void prepare_operator(Context& c) {
     XATU_HIP(hipMemcpyAsync(c.h_diag.data(), c.d_diag.p, c.N * sizeof(double), hipMemcpyDeviceToHost, c.stream));
     XATU_HIP(hipStreamSynchronize(c.stream));
     c.prepared = true;
+    ++c.prep_gen;
 }
 
 }  // namespace
@@ -108,15 +109,15 @@ void apply_operator(Context& c, int precision, int matrix_free, const double2* X
     EventTimer ev(c.stream);
     if (matrix_free) {
         if (!c.prepared) throw Error(XATU_ERR_LOGIC, "operator not prepared (call xatu_build_bse)");
-        if (precision == XATU_FP64) {
-            ev.start();
-            launch_matfree64(c, X, Y, nvec, c.stream);
-            ev.stop();
+        ev.start();
+        if (matrix_free == XATU_MATRIX_FREE_TILES) {
+            if (precision == XATU_FP64) launch_matfree64(c, X, Y, nvec, c.stream);
+            else launch_matfree32(c, X, Y, nvec, c.stream);
         } else {
-            ev.start();
-            launch_matfree32(c, X, Y, nvec, c.stream);
-            ev.stop();
+            if (precision == XATU_FP64) launch_conv64(c, X, Y, nvec, c.stream);
+            else launch_conv32(c, X, Y, nvec, c.stream);
         }
+        ev.stop();
     } else if (precision == XATU_FP64) {
         if (!c.built64) throw Error(XATU_ERR_LOGIC, "FP64 BSE matrix is not resident");
         ev.start();

@@ -6,6 +6,7 @@
 
 #include <cstdint>
 #include <stdexcept>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -85,6 +86,10 @@ struct PotDesc {
     double cutoff;    // |a1|*cutoff_ + 1e-5
 };
 
+// State of the convolution (FFT) form of the matrix-free product (convolution.hip).
+struct ConvState;
+void destroy_conv_state(ConvState* s);
+
 struct Context {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -125,6 +130,8 @@ struct Context {
     double scissor = 0.0;
     bool built64 = false, built32 = false;
     bool prepared = false;  // exchange factors and diagonal ready for products
+    int prep_gen = 0;       // incremented by every preparation of the operator
+    std::unique_ptr<ConvState, void (*)(ConvState*)> conv{nullptr, destroy_conv_state};
     DevBuf<double2> d_H64;
     DevBuf<float2> d_H32;
     std::vector<double> h_diag;  // diagonal of H (FP64), N
@@ -196,6 +203,10 @@ void launch_matvec32(Context& c, const double2* X, double2* Y, int nvec, hipStre
 // C (m x n, leading dimension ldc) = A^H B over the N rows of A (lda) and B (ldb), FP64.
 void launch_gram(Context& c, const double2* A, int lda, const double2* B, int ldb, int m, int n, double2* C, int ldc,
                  hipStream_t s);
+// Matrix-free products Y = H X by k-space convolution of the direct term (FFT) plus
+// the exchange term along the upper triangle, in FP64 / FP32 arithmetic.
+void launch_conv64(Context& c, const double2* X, double2* Y, int nvec, hipStream_t s);
+void launch_conv32(Context& c, const double2* X, double2* Y, int nvec, hipStream_t s);
 // Matrix-free products Y = H X (complex FP64 vectors) with tiles of H recomputed in
 // FP64 / FP32 arithmetic inside the product (FP32: FP32 tiles, FP32 accumulation).
 void launch_matfree64(Context& c, const double2* X, double2* Y, int nvec, hipStream_t s);

@@ -1,0 +1,463 @@
+// Matrix-free BSE products by k-space convolution.
+//
+// The direct term of the reference's matrix (src/ExcitonTB.cpp:520-548, 840-921),
+//   D(kcv, k'c'v') = sum_ab rho^c_a M_ab(k'-k) rho^v_b,
+//   rho^c_a = sum_{al in a} conj(C_kc,al) C_k'c',al,  rho^v_b = sum_{be in b} conj(V_k'v',be) V_kv,be,
+// depends on the pair (k, k') only through the lattice-sum table M(k'-k), so its
+// product with a vector factorises:
+//   z_{al be}(k')  = sum_{c'v'} C_k'c',al conj(V_k'v',be) x(k'c'v')
+//   w_{al be}(k)   = sum_k' M_{a(al) b(be)}(k'-k) z_{al be}(k')        (a correlation on the mesh)
+//   (D x)(kcv)     = sum_{al be} conj(C_kc,al) V_kv,be w_{al be}(k).
+// The correlation is evaluated with FFTs on the mesh of the lattice sums (mesh_n points
+// per axis; k-points sit at their integer offsets, so reduced and shifted meshes work
+// unchanged): w = IFFT( P_ab . FFT(z) ) with P_ab the backward transform of the table
+// M_ab(Delta) = ft[q(Delta)] (q the same mesh index the assembly uses), scaled by 1/mesh.
+// Cost per vector: norb^2 FFT pairs of mesh size plus O(nk norb^2 nc) contractions,
+// instead of O(nk^2) pair evaluations for a recomputed tile.
+//
+// Off the diagonal the reference fills only the upper triangle and mirrors it. The
+// direct formula is Hermitian under the exchange of the pair because the table obeys
+// M(-q) = conj M(q) (including the conjugated diagonal the reference stores), so the
+// full correlation equals the mirrored matrix up to rounding. The exchange term
+// X(i,j) = sum_a E1_a(i) E2_a(j) is not Hermitian when V(Q) has a complex diagonal, so
+// it follows the triangle: (X_tri x)_i = sum_a [E1_a(i) sum_{j>i} E2_a(j) x_j
+// + conj(E2_a(i)) sum_{j<i} conj(E1_a(j)) x_j] + Re(sum_a E1_a(i) E2_a(i)) x_i, with the
+// partial sums from a blocked scan in a fixed order. The diagonal energies enter as
+// e_i x_i, so H x = e x - D x + X_tri x exactly as the assembled matrix.
+//
+// FFTs: rocFFT (batched, in place, interleaved complex); plans are cached per
+// precision and batch size. The exchange part runs in FP64 for both precisions.
+#include <map>
+#include <mutex>
+#include <utility>
+#include <vector>
+
+#include <rocfft/rocfft.h>
+
+#include "context.hpp"
+#include "cplx.hpp"
+#include "mfma.hpp"
+
+#define XATU_FFT(call)                                                                                    \
+    do {                                                                                                  \
+        rocfft_status s_ = (call);                                                                        \
+        if (s_ != rocfft_status_success) {                                                                \
+            throw ::xatu_dev::Error(XATU_ERR_DEVICE,                                                      \
+                                    std::string(#call) + ": rocfft status " + std::to_string(static_cast<int>(s_))); \
+        }                                                                                                 \
+    } while (0)
+
+namespace xatu_dev {
+
+namespace {
+
+constexpr int kScanRows = 256;
+
+__device__ inline int atom_of(const LatticeDesc& lat, int orb) {
+    int a = 0;
+    while (a + 1 < lat.natoms && orb >= lat.orb_begin[a + 1]) ++a;
+    return a;
+}
+
+// Mt[ab][g] = ft[q(g) NA^2 + ab], g = sum_j d_j m^j, q = sum_j ((d_j + m/2) mod m) m^j.
+__global__ void table_grid_kernel(LatticeDesc lat, const double2* __restrict__ ft, double2* __restrict__ Mt) {
+    const int64_t M = lat.nmesh;
+    const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int na2 = lat.natoms * lat.natoms;
+    if (idx >= M * na2) return;
+    const int ab = static_cast<int>(idx / M);
+    int64_t g = idx % M;
+    const int m = lat.mesh_n;
+    int64_t q = 0, mul = 1;
+    for (int j = 0; j < lat.ndim; ++j) {
+        const int d = static_cast<int>(g % m);
+        g /= m;
+        q += static_cast<int64_t>((d + m / 2) % m) * mul;
+        mul *= m;
+    }
+    Mt[idx] = ft[q * na2 + ab];
+}
+
+template <typename R>
+__global__ void scale_spectrum_kernel(const double2* __restrict__ in, int64_t n, double scale, cx_t<R>* __restrict__ out) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = mk(static_cast<R>(in[i].x * scale), static_cast<R>(in[i].y * scale));
+}
+
+// u[((n NC + c) norb + be) nk + k] = sum_v conj(V_k[be, v]) x[k NCV + c NV + v, n]
+template <typename R>
+__global__ void conv_u_kernel(int nk, int nv, int nc, int norb, const double2* __restrict__ evk,
+                              const double2* __restrict__ X, int64_t N, cx_t<R>* __restrict__ U) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = blockIdx.y;
+    if (k >= nk) return;
+    const int nb = nv + nc, ncv = nc * nv;
+    const double2* V = evk + static_cast<size_t>(k) * nb * norb;  // valence columns 0..nv-1
+    for (int c = 0; c < nc; ++c) {
+        for (int be = 0; be < norb; ++be) {
+            double2 s = mk(0.0, 0.0);
+            for (int v = 0; v < nv; ++v)
+                cfma_conj(s, V[static_cast<size_t>(v) * norb + be], X[static_cast<size_t>(k) * ncv + c * nv + v + n * N]);
+            U[((static_cast<size_t>(n) * nc + c) * norb + be) * nk + k] = mk(static_cast<R>(s.x), static_cast<R>(s.y));
+        }
+    }
+}
+
+// G[((al norb + be) nvec + n) M + g(k)] = sum_c C_k[al, c] u[n][c][be][k]
+template <typename R>
+__global__ void conv_z_kernel(int nk, int nv, int nc, int norb, int nvec, int64_t M, const int* __restrict__ gk,
+                              const double2* __restrict__ evkq, const cx_t<R>* __restrict__ U, cx_t<R>* __restrict__ G) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int al = blockIdx.y;
+    const int n = blockIdx.z;
+    if (k >= nk) return;
+    const int nb = nv + nc;
+    constexpr int kMaxC = 8;
+    cx_t<R> Ca[kMaxC];
+    for (int c = 0; c < nc && c < kMaxC; ++c) {
+        const double2 v = evkq[(static_cast<size_t>(k) * nb + nv + c) * norb + al];
+        Ca[c] = mk(static_cast<R>(v.x), static_cast<R>(v.y));
+    }
+    const int64_t g = gk[k];
+    for (int be = 0; be < norb; ++be) {
+        cx_t<R> s = mk(R(0), R(0));
+        for (int c = 0; c < nc && c < kMaxC; ++c) cfma(s, Ca[c], U[((static_cast<size_t>(n) * nc + c) * norb + be) * nk + k]);
+        G[((static_cast<int64_t>(al) * norb + be) * nvec + n) * M + g] = s;
+    }
+}
+
+// G[b][xi] *= P[a(al) + NA b(be)][xi], batch b = (al norb + be) nvec + n
+template <typename R>
+__global__ void conv_multiply_kernel(LatticeDesc lat, int nvec, int64_t M, const cx_t<R>* __restrict__ P,
+                                     cx_t<R>* __restrict__ G) {
+    const int64_t xi = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int b = blockIdx.y;
+    if (xi >= M) return;
+    const int ab = b / nvec;
+    const int al = ab / lat.norb, be = ab % lat.norb;
+    const int pa = atom_of(lat, al) + lat.natoms * atom_of(lat, be);
+    const int64_t i = static_cast<int64_t>(b) * M + xi;
+    G[i] = cmul(G[i], P[static_cast<int64_t>(pa) * M + xi]);
+}
+
+// Y[k NCV + c NV + v, n] = e(kcv) x - sum_be V_k[be, v] sum_al conj(C_k[al, c]) w[(al norb + be) nvec + n][g(k)]
+template <typename R>
+__global__ void conv_y_kernel(int nk, int nv, int nc, int norb, int nvec, int64_t M, const int* __restrict__ gk,
+                              const double2* __restrict__ evk, const double2* __restrict__ evkq,
+                              const double* __restrict__ eval_k, const double* __restrict__ eval_kq, double scissor,
+                              const cx_t<R>* __restrict__ W, const double2* __restrict__ X, int64_t N,
+                              double2* __restrict__ Y) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = blockIdx.y;
+    if (k >= nk) return;
+    const int nb = nv + nc, ncv = nc * nv;
+    constexpr int kMax = 4;  // nc, nv <= 4
+    const int64_t g = gk[k];
+    const double2* Ck = evkq + static_cast<size_t>(k) * nb * norb;
+    const double2* Vk = evk + static_cast<size_t>(k) * nb * norb;
+    cx_t<R> acc[kMax][kMax];
+    for (int c = 0; c < kMax; ++c)
+        for (int v = 0; v < kMax; ++v) acc[c][v] = mk(R(0), R(0));
+    for (int be = 0; be < norb; ++be) {
+        cx_t<R> t[kMax];
+        for (int c = 0; c < kMax; ++c) t[c] = mk(R(0), R(0));
+        for (int al = 0; al < norb; ++al) {
+            const cx_t<R> w = W[((static_cast<int64_t>(al) * norb + be) * nvec + n) * M + g];
+            for (int c = 0; c < nc; ++c) {
+                const double2 cv = Ck[static_cast<size_t>(nv + c) * norb + al];
+                cfma_conj(t[c], mk(static_cast<R>(cv.x), static_cast<R>(cv.y)), w);
+            }
+        }
+        for (int v = 0; v < nv; ++v) {
+            const double2 vv = Vk[static_cast<size_t>(v) * norb + be];
+            const cx_t<R> vr = mk(static_cast<R>(vv.x), static_cast<R>(vv.y));
+            for (int c = 0; c < nc; ++c) cfma(acc[c][v], vr, t[c]);
+        }
+    }
+    for (int c = 0; c < nc; ++c)
+        for (int v = 0; v < nv; ++v) {
+            const int64_t row = static_cast<int64_t>(k) * ncv + c * nv + v;
+            const double e = (scissor + eval_kq[static_cast<size_t>(k) * nb + nv + c]) - eval_k[static_cast<size_t>(k) * nb + v];
+            const double2 x = X[row + n * N];
+            Y[row + n * N] = mk(e * x.x - static_cast<double>(acc[c][v].x), e * x.y - static_cast<double>(acc[c][v].y));
+        }
+}
+
+// ---- exchange term along the upper triangle (FP64) ----
+// Block sums of A_a(i) = E2_a(i) x_i and B_a(i) = conj(E1_a(i)) x_i over kScanRows rows:
+// part[((blk NA + a) 2 + s) nvec + n]
+__global__ __launch_bounds__(kScanRows) void xscan_partial_kernel(int na, int ncv, const double2* __restrict__ e1,
+                                                                  const double2* __restrict__ e2,
+                                                                  const double2* __restrict__ X, int64_t N, int nvec,
+                                                                  double2* __restrict__ part) {
+    __shared__ double2 s[kScanRows];
+    const int n = blockIdx.y;
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kScanRows + threadIdx.x;
+    const bool ok = i < N;
+    const double2 x = ok ? X[i + n * N] : mk(0.0, 0.0);
+    const int64_t k = ok ? i / ncv : 0;
+    const int cv = ok ? static_cast<int>(i % ncv) : 0;
+    for (int a = 0; a < na; ++a) {
+        for (int sidx = 0; sidx < 2; ++sidx) {
+            double2 v = mk(0.0, 0.0);
+            if (ok) {
+                const size_t off = (static_cast<size_t>(k) * na + a) * ncv + cv;
+                v = sidx == 0 ? cmul(e2[off], x) : cmul(cconj(e1[off]), x);
+            }
+            s[threadIdx.x] = v;
+            __syncthreads();
+            for (int st = kScanRows / 2; st > 0; st >>= 1) {
+                if (threadIdx.x < st) s[threadIdx.x] = cadd(s[threadIdx.x], s[threadIdx.x + st]);
+                __syncthreads();
+            }
+            if (threadIdx.x == 0) part[((static_cast<size_t>(blockIdx.x) * na + a) * 2 + sidx) * nvec + n] = s[0];
+            __syncthreads();
+        }
+    }
+}
+
+// Exclusive prefix over blocks, in place, one thread per series (a, s, n); also the totals.
+__global__ void xscan_offsets_kernel(int nblk, int nseries, double2* __restrict__ part, double2* __restrict__ total) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nseries) return;
+    double2 run = mk(0.0, 0.0);
+    for (int b = 0; b < nblk; ++b) {
+        const size_t i = static_cast<size_t>(b) * nseries + t;
+        const double2 v = part[i];
+        part[i] = run;
+        run = cadd(run, v);
+    }
+    total[t] = run;
+}
+
+// y_i += sum_a [E1_a(i) (A_tot - sum_{j<=i} A_a(j)) + conj(E2_a(i)) sum_{j<i} B_a(j)] + Re(sum_a E1_a E2_a) x_i
+__global__ __launch_bounds__(kScanRows) void xscan_apply_kernel(int na, int ncv, const double2* __restrict__ e1,
+                                                                const double2* __restrict__ e2,
+                                                                const double2* __restrict__ X, int64_t N, int nvec,
+                                                                const double2* __restrict__ part,
+                                                                const double2* __restrict__ total,
+                                                                double2* __restrict__ Y) {
+    __shared__ double2 s[kScanRows];
+    const int n = blockIdx.y;
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kScanRows + threadIdx.x;
+    const bool ok = i < N;
+    const double2 x = ok ? X[i + n * N] : mk(0.0, 0.0);
+    const int64_t k = ok ? i / ncv : 0;
+    const int cv = ok ? static_cast<int>(i % ncv) : 0;
+    const int nseries = na * 2 * nvec;
+    double2 y = mk(0.0, 0.0);
+    double diag = 0.0;
+    for (int a = 0; a < na; ++a) {
+        const size_t off = (static_cast<size_t>(k) * na + a) * ncv + cv;
+        const double2 E1 = ok ? e1[off] : mk(0.0, 0.0);
+        const double2 E2 = ok ? e2[off] : mk(0.0, 0.0);
+        diag += E1.x * E2.x - E1.y * E2.y;
+        for (int sidx = 0; sidx < 2; ++sidx) {
+            const double2 v = sidx == 0 ? cmul(E2, x) : cmul(cconj(E1), x);
+            // inclusive scan within the block (Hillis-Steele, fixed order)
+            s[threadIdx.x] = v;
+            __syncthreads();
+            for (int st = 1; st < kScanRows; st <<= 1) {
+                const double2 add = threadIdx.x >= st ? s[threadIdx.x - st] : mk(0.0, 0.0);
+                __syncthreads();
+                s[threadIdx.x] = cadd(s[threadIdx.x], add);
+                __syncthreads();
+            }
+            const size_t ser = (static_cast<size_t>(a) * 2 + sidx) * nvec + n;
+            const double2 before = part[static_cast<size_t>(blockIdx.x) * nseries + ser];  // sum over earlier blocks
+            const double2 incl = cadd(before, s[threadIdx.x]);
+            if (sidx == 0) {
+                const double2 after = csub(total[ser], incl);  // sum_{j>i} A_a(j)
+                cfma(y, E1, after);
+            } else {
+                const double2 excl = csub(incl, v);            // sum_{j<i} B_a(j)
+                cfma(y, cconj(E2), excl);
+            }
+            __syncthreads();
+        }
+    }
+    if (ok) {
+        double2 o = Y[i + n * N];
+        o.x += y.x + diag * x.x;
+        o.y += y.y + diag * x.y;
+        Y[i + n * N] = o;
+    }
+}
+
+template <typename R>
+constexpr rocfft_precision fft_precision() {
+    return sizeof(R) == 8 ? rocfft_precision_double : rocfft_precision_single;
+}
+
+}  // namespace
+
+struct ConvState {
+    int gen = -1;
+    int64_t M = 0;
+    DevBuf<int> d_gk;
+    DevBuf<double2> P64;
+    DevBuf<float2> P32;
+    DevBuf<double2> G64, U64;
+    DevBuf<float2> G32, U32;
+    DevBuf<double2> part, total;
+    struct Plan {
+        rocfft_plan fwd = nullptr, bwd = nullptr;
+        rocfft_execution_info info = nullptr;
+        void* work = nullptr;
+    };
+    std::map<std::pair<int, int>, Plan> plans;  // (bytes per real, batch)
+    ~ConvState() {
+        for (auto& kv : plans) {
+            if (kv.second.fwd) rocfft_plan_destroy(kv.second.fwd);
+            if (kv.second.bwd) rocfft_plan_destroy(kv.second.bwd);
+            if (kv.second.info) rocfft_execution_info_destroy(kv.second.info);
+            if (kv.second.work) (void)hipFree(kv.second.work);
+        }
+    }
+};
+
+void destroy_conv_state(ConvState* s) { delete s; }
+
+namespace {
+
+void fft_setup_once() {
+    static std::once_flag flag;
+    static rocfft_status st = rocfft_status_success;
+    std::call_once(flag, [] { st = rocfft_setup(); });
+    if (st != rocfft_status_success) throw Error(XATU_ERR_DEVICE, "rocfft_setup failed");
+}
+
+template <typename R>
+ConvState::Plan& get_plan(Context& c, ConvState& cs, int batch) {
+    const auto key = std::make_pair(static_cast<int>(sizeof(R)), batch);
+    auto it = cs.plans.find(key);
+    if (it != cs.plans.end()) return it->second;
+    fft_setup_once();
+    ConvState::Plan p;
+    size_t lengths[3] = {static_cast<size_t>(c.lat.mesh_n), static_cast<size_t>(c.lat.mesh_n),
+                         static_cast<size_t>(c.lat.mesh_n)};
+    XATU_FFT(rocfft_plan_create(&p.fwd, rocfft_placement_inplace, rocfft_transform_type_complex_forward,
+                                fft_precision<R>(), c.lat.ndim, lengths, batch, nullptr));
+    XATU_FFT(rocfft_plan_create(&p.bwd, rocfft_placement_inplace, rocfft_transform_type_complex_inverse,
+                                fft_precision<R>(), c.lat.ndim, lengths, batch, nullptr));
+    size_t w1 = 0, w2 = 0;
+    XATU_FFT(rocfft_plan_get_work_buffer_size(p.fwd, &w1));
+    XATU_FFT(rocfft_plan_get_work_buffer_size(p.bwd, &w2));
+    XATU_FFT(rocfft_execution_info_create(&p.info));
+    XATU_FFT(rocfft_execution_info_set_stream(p.info, c.stream));
+    const size_t w = std::max(w1, w2);
+    if (w > 0) {
+        XATU_HIP(hipMalloc(&p.work, w));
+        XATU_FFT(rocfft_execution_info_set_work_buffer(p.info, p.work, w));
+    }
+    return cs.plans.emplace(key, p).first->second;
+}
+
+// Kernel spectra P_ab = (1/M) backward FFT of the table on the mesh, and the grid
+// index of every k-point; rebuilt after each preparation of the operator.
+void ensure_conv(Context& c) {
+    if (!c.conv) c.conv = std::unique_ptr<ConvState, void (*)(ConvState*)>(new ConvState, destroy_conv_state);
+    ConvState& cs = *c.conv;
+    if (cs.gen == c.prep_gen) return;
+    if (c.lat.ndim < 1 || c.lat.ndim > 2) throw Error(XATU_ERR_INVALID_ARGUMENT, "convolution products need ndim 1 or 2");
+    const int64_t M = c.lat.nmesh;
+    cs.M = M;
+    const int m = c.lat.mesh_n;
+    std::vector<int> gk(c.nk);
+    for (int k = 0; k < c.nk; ++k) {
+        int64_t g = 0, mul = 1;
+        for (int j = 0; j < c.lat.ndim; ++j) {
+            int o = c.h_koff[static_cast<size_t>(k) * c.lat.ndim + j] % m;
+            if (o < 0) o += m;
+            g += o * mul;
+            mul *= m;
+        }
+        gk[k] = static_cast<int>(g);
+    }
+    cs.d_gk.alloc(c.nk);
+    XATU_HIP(hipMemcpyAsync(cs.d_gk.p, gk.data(), gk.size() * sizeof(int), hipMemcpyHostToDevice, c.stream));
+    const int na2 = c.lat.natoms * c.lat.natoms;
+    DevBuf<double2> Mt;
+    Mt.alloc(static_cast<size_t>(na2) * M);
+    hipLaunchKernelGGL(table_grid_kernel, dim3((Mt.n + 255) / 256), dim3(256), 0, c.stream, c.lat, c.d_ft.p, Mt.p);
+    XATU_HIP(hipGetLastError());
+    ConvState::Plan& p = get_plan<double>(c, cs, na2);
+    void* buf[1] = {Mt.p};
+    XATU_FFT(rocfft_execute(p.bwd, buf, nullptr, p.info));
+    cs.P64.alloc(Mt.n);
+    cs.P32.alloc(Mt.n);
+    const double scale = 1.0 / static_cast<double>(M);
+    hipLaunchKernelGGL((scale_spectrum_kernel<double>), dim3((Mt.n + 255) / 256), dim3(256), 0, c.stream, Mt.p,
+                       static_cast<int64_t>(Mt.n), scale, cs.P64.p);
+    hipLaunchKernelGGL((scale_spectrum_kernel<float>), dim3((Mt.n + 255) / 256), dim3(256), 0, c.stream, Mt.p,
+                       static_cast<int64_t>(Mt.n), scale, cs.P32.p);
+    XATU_HIP(hipGetLastError());
+    XATU_HIP(hipStreamSynchronize(c.stream));  // Mt goes out of scope
+    cs.gen = c.prep_gen;
+}
+
+template <typename R>
+void conv_product(Context& c, const double2* X, double2* Y, int nvec, hipStream_t s, DevBuf<cx_t<R>>& G,
+                  DevBuf<cx_t<R>>& U, const cx_t<R>* P) {
+    ConvState& cs = *c.conv;
+    const int norb = c.lat.norb;
+    const int64_t M = cs.M;
+    const int64_t N = c.N;
+    if (c.nc > 4 || c.nv > 4) throw Error(XATU_ERR_INVALID_ARGUMENT, "convolution products need nc, nv <= 4");
+    const bool full = (static_cast<int64_t>(c.nk) == M);
+    for (int v0 = 0; v0 < nvec; v0 += kBlockVectors) {
+        const int nv = std::min(kBlockVectors, nvec - v0);
+        const double2* Xb = X + static_cast<int64_t>(v0) * N;
+        double2* Yb = Y + static_cast<int64_t>(v0) * N;
+        const int batch = norb * norb * nv;
+        G.alloc(std::max<size_t>(G.n, static_cast<size_t>(batch) * M));
+        U.alloc(std::max<size_t>(U.n, static_cast<size_t>(nv) * c.nc * norb * c.nk));
+        if (!full) XATU_HIP(hipMemsetAsync(G.p, 0, static_cast<size_t>(batch) * M * sizeof(cx_t<R>), s));
+        const dim3 kb((c.nk + 255) / 256);
+        hipLaunchKernelGGL((conv_u_kernel<R>), dim3(kb.x, nv), dim3(256), 0, s, c.nk, c.nv, c.nc, norb, c.d_evec_k.p,
+                           Xb, N, U.p);
+        hipLaunchKernelGGL((conv_z_kernel<R>), dim3(kb.x, norb, nv), dim3(256), 0, s, c.nk, c.nv, c.nc, norb, nv, M,
+                           cs.d_gk.p, c.d_evec_kq.p, U.p, G.p);
+        XATU_HIP(hipGetLastError());
+        ConvState::Plan& p = get_plan<R>(c, cs, batch);
+        void* buf[1] = {G.p};
+        XATU_FFT(rocfft_execute(p.fwd, buf, nullptr, p.info));
+        hipLaunchKernelGGL((conv_multiply_kernel<R>), dim3((M + 255) / 256, batch), dim3(256), 0, s, c.lat, nv, M, P,
+                           G.p);
+        XATU_HIP(hipGetLastError());
+        XATU_FFT(rocfft_execute(p.bwd, buf, nullptr, p.info));
+        hipLaunchKernelGGL((conv_y_kernel<R>), dim3(kb.x, nv), dim3(256), 0, s, c.nk, c.nv, c.nc, norb, nv, M,
+                           cs.d_gk.p, c.d_evec_k.p, c.d_evec_kq.p, c.d_eval_k.p, c.d_eval_kq.p, c.scissor, G.p, Xb, N,
+                           Yb);
+        XATU_HIP(hipGetLastError());
+        if (c.inter.exchange) {
+            const int na = c.lat.natoms;
+            const int ncv = c.nc * c.nv;
+            const int nblk = static_cast<int>((N + kScanRows - 1) / kScanRows);
+            const int nseries = na * 2 * nv;
+            cs.part.alloc(std::max<size_t>(cs.part.n, static_cast<size_t>(nblk) * nseries));
+            cs.total.alloc(std::max<size_t>(cs.total.n, static_cast<size_t>(nseries)));
+            hipLaunchKernelGGL(xscan_partial_kernel, dim3(nblk, nv), dim3(kScanRows), 0, s, na, ncv, c.d_e1.p,
+                               c.d_e2.p, Xb, N, nv, cs.part.p);
+            hipLaunchKernelGGL(xscan_offsets_kernel, dim3((nseries + 255) / 256), dim3(256), 0, s, nblk, nseries,
+                               cs.part.p, cs.total.p);
+            hipLaunchKernelGGL(xscan_apply_kernel, dim3(nblk, nv), dim3(kScanRows), 0, s, na, ncv, c.d_e1.p, c.d_e2.p,
+                               Xb, N, nv, cs.part.p, cs.total.p, Yb);
+            XATU_HIP(hipGetLastError());
+        }
+    }
+}
+
+}  // namespace
+
+void launch_conv64(Context& c, const double2* X, double2* Y, int nvec, hipStream_t s) {
+    ensure_conv(c);
+    conv_product<double>(c, X, Y, nvec, s, c.conv->G64, c.conv->U64, c.conv->P64.p);
+}
+
+void launch_conv32(Context& c, const double2* X, double2* Y, int nvec, hipStream_t s) {
+    ensure_conv(c);
+    conv_product<float>(c, X, Y, nvec, s, c.conv->G32, c.conv->U32, c.conv->P32.p);
+}
+
+}  // namespace xatu_dev

@@ -357,7 +357,7 @@ void solve_davidson(Context& c, const xatu_solve_options& opt, double* eigval_ou
         // (matrix-free when no FP64 matrix is resident), until the FP64 residuals meet tol.
         const int bref = nev;
         const int mref = static_cast<int>(std::min<int64_t>(N, std::max(4 * bref, bref + 1)));
-        dav = std::make_unique<Davidson>(c, XATU_FP64, opt.matrix_free || !c.built64, nev, bref, mref, rep);
+        dav = std::make_unique<Davidson>(c, XATU_FP64, opt.matrix_free ? opt.matrix_free : (c.built64 ? 0 : 1), nev, bref, mref, rep);
         dav->init_from(d32->ritz_vectors());
         ok = dav->run(tol, maxit);
         rep->refine_iterations = rep->iterations - it32;
