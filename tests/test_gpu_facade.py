@@ -51,12 +51,7 @@ def test_hbn_matrix_matches_oracle():
 def test_hbn_kwf_hash():
     # test/tests.cpp:281-321
     ex, res = hbn("diag", 2)
-    v = res.eigvec
-    kp = ex.system.kpoints
-    dk = np.linalg.norm(kp[1] - kp[0])
-    kwf = np.zeros(len(kp), complex)
-    for n in range(2):
-        kwf += (np.abs(v[:, n]) ** 2).reshape(len(kp), 1).sum(1) / dk
+    kwf = sum(np.asarray(res.reciprocalAmplitude(n)) for n in range(2)) + 0j
     assert abs(O.array2hash(kwf) - 1.086145105) < 1e-5
 
 
@@ -112,13 +107,22 @@ def test_mos2_reduced_bz():
 def test_mos2_kwf_hash():
     # test/tests.cpp:579-619
     ex, res = mos2()
-    v = res.eigvec
-    kp = ex.system.kpoints
-    dk = np.linalg.norm(kp[1] - kp[0])
-    kwf = np.zeros(len(kp), complex)
-    for n in range(2):
-        kwf += (np.abs(v[:, n]) ** 2).reshape(len(kp), 4).sum(1) / dk
+    kwf = sum(np.asarray(res.reciprocalAmplitude(n)) for n in range(2)) + 0j
     assert abs(O.array2hash(kwf) - 1.1814790902) < 1e-5
+
+
+def test_reciprocal_amplitude_file_format(tmp_path):
+    """writeReciprocalAmplitude (Result.tpp:163-177): header, kx ky kz Prob. rows, '#'."""
+    ex, res = hbn("diag", 2)
+    f = tmp_path / "kwf.txt"
+    res.writeReciprocalAmplitude(0, str(f))
+    lines = f.read_text().splitlines()
+    assert lines[0] == "kx\tky\tkz\tProb." and lines[-1] == "#"
+    vals = np.array([[float(x) for x in ln.split("\t")] for ln in lines[1:-1]])
+    kp = np.asarray(ex.system.kpoints)
+    assert vals.shape == (len(kp), 4)
+    assert np.allclose(vals[:, :3], kp, atol=1e-8)
+    assert np.allclose(vals[:, 3], np.asarray(res.reciprocalAmplitude(0)), atol=1e-8)
 
 
 def test_mos2_davidson_fp32_mixed():

@@ -641,6 +641,50 @@ void ResultTB::writeStates(FILE* f, int n) const {
     }
 }
 
+// Squared k-space amplitude of a state: sum over its band pairs at each k of |coef|^2,
+// divided by |k_1 - k_0| (include/xatu/Result.tpp:163-177).
+std::vector<double> ResultTB::reciprocalAmplitude(const std::vector<cdouble>& coefs) const {
+    const SystemTB& s = *exciton->system;
+    const long long nbc = static_cast<long long>(exciton->conductionBands.size() * exciton->valenceBands.size());
+    if (static_cast<long long>(coefs.size()) != nrows) throw std::invalid_argument("reciprocalAmplitude: wrong state size");
+    if (s.nk < 2) throw std::invalid_argument("reciprocalAmplitude: needs at least two k-points");
+    double dk = 0.0;
+    for (int x = 0; x < 3; ++x) dk += (s.kpoints[3 + x] - s.kpoints[x]) * (s.kpoints[3 + x] - s.kpoints[x]);
+    dk = std::sqrt(dk);
+    std::vector<double> amp(s.nk, 0.0);
+    for (int k = 0; k < s.nk; ++k) {
+        double a = 0.0;
+        for (long long b = 0; b < nbc; ++b) a += std::norm(coefs[nbc * k + b]);
+        amp[k] = a / dk;
+    }
+    return amp;
+}
+
+std::vector<double> ResultTB::reciprocalAmplitude(int stateindex) const {
+    if (stateindex < 0 || stateindex >= ncols) throw std::invalid_argument("reciprocalAmplitude: state index out of range");
+    std::vector<cdouble> c(eigvec.begin() + static_cast<size_t>(stateindex) * nrows,
+                           eigvec.begin() + static_cast<size_t>(stateindex + 1) * nrows);
+    return reciprocalAmplitude(c);
+}
+
+void ResultTB::writeReciprocalAmplitude(const std::vector<cdouble>& coefs, FILE* f) const {
+    const SystemTB& s = *exciton->system;
+    const std::vector<double> amp = reciprocalAmplitude(coefs);
+    fprintf(f, "kx\tky\tkz\tProb.\n");
+    for (int k = 0; k < s.nk; ++k) {
+        const double* kp = s.kpoints.data() + 3 * static_cast<size_t>(k);
+        fprintf(f, "%11.8lf\t%11.8lf\t%11.8lf\t%11.8lf\n", kp[0], kp[1], kp[2], amp[k]);
+    }
+    fprintf(f, "#\n");
+}
+
+void ResultTB::writeReciprocalAmplitude(int stateindex, FILE* f) const {
+    if (stateindex < 0 || stateindex >= ncols) throw std::invalid_argument("writeReciprocalAmplitude: state index out of range");
+    std::vector<cdouble> c(eigvec.begin() + static_cast<size_t>(stateindex) * nrows,
+                           eigvec.begin() + static_cast<size_t>(stateindex + 1) * nrows);
+    writeReciprocalAmplitude(c, f);
+}
+
 std::vector<std::vector<double>> detectDegeneracies(const std::vector<double>& eigval, int n, int precision) {
     if (n < 0) throw std::invalid_argument("detectDegeneracies: n must be a positive integer");
     if (n > static_cast<int>(eigval.size())) {

@@ -153,6 +153,11 @@ public:
     // Output formats (Result.tpp:404-448)
     void writeEigenvalues(FILE* textfile, int n = 0) const;
     void writeStates(FILE* textfile, int n = 0) const;
+    // k-space amplitude (Result.tpp:163-190): per k, sum_pairs |coef|^2 / |k_1 - k_0|
+    std::vector<double> reciprocalAmplitude(const std::vector<cdouble>& coefs) const;
+    std::vector<double> reciprocalAmplitude(int stateindex) const;
+    void writeReciprocalAmplitude(const std::vector<cdouble>& coefs, FILE* textfile) const;
+    void writeReciprocalAmplitude(int stateindex, FILE* textfile) const;
 };
 
 // Single-particle velocity matrix elements of the k stack, V[((k*3 + j)*nb + n)*nb + m] (a.u.).

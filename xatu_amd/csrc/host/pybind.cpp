@@ -189,6 +189,18 @@ PYBIND11_MODULE(_xatu, m) {
                  std::fclose(f);
              },
              py::arg("filename"), py::arg("n") = 0)
+        .def("reciprocalAmplitude",
+             [](const ResultTB& r, int i) {
+                 std::vector<double> a = r.reciprocalAmplitude(i);
+                 return py::array_t<double>(a.size(), a.data());
+             })
+        .def("writeReciprocalAmplitude",
+             [](const ResultTB& r, int i, const std::string& name) {
+                 FILE* f = open_or_throw(name);
+                 r.writeReciprocalAmplitude(i, f);
+                 std::fclose(f);
+             },
+             py::arg("stateindex"), py::arg("filename"))
         .def("writeStates",
              [](const ResultTB& r, const std::string& name, int n) {
                  FILE* f = open_or_throw(name);
