@@ -148,6 +148,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the FP64 plain path")
     args = ap.parse_args()
+    # exactly one JSON line on stdout: everything else (including what the native library
+    # prints) goes to stderr
+    sys.stdout.flush()
+    json_fd = os.dup(1)
+    os.dup2(2, 1)
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path runs on a single "
               "MI355X; see DESIGN.md for how it would shard)", file=sys.stderr)
@@ -262,7 +267,9 @@ def main():
         out["cpu_baseline"] = cpu_baseline(model, cfg, n_products, nstates, block)
     if ctx is not None:
         ctx.close()
-    print(json.dumps(out), flush=True)
+    sys.stdout.flush()
+    os.write(json_fd, (json.dumps(out) + "\n").encode())
+    os.close(json_fd)
 
 
 if __name__ == "__main__":

@@ -207,6 +207,8 @@ void launch_gram(Context& c, const double2* A, int lda, const double2* B, int ld
 // the exchange term along the upper triangle, in FP64 / FP32 arithmetic.
 void launch_conv64(Context& c, const double2* X, double2* Y, int nvec, hipStream_t s);
 void launch_conv32(Context& c, const double2* X, double2* Y, int nvec, hipStream_t s);
+// S[(n nmesh + q) NA^2 + a + NA b] = sum_k G_ab(k, k+q) for ns <= 32 states (FFT autocorrelation).
+void conv_pair_density(Context& c, const double2* psi, int ns, double2* S);
 // Matrix-free products Y = H X (complex FP64 vectors) with tiles of H recomputed in
 // FP64 / FP32 arithmetic inside the product (FP32: FP32 tiles, FP32 accumulation).
 void launch_matfree64(Context& c, const double2* X, double2* Y, int nvec, hipStream_t s);

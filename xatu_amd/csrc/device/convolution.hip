@@ -284,6 +284,45 @@ __global__ __launch_bounds__(kScanRows) void xscan_apply_kernel(int na, int ncv,
     }
 }
 
+// Pw[(n NA^2 + a + NA b) M + xi] = sum_{al in a, be in b} |G[((al norb + be) ns + n) M + xi]|^2
+__global__ void power_sum_kernel(LatticeDesc lat, int ns, int64_t M, const double2* __restrict__ G,
+                                 double2* __restrict__ Pw) {
+    const int64_t xi = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int ab = blockIdx.y, n = blockIdx.z;
+    if (xi >= M) return;
+    const int a = ab % lat.natoms, b = ab / lat.natoms;
+    double sum = 0.0;
+    for (int al = lat.orb_begin[a]; al < lat.orb_begin[a + 1]; ++al)
+        for (int be = lat.orb_begin[b]; be < lat.orb_begin[b + 1]; ++be) {
+            const double2 z = G[((static_cast<int64_t>(al) * lat.norb + be) * ns + n) * M + xi];
+            sum += z.x * z.x + z.y * z.y;
+        }
+    Pw[(static_cast<int64_t>(n) * lat.natoms * lat.natoms + ab) * M + xi] = mk(sum, 0.0);
+}
+
+// S[(n nmesh + q) NA^2 + ab] = W[(n NA^2 + ab) M + g(Delta(q))] / M with Delta_j = (q_j - m/2) mod m
+__global__ void scatter_pair_density_kernel(LatticeDesc lat, int ns, const double2* __restrict__ W,
+                                            double2* __restrict__ S) {
+    const int64_t M = lat.nmesh;
+    const int na2 = lat.natoms * lat.natoms;
+    const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= M * na2 * ns) return;
+    const int ab = static_cast<int>(idx % na2);
+    const int64_t q = (idx / na2) % M;
+    const int n = static_cast<int>(idx / (na2 * M));
+    const int m = lat.mesh_n;
+    int64_t g = 0, mul = 1, rem = q;
+    for (int j = 0; j < lat.ndim; ++j) {
+        const int d = static_cast<int>(rem % m);
+        rem /= m;
+        g += static_cast<int64_t>(((d - m / 2) % m + m) % m) * mul;
+        mul *= m;
+    }
+    const double2 w = W[(static_cast<int64_t>(n) * na2 + ab) * M + g];
+    const double inv = 1.0 / static_cast<double>(M);
+    S[idx] = mk(w.x * inv, w.y * inv);
+}
+
 template <typename R>
 constexpr rocfft_precision fft_precision() {
     return sizeof(R) == 8 ? rocfft_precision_double : rocfft_precision_single;
@@ -449,6 +488,46 @@ void conv_product(Context& c, const double2* X, double2* Y, int nvec, hipStream_
 }
 
 }  // namespace
+
+// Pair-density adjoint of the direct term for the Hellmann-Feynman gradients:
+//   S_ab(q) = sum_k G_ab(k, k+q),  G_ab(k,k') = sum_{al in a, be in b} conj(z_ab(k)) z_ab(k'),
+// with z_{al be}(k) = sum_{cv} C_kc,al conj(V_kv,be) psi_kcv (the vector the convolution
+// product forms from x = psi). The sum over k is an autocorrelation on the mesh:
+// S_ab(Delta) = (1/M) backward( sum_{al in a, be in b} |forward(z_{al be})|^2 )(Delta).
+// Output layout S[(n nmesh + q) NA^2 + a + NA b] for the ns states of psi (N x ns).
+void conv_pair_density(Context& c, const double2* psi, int ns, double2* S) {
+    ensure_conv(c);
+    ConvState& cs = *c.conv;
+    const int norb = c.lat.norb;
+    const int na2 = c.lat.natoms * c.lat.natoms;
+    const int64_t M = cs.M;
+    const int64_t N = c.N;
+    hipStream_t s = c.stream;
+    if (ns < 1 || ns > kBlockVectors) throw Error(XATU_ERR_INVALID_ARGUMENT, "pair density: 1..32 states per call");
+    const int batch = norb * norb * ns;
+    cs.G64.alloc(std::max<size_t>(cs.G64.n, static_cast<size_t>(batch) * M));
+    cs.U64.alloc(std::max<size_t>(cs.U64.n, static_cast<size_t>(ns) * c.nc * norb * c.nk));
+    if (static_cast<int64_t>(c.nk) != M) XATU_HIP(hipMemsetAsync(cs.G64.p, 0, static_cast<size_t>(batch) * M * sizeof(double2), s));
+    const dim3 kb((c.nk + 255) / 256);
+    hipLaunchKernelGGL((conv_u_kernel<double>), dim3(kb.x, ns), dim3(256), 0, s, c.nk, c.nv, c.nc, norb, c.d_evec_k.p,
+                       psi, N, cs.U64.p);
+    hipLaunchKernelGGL((conv_z_kernel<double>), dim3(kb.x, norb, ns), dim3(256), 0, s, c.nk, c.nv, c.nc, norb, ns, M,
+                       cs.d_gk.p, c.d_evec_kq.p, cs.U64.p, cs.G64.p);
+    XATU_HIP(hipGetLastError());
+    ConvState::Plan& pf = get_plan<double>(c, cs, batch);
+    void* buf[1] = {cs.G64.p};
+    XATU_FFT(rocfft_execute(pf.fwd, buf, nullptr, pf.info));
+    DevBuf<double2> Pw;
+    Pw.alloc(static_cast<size_t>(ns) * na2 * M);
+    hipLaunchKernelGGL(power_sum_kernel, dim3((M + 255) / 256, na2, ns), dim3(256), 0, s, c.lat, ns, M, cs.G64.p, Pw.p);
+    XATU_HIP(hipGetLastError());
+    ConvState::Plan& pb = get_plan<double>(c, cs, na2 * ns);
+    void* buf2[1] = {Pw.p};
+    XATU_FFT(rocfft_execute(pb.bwd, buf2, nullptr, pb.info));
+    hipLaunchKernelGGL(scatter_pair_density_kernel, dim3((Pw.n + 255) / 256), dim3(256), 0, s, c.lat, ns, Pw.p, S);
+    XATU_HIP(hipGetLastError());
+    XATU_HIP(hipStreamSynchronize(s));  // Pw goes out of scope
+}
 
 void launch_conv64(Context& c, const double2* X, double2* Y, int nvec, hipStream_t s) {
     ensure_conv(c);

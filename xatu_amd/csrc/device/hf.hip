@@ -260,8 +260,13 @@ void hf_gradients(Context& c, int nstates, const double2* d_vec, int nparams, co
     shat.alloc(static_cast<size_t>(nstates) * npairs * nR);
     for (int s0 = 0; s0 < nstates; s0 += kHfStates) {
         const int ns = std::min(kHfStates, nstates - s0);
-        hipLaunchKernelGGL(pair_adjoint_kernel, dim3(c.lat.nmesh), dim3(kHfThreads), 0, s, c.lat, c.nk, c.nv, c.nc,
-                           c.d_evec_k.p, c.d_evec_kq.p, c.d_kcoord.p, d_kmap.p, d_vec, c.N, s0, ns, S.p);
+        if (c.lat.ndim <= 2) {
+            // autocorrelation on the mesh with FFTs: O(norb^2 M log M) instead of O(nk^2) pairs
+            conv_pair_density(c, d_vec + static_cast<int64_t>(s0) * c.N, ns, S.p);
+        } else {
+            hipLaunchKernelGGL(pair_adjoint_kernel, dim3(c.lat.nmesh), dim3(kHfThreads), 0, s, c.lat, c.nk, c.nv,
+                               c.nc, c.d_evec_k.p, c.d_evec_kq.p, c.d_kcoord.p, d_kmap.p, d_vec, c.N, s0, ns, S.p);
+        }
         hipLaunchKernelGGL(adjoint_fourier_kernel, dim3((nR + 255) / 256, npairs, ns), dim3(256), 0, s, c.lat,
                            c.d_cells.p, nR, npairs, S.p, c.d_twiddle.p,
                            shat.p + static_cast<size_t>(s0) * npairs * nR);
