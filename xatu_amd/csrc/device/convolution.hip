@@ -29,6 +29,7 @@
 // precision and batch size. The exchange part runs in FP64 for both precisions.
 #include <map>
 #include <mutex>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -84,45 +85,105 @@ __global__ void scale_spectrum_kernel(const double2* __restrict__ in, int64_t n,
     if (i < n) out[i] = mk(static_cast<R>(in[i].x * scale), static_cast<R>(in[i].y * scale));
 }
 
-// u[((n NC + c) norb + be) nk + k] = sum_v conj(V_k[be, v]) x[k NCV + c NV + v, n]
+// Band stacks transposed for coalesced access across k: Ct[(c norb + al) nk + k] =
+// C^{k+Q}_{nv+c}(k)[al] (conduction) and Vt[(v norb + be) nk + k] = C^{k}_{v}(k)[be] (valence).
 template <typename R>
-__global__ void conv_u_kernel(int nk, int nv, int nc, int norb, const double2* __restrict__ evk,
-                              const double2* __restrict__ X, int64_t N, cx_t<R>* __restrict__ U) {
+__global__ void transpose_stacks_kernel(int nk, int nv, int nc, int norb, const double2* __restrict__ evk,
+                                        const double2* __restrict__ evkq, cx_t<R>* __restrict__ Ct,
+                                        cx_t<R>* __restrict__ Vt) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int n = blockIdx.y;
+    const int col = blockIdx.y;  // 0..nc*norb-1 conduction, then nv*norb valence
     if (k >= nk) return;
-    const int nb = nv + nc, ncv = nc * nv;
-    const double2* V = evk + static_cast<size_t>(k) * nb * norb;  // valence columns 0..nv-1
-    for (int c = 0; c < nc; ++c) {
-        for (int be = 0; be < norb; ++be) {
-            double2 s = mk(0.0, 0.0);
-            for (int v = 0; v < nv; ++v)
-                cfma_conj(s, V[static_cast<size_t>(v) * norb + be], X[static_cast<size_t>(k) * ncv + c * nv + v + n * N]);
-            U[((static_cast<size_t>(n) * nc + c) * norb + be) * nk + k] = mk(static_cast<R>(s.x), static_cast<R>(s.y));
+    const int nb = nv + nc;
+    if (col < nc * norb) {
+        const int c = col / norb, al = col % norb;
+        const double2 v = evkq[(static_cast<size_t>(k) * nb + nv + c) * norb + al];
+        Ct[static_cast<size_t>(col) * nk + k] = mk(static_cast<R>(v.x), static_cast<R>(v.y));
+    } else {
+        const int cv = col - nc * norb;
+        const int v = cv / norb, be = cv % norb;
+        const double2 w = evk[(static_cast<size_t>(k) * nb + v) * norb + be];
+        Vt[static_cast<size_t>(cv) * nk + k] = mk(static_cast<R>(w.x), static_cast<R>(w.y));
+    }
+}
+
+constexpr int kConvOrbGroup = 4;  // orbitals per thread in the z contraction
+
+// vectors per thread in the u and y contractions (register budget: FP64 halves it)
+template <typename R>
+constexpr int conv_vec_group() { return sizeof(R) == 8 ? 2 : 4; }
+
+// u[((n NC + c) norb + be) nk + k] = sum_v conj(V_k[be, v]) x[k NCV + c NV + v, n]
+// thread per k, blockIdx.y = group of vectors
+template <typename R, int NC, int NV>
+__global__ __launch_bounds__(256) void conv_u_kernel(int nk, int norb, int nvec, const cx_t<R>* __restrict__ Vt,
+                                                     const double2* __restrict__ X, int64_t N, cx_t<R>* __restrict__ U) {
+    constexpr int VG = conv_vec_group<R>();
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n0 = blockIdx.y * VG;
+    if (k >= nk) return;
+    constexpr int ncv = NC * NV;
+    cx_t<R> x[VG][NC][NV];
+#pragma unroll
+    for (int g = 0; g < VG; ++g)
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                const int n = n0 + g;
+                const double2 xv = n < nvec ? X[static_cast<size_t>(k) * ncv + c * NV + v + n * N] : mk(0.0, 0.0);
+                x[g][c][v] = mk(static_cast<R>(xv.x), static_cast<R>(xv.y));
+            }
+    for (int be = 0; be < norb; ++be) {
+        cx_t<R> vb[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) vb[v] = Vt[(static_cast<size_t>(v) * norb + be) * nk + k];
+#pragma unroll
+        for (int g = 0; g < VG; ++g) {
+            const int n = n0 + g;
+            if (n < nvec) {
+#pragma unroll
+                for (int c = 0; c < NC; ++c) {
+                    cx_t<R> s = mk(R(0), R(0));
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) cfma_conj(s, vb[v], x[g][c][v]);
+                    U[((static_cast<size_t>(n) * NC + c) * norb + be) * nk + k] = s;
+                }
+            }
         }
     }
 }
 
 // G[((al norb + be) nvec + n) M + g(k)] = sum_c C_k[al, c] u[n][c][be][k]
-template <typename R>
-__global__ void conv_z_kernel(int nk, int nv, int nc, int norb, int nvec, int64_t M, const int* __restrict__ gk,
-                              const double2* __restrict__ evkq, const cx_t<R>* __restrict__ U, cx_t<R>* __restrict__ G) {
+// thread per k, blockIdx.y = group of kConvOrbGroup orbitals al, blockIdx.z = vector n
+template <typename R, int NC>
+__global__ __launch_bounds__(256) void conv_z_kernel(int nk, int norb, int nvec, int64_t M, const int* __restrict__ gk,
+                                                     const cx_t<R>* __restrict__ Ct, const cx_t<R>* __restrict__ U,
+                                                     cx_t<R>* __restrict__ G) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int al = blockIdx.y;
+    const int al0 = blockIdx.y * kConvOrbGroup;
     const int n = blockIdx.z;
     if (k >= nk) return;
-    const int nb = nv + nc;
-    constexpr int kMaxC = 8;
-    cx_t<R> Ca[kMaxC];
-    for (int c = 0; c < nc && c < kMaxC; ++c) {
-        const double2 v = evkq[(static_cast<size_t>(k) * nb + nv + c) * norb + al];
-        Ca[c] = mk(static_cast<R>(v.x), static_cast<R>(v.y));
-    }
+    cx_t<R> Ca[kConvOrbGroup][NC];
+#pragma unroll
+    for (int a = 0; a < kConvOrbGroup; ++a)
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+            Ca[a][c] = al0 + a < norb ? Ct[(static_cast<size_t>(c) * norb + al0 + a) * nk + k] : mk(R(0), R(0));
     const int64_t g = gk[k];
     for (int be = 0; be < norb; ++be) {
-        cx_t<R> s = mk(R(0), R(0));
-        for (int c = 0; c < nc && c < kMaxC; ++c) cfma(s, Ca[c], U[((static_cast<size_t>(n) * nc + c) * norb + be) * nk + k]);
-        G[((static_cast<int64_t>(al) * norb + be) * nvec + n) * M + g] = s;
+        cx_t<R> ub[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) ub[c] = U[((static_cast<size_t>(n) * NC + c) * norb + be) * nk + k];
+#pragma unroll
+        for (int a = 0; a < kConvOrbGroup; ++a) {
+            if (al0 + a < norb) {
+                cx_t<R> s = mk(R(0), R(0));
+#pragma unroll
+                for (int c = 0; c < NC; ++c) cfma(s, Ca[a][c], ub[c]);
+                G[((static_cast<int64_t>(al0 + a) * norb + be) * nvec + n) * M + g] = s;
+            }
+        }
     }
 }
 
@@ -141,46 +202,90 @@ __global__ void conv_multiply_kernel(LatticeDesc lat, int nvec, int64_t M, const
 }
 
 // Y[k NCV + c NV + v, n] = e(kcv) x - sum_be V_k[be, v] sum_al conj(C_k[al, c]) w[(al norb + be) nvec + n][g(k)]
-template <typename R>
-__global__ void conv_y_kernel(int nk, int nv, int nc, int norb, int nvec, int64_t M, const int* __restrict__ gk,
-                              const double2* __restrict__ evk, const double2* __restrict__ evkq,
-                              const double* __restrict__ eval_k, const double* __restrict__ eval_kq, double scissor,
-                              const cx_t<R>* __restrict__ W, const double2* __restrict__ X, int64_t N,
-                              double2* __restrict__ Y) {
+// thread per k, blockIdx.y = group of vectors (the band coefficients are loaded once per group)
+template <typename R, int NC, int NV>
+__global__ __launch_bounds__(256) void conv_y_kernel(int nk, int norb, int nvec, int64_t M, const int* __restrict__ gk,
+                                                     const cx_t<R>* __restrict__ Ct, const cx_t<R>* __restrict__ Vt,
+                                                     const double* __restrict__ eval_k, const double* __restrict__ eval_kq,
+                                                     double scissor, const cx_t<R>* __restrict__ W,
+                                                     const double2* __restrict__ X, int64_t N, double2* __restrict__ Y) {
+    constexpr int VG = conv_vec_group<R>();
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int n = blockIdx.y;
+    const int n0 = blockIdx.y * VG;
     if (k >= nk) return;
-    const int nb = nv + nc, ncv = nc * nv;
-    constexpr int kMax = 4;  // nc, nv <= 4
+    constexpr int nb = NV + NC, ncv = NC * NV;
+    const int ng = min(VG, nvec - n0);
     const int64_t g = gk[k];
-    const double2* Ck = evkq + static_cast<size_t>(k) * nb * norb;
-    const double2* Vk = evk + static_cast<size_t>(k) * nb * norb;
-    cx_t<R> acc[kMax][kMax];
-    for (int c = 0; c < kMax; ++c)
-        for (int v = 0; v < kMax; ++v) acc[c][v] = mk(R(0), R(0));
+    cx_t<R> acc[VG][NC][NV];
+#pragma unroll
+    for (int q = 0; q < VG; ++q)
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+#pragma unroll
+            for (int v = 0; v < NV; ++v) acc[q][c][v] = mk(R(0), R(0));
     for (int be = 0; be < norb; ++be) {
-        cx_t<R> t[kMax];
-        for (int c = 0; c < kMax; ++c) t[c] = mk(R(0), R(0));
+        cx_t<R> t[VG][NC];
+#pragma unroll
+        for (int q = 0; q < VG; ++q)
+#pragma unroll
+            for (int c = 0; c < NC; ++c) t[q][c] = mk(R(0), R(0));
         for (int al = 0; al < norb; ++al) {
-            const cx_t<R> w = W[((static_cast<int64_t>(al) * norb + be) * nvec + n) * M + g];
-            for (int c = 0; c < nc; ++c) {
-                const double2 cv = Ck[static_cast<size_t>(nv + c) * norb + al];
-                cfma_conj(t[c], mk(static_cast<R>(cv.x), static_cast<R>(cv.y)), w);
+            cx_t<R> ca[NC];
+#pragma unroll
+            for (int c = 0; c < NC; ++c) ca[c] = Ct[(static_cast<size_t>(c) * norb + al) * nk + k];
+            const cx_t<R>* w = W + ((static_cast<int64_t>(al) * norb + be) * nvec + n0) * M + g;
+#pragma unroll
+            for (int q = 0; q < VG; ++q) {
+                if (q < ng) {
+                    const cx_t<R> wq = w[q * M];
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) cfma_conj(t[q][c], ca[c], wq);
+                }
             }
         }
-        for (int v = 0; v < nv; ++v) {
-            const double2 vv = Vk[static_cast<size_t>(v) * norb + be];
-            const cx_t<R> vr = mk(static_cast<R>(vv.x), static_cast<R>(vv.y));
-            for (int c = 0; c < nc; ++c) cfma(acc[c][v], vr, t[c]);
+        cx_t<R> vb[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) vb[v] = Vt[(static_cast<size_t>(v) * norb + be) * nk + k];
+#pragma unroll
+        for (int q = 0; q < VG; ++q)
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+#pragma unroll
+                for (int v = 0; v < NV; ++v) cfma(acc[q][c][v], vb[v], t[q][c]);
+    }
+#pragma unroll
+    for (int q = 0; q < VG; ++q) {
+        if (q < ng) {
+            const int n = n0 + q;
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+#pragma unroll
+                for (int v = 0; v < NV; ++v) {
+                    const int64_t row = static_cast<int64_t>(k) * ncv + c * NV + v;
+                    const double e = (scissor + eval_kq[static_cast<size_t>(k) * nb + NV + c]) -
+                                     eval_k[static_cast<size_t>(k) * nb + v];
+                    const double2 x = X[row + n * N];
+                    Y[row + n * N] = mk(e * x.x - static_cast<double>(acc[q][c][v].x),
+                                        e * x.y - static_cast<double>(acc[q][c][v].y));
+                }
         }
     }
-    for (int c = 0; c < nc; ++c)
-        for (int v = 0; v < nv; ++v) {
-            const int64_t row = static_cast<int64_t>(k) * ncv + c * nv + v;
-            const double e = (scissor + eval_kq[static_cast<size_t>(k) * nb + nv + c]) - eval_k[static_cast<size_t>(k) * nb + v];
-            const double2 x = X[row + n * N];
-            Y[row + n * N] = mk(e * x.x - static_cast<double>(acc[c][v].x), e * x.y - static_cast<double>(acc[c][v].y));
-        }
+}
+
+// Launch the contraction kernels for the context's band counts (nc, nv in 1..4).
+template <typename R, typename Fn>
+void with_bands(const Context& c, Fn&& fn) {
+#define XATU_CONV_BANDS(NC, NV)                          \
+    if (c.nc == NC && c.nv == NV) {                       \
+        fn(std::integral_constant<int, NC>{}, std::integral_constant<int, NV>{}); \
+        return;                                          \
+    }
+    XATU_CONV_BANDS(1, 1) XATU_CONV_BANDS(1, 2) XATU_CONV_BANDS(1, 3) XATU_CONV_BANDS(1, 4)
+    XATU_CONV_BANDS(2, 1) XATU_CONV_BANDS(2, 2) XATU_CONV_BANDS(2, 3) XATU_CONV_BANDS(2, 4)
+    XATU_CONV_BANDS(3, 1) XATU_CONV_BANDS(3, 2) XATU_CONV_BANDS(3, 3) XATU_CONV_BANDS(3, 4)
+    XATU_CONV_BANDS(4, 1) XATU_CONV_BANDS(4, 2) XATU_CONV_BANDS(4, 3) XATU_CONV_BANDS(4, 4)
+#undef XATU_CONV_BANDS
+    throw Error(XATU_ERR_INVALID_ARGUMENT, "convolution products support 1 <= nc, nv <= 4");
 }
 
 // ---- exchange term along the upper triangle (FP64) ----
@@ -336,8 +441,8 @@ struct ConvState {
     DevBuf<int> d_gk;
     DevBuf<double2> P64;
     DevBuf<float2> P32;
-    DevBuf<double2> G64, U64;
-    DevBuf<float2> G32, U32;
+    DevBuf<double2> G64, U64, Ct64, Vt64;
+    DevBuf<float2> G32, U32, Ct32, Vt32;
     DevBuf<double2> part, total;
     struct Plan {
         rocfft_plan fwd = nullptr, bwd = nullptr;
@@ -415,6 +520,17 @@ void ensure_conv(Context& c) {
     }
     cs.d_gk.alloc(c.nk);
     XATU_HIP(hipMemcpyAsync(cs.d_gk.p, gk.data(), gk.size() * sizeof(int), hipMemcpyHostToDevice, c.stream));
+    const int norb = c.lat.norb;
+    cs.Ct64.alloc(static_cast<size_t>(c.nc) * norb * c.nk);
+    cs.Vt64.alloc(static_cast<size_t>(c.nv) * norb * c.nk);
+    cs.Ct32.alloc(cs.Ct64.n);
+    cs.Vt32.alloc(cs.Vt64.n);
+    const dim3 tg((c.nk + 255) / 256, (c.nc + c.nv) * norb);
+    hipLaunchKernelGGL((transpose_stacks_kernel<double>), tg, dim3(256), 0, c.stream, c.nk, c.nv, c.nc, norb,
+                       c.d_evec_k.p, c.d_evec_kq.p, cs.Ct64.p, cs.Vt64.p);
+    hipLaunchKernelGGL((transpose_stacks_kernel<float>), tg, dim3(256), 0, c.stream, c.nk, c.nv, c.nc, norb,
+                       c.d_evec_k.p, c.d_evec_kq.p, cs.Ct32.p, cs.Vt32.p);
+    XATU_HIP(hipGetLastError());
     const int na2 = c.lat.natoms * c.lat.natoms;
     DevBuf<double2> Mt;
     Mt.alloc(static_cast<size_t>(na2) * M);
@@ -437,12 +553,11 @@ void ensure_conv(Context& c) {
 
 template <typename R>
 void conv_product(Context& c, const double2* X, double2* Y, int nvec, hipStream_t s, DevBuf<cx_t<R>>& G,
-                  DevBuf<cx_t<R>>& U, const cx_t<R>* P) {
+                  DevBuf<cx_t<R>>& U, const cx_t<R>* P, const cx_t<R>* Ct, const cx_t<R>* Vt) {
     ConvState& cs = *c.conv;
     const int norb = c.lat.norb;
     const int64_t M = cs.M;
     const int64_t N = c.N;
-    if (c.nc > 4 || c.nv > 4) throw Error(XATU_ERR_INVALID_ARGUMENT, "convolution products need nc, nv <= 4");
     const bool full = (static_cast<int64_t>(c.nk) == M);
     for (int v0 = 0; v0 < nvec; v0 += kBlockVectors) {
         const int nv = std::min(kBlockVectors, nvec - v0);
@@ -453,10 +568,14 @@ void conv_product(Context& c, const double2* X, double2* Y, int nvec, hipStream_
         U.alloc(std::max<size_t>(U.n, static_cast<size_t>(nv) * c.nc * norb * c.nk));
         if (!full) XATU_HIP(hipMemsetAsync(G.p, 0, static_cast<size_t>(batch) * M * sizeof(cx_t<R>), s));
         const dim3 kb((c.nk + 255) / 256);
-        hipLaunchKernelGGL((conv_u_kernel<R>), dim3(kb.x, nv), dim3(256), 0, s, c.nk, c.nv, c.nc, norb, c.d_evec_k.p,
-                           Xb, N, U.p);
-        hipLaunchKernelGGL((conv_z_kernel<R>), dim3(kb.x, norb, nv), dim3(256), 0, s, c.nk, c.nv, c.nc, norb, nv, M,
-                           cs.d_gk.p, c.d_evec_kq.p, U.p, G.p);
+        const int vg = (nv + conv_vec_group<R>() - 1) / conv_vec_group<R>();
+        with_bands<R>(c, [&](auto ncc, auto nvc) {
+            constexpr int NC = decltype(ncc)::value, NV = decltype(nvc)::value;
+            hipLaunchKernelGGL((conv_u_kernel<R, NC, NV>), dim3(kb.x, vg), dim3(256), 0, s, c.nk, norb, nv, Vt, Xb, N,
+                               U.p);
+            hipLaunchKernelGGL((conv_z_kernel<R, NC>), dim3(kb.x, (norb + kConvOrbGroup - 1) / kConvOrbGroup, nv),
+                               dim3(256), 0, s, c.nk, norb, nv, M, cs.d_gk.p, Ct, U.p, G.p);
+        });
         XATU_HIP(hipGetLastError());
         ConvState::Plan& p = get_plan<R>(c, cs, batch);
         void* buf[1] = {G.p};
@@ -465,9 +584,11 @@ void conv_product(Context& c, const double2* X, double2* Y, int nvec, hipStream_
                            G.p);
         XATU_HIP(hipGetLastError());
         XATU_FFT(rocfft_execute(p.bwd, buf, nullptr, p.info));
-        hipLaunchKernelGGL((conv_y_kernel<R>), dim3(kb.x, nv), dim3(256), 0, s, c.nk, c.nv, c.nc, norb, nv, M,
-                           cs.d_gk.p, c.d_evec_k.p, c.d_evec_kq.p, c.d_eval_k.p, c.d_eval_kq.p, c.scissor, G.p, Xb, N,
-                           Yb);
+        with_bands<R>(c, [&](auto ncc, auto nvc) {
+            constexpr int NC = decltype(ncc)::value, NV = decltype(nvc)::value;
+            hipLaunchKernelGGL((conv_y_kernel<R, NC, NV>), dim3(kb.x, vg), dim3(256), 0, s, c.nk, norb, nv, M,
+                               cs.d_gk.p, Ct, Vt, c.d_eval_k.p, c.d_eval_kq.p, c.scissor, G.p, Xb, N, Yb);
+        });
         XATU_HIP(hipGetLastError());
         if (c.inter.exchange) {
             const int na = c.lat.natoms;
@@ -509,10 +630,14 @@ void conv_pair_density(Context& c, const double2* psi, int ns, double2* S) {
     cs.U64.alloc(std::max<size_t>(cs.U64.n, static_cast<size_t>(ns) * c.nc * norb * c.nk));
     if (static_cast<int64_t>(c.nk) != M) XATU_HIP(hipMemsetAsync(cs.G64.p, 0, static_cast<size_t>(batch) * M * sizeof(double2), s));
     const dim3 kb((c.nk + 255) / 256);
-    hipLaunchKernelGGL((conv_u_kernel<double>), dim3(kb.x, ns), dim3(256), 0, s, c.nk, c.nv, c.nc, norb, c.d_evec_k.p,
-                       psi, N, cs.U64.p);
-    hipLaunchKernelGGL((conv_z_kernel<double>), dim3(kb.x, norb, ns), dim3(256), 0, s, c.nk, c.nv, c.nc, norb, ns, M,
-                       cs.d_gk.p, c.d_evec_kq.p, cs.U64.p, cs.G64.p);
+    with_bands<double>(c, [&](auto ncc, auto nvc) {
+        constexpr int NC = decltype(ncc)::value, NV = decltype(nvc)::value;
+        constexpr int VG = conv_vec_group<double>();
+        hipLaunchKernelGGL((conv_u_kernel<double, NC, NV>), dim3(kb.x, (ns + VG - 1) / VG), dim3(256), 0, s, c.nk, norb,
+                           ns, cs.Vt64.p, psi, N, cs.U64.p);
+        hipLaunchKernelGGL((conv_z_kernel<double, NC>), dim3(kb.x, (norb + kConvOrbGroup - 1) / kConvOrbGroup, ns),
+                           dim3(256), 0, s, c.nk, norb, ns, M, cs.d_gk.p, cs.Ct64.p, cs.U64.p, cs.G64.p);
+    });
     XATU_HIP(hipGetLastError());
     ConvState::Plan& pf = get_plan<double>(c, cs, batch);
     void* buf[1] = {cs.G64.p};
@@ -531,12 +656,12 @@ void conv_pair_density(Context& c, const double2* psi, int ns, double2* S) {
 
 void launch_conv64(Context& c, const double2* X, double2* Y, int nvec, hipStream_t s) {
     ensure_conv(c);
-    conv_product<double>(c, X, Y, nvec, s, c.conv->G64, c.conv->U64, c.conv->P64.p);
+    conv_product<double>(c, X, Y, nvec, s, c.conv->G64, c.conv->U64, c.conv->P64.p, c.conv->Ct64.p, c.conv->Vt64.p);
 }
 
 void launch_conv32(Context& c, const double2* X, double2* Y, int nvec, hipStream_t s) {
     ensure_conv(c);
-    conv_product<float>(c, X, Y, nvec, s, c.conv->G32, c.conv->U32, c.conv->P32.p);
+    conv_product<float>(c, X, Y, nvec, s, c.conv->G32, c.conv->U32, c.conv->P32.p, c.conv->Ct32.p, c.conv->Vt32.p);
 }
 
 }  // namespace xatu_dev

@@ -50,22 +50,49 @@ struct ProductTiling<float> {
 };
 template <>
 struct ProductTiling<double> {
-    static constexpr int TW = 2, U = 2;
+    static constexpr int TW = 2, U = 4;
     static constexpr bool G3M = false;
 };
+
+template <typename R, bool G3M, int TW>
+__device__ inline void product_step(const cx_t<R> (&h)[TW], const cx_t<R> (&x)[2],
+                                    typename Mfma<R>::acc_t (&acc)[TW][2][G3M ? 3 : 2]) {
+    using M = Mfma<R>;
+#pragma unroll
+    for (int t = 0; t < TW; ++t)
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const cx_t<R> hv = h[t], xv = x[u];
+            if (G3M) {
+                acc[t][u][0] = M::mma(hv.x + hv.y, xv.x, acc[t][u][0]);
+                acc[t][u][1] = M::mma(hv.x, xv.y - xv.x, acc[t][u][1]);
+                acc[t][u][2] = M::mma(hv.y, xv.x + xv.y, acc[t][u][2]);
+            } else {
+                acc[t][u][0] = M::mma(hv.x, xv.x, acc[t][u][0]);
+                acc[t][u][0] = M::mma(hv.y, -xv.y, acc[t][u][0]);
+                acc[t][u][1] = M::mma(hv.x, xv.y, acc[t][u][1]);
+                acc[t][u][1] = M::mma(hv.y, xv.x, acc[t][u][1]);
+            }
+        }
+}
 
 // 1-D grid of row_blocks * splits workgroups, block 256. Workgroup id -> (split = id %
 // splits, row block = id / splits): with splits a multiple of 8 every split, and with
 // it its slice of Xt, lives on one XCD and its L2. Partial sums to Yp[split][row][32].
+// The main loop has no conditional loads: rows past N are clamped to row N-1 once (they
+// only feed outputs that are never stored) and whole 4U-column groups are loaded
+// unconditionally, so the compiler can issue all loads of a group together; only the
+// column tail of a split is masked.
 template <typename R>
-__global__ __launch_bounds__(256) void dense_product_kernel(const cx_t<R>* __restrict__ H, const cx_t<R>* __restrict__ Xt,
+__global__ __launch_bounds__(256, 3) void dense_product_kernel(const cx_t<R>* __restrict__ H, const cx_t<R>* __restrict__ Xt,
                                                             int64_t N, int64_t kchunk, int splits,
                                                             cx_t<R>* __restrict__ Yp) {
     using M = Mfma<R>;
     using acc_t = typename M::acc_t;
     using T = ProductTiling<R>;
     constexpr int TW = T::TW, U = T::U;
-    constexpr int NACC = T::G3M ? 3 : 2;
+    constexpr bool G3 = T::G3M;
+    constexpr int NACC = G3 ? 3 : 2;
     const int sp = blockIdx.x % splits;
     const int rb = blockIdx.x / splits;
     const int lane = threadIdx.x & 63;
@@ -84,43 +111,44 @@ __global__ __launch_bounds__(256) void dense_product_kernel(const cx_t<R>* __res
 #pragma unroll
             for (int a = 0; a < NACC; ++a) acc[t][u][a] = acc_t{0, 0, 0, 0};
 
-    int64_t rows[TW];
-    bool rok[TW];
+    const cx_t<R>* hrow[TW];
 #pragma unroll
     for (int t = 0; t < TW; ++t) {
-        rows[t] = row0 + 16 * t + li;
-        rok[t] = rows[t] < N;
+        const int64_t r = row0 + 16 * t + li;
+        hrow[t] = H + (r < N ? r : N - 1) + static_cast<int64_t>(lk) * N;
     }
+    const cx_t<R>* xl = Xt + static_cast<int64_t>(lk) * kVec + li;
 
-    for (int64_t j = kbeg; j < kend; j += 4 * U) {
+    const int64_t kmain = kbeg + (kend > kbeg ? (kend - kbeg) / (4 * U) * (4 * U) : 0);
+    int64_t j = kbeg;
+    for (; j < kmain; j += 4 * U) {
         cx_t<R> h[U][TW], x[U][2];
 #pragma unroll
         for (int s = 0; s < U; ++s) {
-            const int64_t col = j + 4 * s + lk;
-            const bool cok = col < kend;
+            const int64_t col = j + 4 * s;
 #pragma unroll
-            for (int t = 0; t < TW; ++t) h[s][t] = (cok && rok[t]) ? H[rows[t] + col * N] : mk(R(0), R(0));
+            for (int t = 0; t < TW; ++t) h[s][t] = hrow[t][col * N];
 #pragma unroll
-            for (int u = 0; u < 2; ++u) x[s][u] = cok ? Xt[col * kVec + 16 * u + li] : mk(R(0), R(0));
+            for (int u = 0; u < 2; ++u) x[s][u] = xl[col * kVec + 16 * u];
         }
 #pragma unroll
-        for (int s = 0; s < U; ++s)
+        for (int s = 0; s < U; ++s) product_step<R, G3, TW>(h[s], x[s], acc);
+    }
+    for (; j < kend; j += 4) {  // masked tail, fewer than 4U columns
+        const bool cok = j + lk < kend;
+        const int64_t col = cok ? j : kend - 1 - lk;  // keeps the lane's address inside the split
+        cx_t<R> h[TW], x[2];
 #pragma unroll
-            for (int t = 0; t < TW; ++t)
+        for (int t = 0; t < TW; ++t) {
+            const cx_t<R> v = hrow[t][col * N];
+            h[t] = cok ? v : mk(R(0), R(0));
+        }
 #pragma unroll
-                for (int u = 0; u < 2; ++u) {
-                    const cx_t<R> hv = h[s][t], xv = x[s][u];
-                    if (T::G3M) {
-                        acc[t][u][0] = M::mma(hv.x + hv.y, xv.x, acc[t][u][0]);
-                        acc[t][u][1] = M::mma(hv.x, xv.y - xv.x, acc[t][u][1]);
-                        acc[t][u][2] = M::mma(hv.y, xv.x + xv.y, acc[t][u][2]);
-                    } else {
-                        acc[t][u][0] = M::mma(hv.x, xv.x, acc[t][u][0]);
-                        acc[t][u][0] = M::mma(hv.y, -xv.y, acc[t][u][0]);
-                        acc[t][u][1] = M::mma(hv.x, xv.y, acc[t][u][1]);
-                        acc[t][u][1] = M::mma(hv.y, xv.x, acc[t][u][1]);
-                    }
-                }
+        for (int u = 0; u < 2; ++u) {
+            const cx_t<R> v = xl[col * kVec + 16 * u];
+            x[u] = cok ? v : mk(R(0), R(0));
+        }
+        product_step<R, G3, TW>(h, x, acc);
     }
     cx_t<R>* out = Yp + static_cast<int64_t>(sp) * N * kVec;
 #pragma unroll
@@ -132,7 +160,7 @@ __global__ __launch_bounds__(256) void dense_product_kernel(const cx_t<R>* __res
 #pragma unroll
                 for (int u = 0; u < 2; ++u) {
                     R re, im;
-                    if (T::G3M) {
+                    if (G3) {
                         re = acc[t][u][0][v] - acc[t][u][2][v];
                         im = acc[t][u][0][v] + acc[t][u][1][v];
                     } else {
