@@ -147,6 +147,7 @@ def main():
     ap.add_argument("--config", default="c4", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the FP64 plain path")
+    ap.add_argument("--max-basis", type=int, default=0, help="Davidson basis size before restart (0: library default)")
     args = ap.parse_args()
     # exactly one JSON line on stdout: everything else (including what the native library
     # prints) goes to stderr
@@ -191,7 +192,7 @@ def main():
         ctx.build(prec if not mf else capi.XATU_MATRIX_FREE)
         t_build = ctx.timings()
         w, v, rep = ctx.solve(method=capi.XATU_SOLVE_DAVIDSON, nstates=nstates, precision=prec,
-                              matrix_free=mf, block=block, tol=TOL, want_vectors=False)
+                              matrix_free=mf, block=block, max_basis=args.max_basis, tol=TOL, want_vectors=False)
         return w, rep, t_build
 
     results = {}

@@ -71,8 +71,9 @@ def analytic(G, cells, r, a, b):
     return d_re, d_im
 
 
-def test_band_sensitivities_are_gauge_invariant(synthetic18):
-    ex = oracle_case(synthetic18, NCELL, 2, 0, PARAMS, Q=Q, exchange=True, cutoff=25)
+@pytest.mark.parametrize("model,nbands", [("synthetic18", 2), ("synthetic36", 3)])
+def test_band_sensitivities_are_gauge_invariant(model, nbands, request):
+    ex = oracle_case(request.getfixturevalue(model), NCELL, nbands, 0, PARAMS, Q=Q, exchange=True, cutoff=25)
     ctx = gpu_context(ex)
     ctx.build()
     w, V = np.linalg.eigh(ex.bse())

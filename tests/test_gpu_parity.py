@@ -28,6 +28,9 @@ CASES = {
                          shift=(0.6628, -1.1480, 0)),
     "mos2_odd": dict(model=MOS2, ncell=7, nbands=2, params=(1., 4., 13.55)),
     "random18": dict(model="synthetic18", ncell=12, nbands=2, params=(1, 1, 40), cutoff=25, exchange=True),
+    # the shape of the 120 x 120 workload (4 atoms x 9 orbitals, nv = nc = 3) on a small mesh
+    "random36": dict(model="synthetic36", ncell=6, nbands=3, params=(1, 1, 40), cutoff=25, exchange=True,
+                     Q=(0.0, 0.03, 0.0)),
 }
 
 _cache = {}
@@ -36,8 +39,8 @@ _cache = {}
 def get_case(name, request):
     if name not in _cache:
         kw = dict(CASES[name])
-        if kw["model"] == "synthetic18":
-            kw["model"] = request.getfixturevalue("synthetic18")
+        if kw["model"] in ("synthetic18", "synthetic36"):
+            kw["model"] = request.getfixturevalue(kw["model"])
         print(f"[case {name}] oracle build", flush=True)
         ex = oracle_case(**kw)
         H = ex.bse()
@@ -109,7 +112,7 @@ def test_davidson_fp64(name, nstates, request):
     ctx.close()
 
 
-@pytest.mark.parametrize("name,nstates", [("hbn30_cut20_x", 8), ("mos2_q_x", 4), ("random18", 16)])
+@pytest.mark.parametrize("name,nstates", [("hbn30_cut20_x", 8), ("mos2_q_x", 4), ("random18", 16), ("random36", 8)])
 def test_fp32_build_and_mixed_solve(name, nstates, request):
     ex, H = get_case(name, request)
     ctx = gpu_context(ex)
@@ -147,7 +150,7 @@ MF_FORMS = [1, 2]  # XATU_MATRIX_FREE_CONV (k-space convolution), XATU_MATRIX_FR
 
 
 @pytest.mark.parametrize("form", MF_FORMS)
-@pytest.mark.parametrize("name", ["hbn20", "mos2_q_x", "mos2_reduced", "mos2_odd", "random18"])
+@pytest.mark.parametrize("name", ["hbn20", "mos2_q_x", "mos2_reduced", "mos2_odd", "random18", "random36"])
 def test_matrix_free_products(name, form, request):
     print(f"[matrix-free products] {name} form {form}", flush=True)
     ex, H = get_case(name, request)
