@@ -45,6 +45,10 @@ typedef struct xatu_ctx xatu_ctx;
 #define XATU_MATRIX_FREE_CONV 1
 #define XATU_MATRIX_FREE_TILES 2
 
+/* Interaction modes (ExcitonTB::setMode) */
+#define XATU_MODE_REALSPACE 0
+#define XATU_MODE_RECIPROCAL 1
+
 /* Interaction potentials (src/ExcitonTB.cpp:347-399) */
 #define XATU_POTENTIAL_KELDYSH 0
 #define XATU_POTENTIAL_COULOMB 1
@@ -72,6 +76,10 @@ typedef struct xatu_interaction {
     int exchange_potential; /* XATU_POTENTIAL_* for the exchange term */
     int exchange;           /* non-zero: also compute V(Q) for the exchange term */
     double Q[3];            /* centre-of-mass momentum (1/Angstrom, Cartesian) */
+    int mode;               /* XATU_MODE_REALSPACE (lattice sums) or XATU_MODE_RECIPROCAL (sums over G,
+                               ExcitonTB.cpp:408-426, 562-621) */
+    int n_reciprocal;       /* reciprocal mode: G vectors from the centred n_reciprocal^ndim combinations
+                               with |G| < cutoff*|b1| + 1e-5 (Lattice.cpp:349-370) */
 } xatu_interaction;
 
 typedef struct xatu_solve_options {
@@ -81,7 +89,8 @@ typedef struct xatu_solve_options {
     int matrix_free; /* 0: resident matrix. XATU_MATRIX_FREE_CONV (1): matrix-free, the direct term as a
                         k-space convolution (FFT) and the exchange term along the upper triangle;
                         XATU_MATRIX_FREE_TILES (2): matrix-free, tiles of H recomputed inside each
-                        product on the matrix cores */
+                        product on the matrix cores. The reciprocal-space mode has no periodic
+                        table to convolve with: its matrix-free products always use the tiles */
     int nstates;     /* number of lowest eigenpairs wanted */
     int block;       /* Davidson block size; 0 selects max(2*nstates, 8) */
     int max_basis;   /* Davidson basis size before restart; 0 selects a default */
@@ -169,7 +178,8 @@ int xatu_bse_matvec(xatu_ctx* ctx, int precision, int matrix_free, int nvec, con
 
 /* Hellmann-Feynman gradients dE_n/dtheta = psi_n^H (dH/dtheta) psi_n for the screening
  * parameters (XATU_PARAM_*), without forming dH. eigvec: N x nstates complex128 (null: the
- * eigenvectors of the last solve). grad_out: nparams x nstates, state index fastest. */
+ * eigenvectors of the last solve). grad_out: nparams x nstates, state index fastest.
+ * Real-space mode only. */
 int xatu_hf_gradients(xatu_ctx* ctx, int nstates, const double* eigvec, int nparams, const int* params,
                       double* grad_out);
 
@@ -198,6 +208,7 @@ int xatu_oscillator_strengths(xatu_ctx* ctx, int nstates, const double* eigvec, 
 
 /* Read-backs for parity checks.
  * ft_out: natoms x natoms x nmesh complex128 (ftMotifStack); ftq_out: natoms x natoms (ftMotifQ).
+ * Real-space mode only (the reciprocal mode has no lattice-sum stack).
  * cells: number of lattice vectors nR and (if coefs non-null) nR x ndim integer coefficients. */
 int xatu_get_fourier(xatu_ctx* ctx, double* ft_out, double* ftq_out);
 int xatu_get_cells(xatu_ctx* ctx, int* n_cells, int* coefs);

@@ -509,3 +509,82 @@ def exciton_spectrum(vme_ex, e_ex, nk, vcell_A2, w0=0.0, wrange=6.0, nw=2000, et
     L = (1 / np.pi) * np.imag(1.0 / (wp[:, None] - e[None, :] - 1j * et))   # [w, n]
     sig = np.einsum("wn,jkn->wjk", L, np.real(sk[:2, :2]))
     return wp * HARTREE, sig
+
+
+# --------------------------------------------------------------------------- reciprocal-space mode
+
+EC = 1.6021766E-19    # include/xatu/ExcitonTB.hpp:10-14
+EPS0 = 8.8541878E-12
+
+
+def truncate_reciprocal_supercell(system, ncell, radius):
+    """Lattice::truncateReciprocalSupercell (src/Lattice.cpp:349-370): centred combinations
+    (first coefficient fastest) of the reciprocal vectors with |G| < radius + 1e-5."""
+    combos = system.combinations(ncell, centered=True)
+    G = combos @ system.recip
+    return G[np.linalg.norm(G, axis=1) < radius + 1e-5]
+
+
+def unit_cell_area(system):
+    """Lattice::computeUnitCellArea (src/Lattice.cpp:119-131)."""
+    b = system.bravais
+    if system.ndim == 1:
+        return float(np.linalg.norm(b[0]))
+    c = np.cross(b[0], b[1])
+    return float(np.linalg.norm(c)) if system.ndim == 2 else float(np.dot(c, b[2]))
+
+
+def keldysh_ft(ex, q):
+    """ExcitonTB::keldyshFT (src/ExcitonTB.cpp:408-426), vectorised over the rows of q."""
+    s = ex.sys
+    b1 = float(np.linalg.norm(s.recip[0]))
+    qn = np.linalg.norm(np.atleast_2d(q), axis=1)
+    eps_bar = (ex.eps_m + ex.eps_s) / 2
+    with np.errstate(divide="ignore"):
+        pot = np.where(qn < b1 / ex.total_cells, 0.0, 1.0 / (qn * (1 + ex.r0 * qn)))
+    return pot * EC * 1E10 / (2 * EPS0 * eps_bar * unit_cell_area(s) * ex.total_cells)
+
+
+def bse_reciprocal(ex, n_reciprocal):
+    """BShamiltonian in reciprocal-space mode (src/ExcitonTB.cpp:562-621, 895-916):
+    D = sum_G Ic conj(Iv) W(k - k' + G) with the Bloch coherence factors
+    Ic = sum_al conj(C_kc,al) C_k'c',al e^{i(k-k'+G).tau_al} (conduction, k+Q stack) and Iv
+    likewise for the valence pair; X = sum_G Ic' conj(Iv') W(Q + G) with
+    Ic' = sum_al conj(C_kc,al) V_kv,al e^{i(Q+G).tau_al}, Iv' the same at k'. Upper triangle,
+    diagonal rule and conjugate mirror as in realspace."""
+    s = ex.sys
+    G = truncate_reciprocal_supercell(s, n_reciprocal, ex.cutoff * float(np.linalg.norm(s.recip[0])))
+    tau = np.repeat(s.motif[:, :3], s.orb_per_atom, axis=0)            # norb x 3
+    kp = s.kpoints
+    nv, nc, nk = ex.nv, ex.nc, ex.nk
+    N = ex.N
+    ki = np.repeat(np.arange(nk), nc * nv)
+    ci = np.tile(np.repeat(np.arange(nc), nv), nk)
+    vi = np.tile(np.arange(nv), nk * nc)
+    C = ex.eigvec_kq[:, nv:, :]                                          # norb x nc x nk
+    V = ex.eigvec_k[:, :nv, :]
+    Cr = C[:, ci, ki]                                                    # norb x N (conduction of row)
+    Vr = V[:, vi, ki]
+    H = np.zeros((N, N), complex)
+    for i in range(N):
+        j = np.arange(i, N)
+        q = kp[ki[i]] - kp[ki[j]]                                        # (k - k') per column
+        D = np.zeros(len(j), complex)
+        X = np.zeros(len(j), complex)
+        for g in G:
+            ph = np.exp(1j * ((q + g) @ tau.T))                          # ncol x norb
+            Ic = ((Cr[:, i].conj()[:, None] * Cr[:, j]).T * ph).sum(1)
+            Iv = ((Vr[:, i].conj()[:, None] * Vr[:, j]).T * ph).sum(1)
+            D += Ic * np.conj(Iv) * keldysh_ft(ex, q + g)
+            if ex.exchange:
+                phq = np.exp(1j * ((ex.Q + g) @ tau.T))                 # norb
+                Icx = np.sum(Cr[:, i].conj() * Vr[:, i] * phq)
+                Ivx = ((Cr[:, j].conj() * Vr[:, j]).T * phq).sum(1)
+                X += Icx * np.conj(Ivx) * keldysh_ft(ex, ex.Q + g)[0]
+        F = D - X
+        e = (ex.scissor + ex.eigval_kq[nv + ci[i], ki[i]]) - ex.eigval_k[vi[i], ki[i]]
+        H[i, j] = -F
+        h = e / 2 - F[0] / 2
+        H[i, i] = h + h
+    H = np.triu(H, 1) + np.triu(H, 1).conj().T + np.diag(np.real(np.diag(H)))
+    return H

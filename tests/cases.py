@@ -40,7 +40,9 @@ def k_offsets(ex):
     return np.rint(coords * n).astype(np.int32)
 
 
-def gpu_context(ex, device=0):
+def gpu_context(ex, device=0, n_reciprocal=None):
+    """Context with the oracle's lattice, band stacks and interaction; n_reciprocal selects
+    the reciprocal-space mode with that many reciprocal vectors per axis."""
     from xatu_amd import capi
     s = ex.sys
     ctx = capi.Context(device)
@@ -49,7 +51,9 @@ def gpu_context(ex, device=0):
     pot = {"keldysh": capi.XATU_POTENTIAL_KELDYSH, "coulomb": capi.XATU_POTENTIAL_COULOMB}
     ctx.fourier_sums(ex.eps_m, ex.eps_s, ex.r0, ex.cutoff, ex.ncell, ex.total_cells,
                      regularization=ex.regularization, potential=pot[ex.potential],
-                     exchange_potential=pot[ex.exchange_potential], exchange=ex.exchange, Q=tuple(ex.Q))
+                     exchange_potential=pot[ex.exchange_potential], exchange=ex.exchange, Q=tuple(ex.Q),
+                     mode=0 if n_reciprocal is None else capi.XATU_MODE_RECIPROCAL,
+                     n_reciprocal=1 if n_reciprocal is None else n_reciprocal)
     return ctx
 
 

@@ -315,3 +315,56 @@ def test_exciton_spectrum_is_a_sum_of_unit_lorentzians():
     area = np.sum(sig[:, 0, 0]) * (w[1] - w[0]) / O.HARTREE
     assert abs(area - amp) < 0.02 * amp  # tails beyond the window
     assert np.allclose(sig[:, 0, 1], sig[:, 1, 0])
+
+
+# --------------------------------------------------------------- reciprocal-space mode
+
+def test_hbn_energies_reciprocal_mode():
+    # test/tests.cpp:246-279 (setMode("reciprocalspace"), setReciprocalVectors(5))
+    ex = oracle_case(HBN, 20, 1, 0, (1, 1, 10))
+    check_levels(levels(O.bse_reciprocal(ex, 5), 3), [(6.234291, 1), (6.236636, 1), (6.731819, 1)], 1e-4)
+
+
+def test_reciprocal_supercell_is_centred_and_truncated():
+    s = O.System(str(HBN))
+    b1 = float(np.linalg.norm(s.recip[0]))
+    G = O.truncate_reciprocal_supercell(s, 5, 100 * b1)
+    assert len(G) == 25 and np.allclose(G[0], -2 * s.recip[0] - 2 * s.recip[1])
+    assert np.allclose(G[1] - G[0], s.recip[0])                       # first coefficient fastest
+    assert len(O.truncate_reciprocal_supercell(s, 4, 100 * b1)) == 16  # even n: offsets -2..1
+    assert len(O.truncate_reciprocal_supercell(s, 3, 1.01 * b1)) == 7  # hexagonal: G = 0 and six |b1| vectors
+    assert len(O.truncate_reciprocal_supercell(s, 3, 0.5 * b1)) == 1
+
+
+def test_keldysh_ft_limits():
+    # 1/(q (1 + r0 q)) above |b1|/T, zero below; the Coulomb limit r0 -> 0 is 1/q
+    ex = oracle_case(HBN, 4, 1, 0, (1, 1, 10))
+    b1 = float(np.linalg.norm(ex.sys.recip[0]))
+    pref = O.EC * 1e10 / (2 * O.EPS0 * 1.0 * O.unit_cell_area(ex.sys) * ex.total_cells)
+    q = np.array([[0.3, 0.4, 0.0]])
+    assert O.keldysh_ft(ex, q)[0] == pytest.approx(pref / (0.5 * (1 + 10 * 0.5)), rel=1e-14)
+    assert O.keldysh_ft(ex, np.array([[0.99 * b1 / ex.total_cells, 0, 0]]))[0] == 0.0
+    ex.r0 = 1e-12
+    assert O.keldysh_ft(ex, q)[0] == pytest.approx(pref / 0.5, rel=1e-10)
+
+
+def test_reciprocal_exchange_is_positive_and_rank_natoms():
+    # X = E M(Q) E^H with M(Q) = sum_G W(Q+G) u_G u_G^H (W >= 0): positive semidefinite,
+    # rank <= natoms, and it raises every energy
+    ex = oracle_case(MOS2, 4, 2, 0, (1., 4., 13.55), Q=(0, 0.1, 0), exchange=True)
+    Hx = O.bse_reciprocal(ex, 3)
+    ex.exchange = False
+    H0 = O.bse_reciprocal(ex, 3)
+    w = np.linalg.eigvalsh(Hx - H0)
+    assert w[0] > -1e-12 * w[-1] and w[-1] > 0
+    assert np.sum(w > 1e-10 * w[-1]) <= ex.sys.natoms
+    assert np.all(np.linalg.eigvalsh(Hx) >= np.linalg.eigvalsh(H0) - 1e-12)
+
+
+def test_reciprocal_without_g_vectors_is_band_differences():
+    # no reciprocal vectors (n = 0): no interaction, H = diag(e_c(k+Q) - e_v(k) + scissor)
+    ex = oracle_case(MOS2, 3, 2, 0, (1., 4., 13.55), scissor=0.3)
+    H = O.bse_reciprocal(ex, 0)
+    e = np.array([ex.scissor + ex.eigval_kq[ex.nv + c, k] - ex.eigval_k[v, k]
+                  for k in range(ex.nk) for c in range(ex.nc) for v in range(ex.nv)])
+    assert np.array_equal(H, np.diag(e))

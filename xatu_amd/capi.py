@@ -24,6 +24,7 @@ XATU_MATRIX_FREE = 2
 XATU_MATRIX_FREE_CONV = 1    # matrix-free product forms (solve / matvec)
 XATU_MATRIX_FREE_TILES = 2
 XATU_POTENTIAL_KELDYSH, XATU_POTENTIAL_COULOMB = 0, 1
+XATU_MODE_REALSPACE, XATU_MODE_RECIPROCAL = 0, 1
 XATU_SOLVE_DIAG, XATU_SOLVE_DAVIDSON = 0, 1
 XATU_PARAM_R0, XATU_PARAM_EPS_M, XATU_PARAM_EPS_S = 0, 1, 2
 
@@ -41,7 +42,8 @@ class Interaction(ctypes.Structure):
     _fields_ = [("eps_m", ctypes.c_double), ("eps_s", ctypes.c_double), ("r0", ctypes.c_double),
                 ("regularization", ctypes.c_double), ("cutoff", ctypes.c_double), ("ncell", ctypes.c_int),
                 ("total_cells", ctypes.c_int), ("potential", ctypes.c_int), ("exchange_potential", ctypes.c_int),
-                ("exchange", ctypes.c_int), ("Q", ctypes.c_double * 3)]
+                ("exchange", ctypes.c_int), ("Q", ctypes.c_double * 3), ("mode", ctypes.c_int),
+                ("n_reciprocal", ctypes.c_int)]
 
 
 class SolveOptions(ctypes.Structure):
@@ -166,9 +168,10 @@ class Context:
 
     def fourier_sums(self, eps_m, eps_s, r0, cutoff, ncell, total_cells, regularization=0.0,
                      potential=XATU_POTENTIAL_KELDYSH, exchange_potential=XATU_POTENTIAL_KELDYSH,
-                     exchange=False, Q=(0.0, 0.0, 0.0)):
+                     exchange=False, Q=(0.0, 0.0, 0.0), mode=0, n_reciprocal=1):
         it = Interaction(eps_m, eps_s, r0, regularization, cutoff, ncell, total_cells, potential,
-                         exchange_potential, int(bool(exchange)), (ctypes.c_double * 3)(*Q))
+                         exchange_potential, int(bool(exchange)), (ctypes.c_double * 3)(*Q), int(mode),
+                         int(n_reciprocal))
         self._check(self.L.xatu_fourier_sums(self._h, ctypes.byref(it)))
 
     def build(self, precision=XATU_FP64, scissor=0.0, copy_out=False):

@@ -108,7 +108,7 @@ __device__ inline void pair_block(const LatticeDesc& lat, const cx_t<R>* ca_, co
     using C = cx_t<R>;
     constexpr int NB = NC + NV;
     constexpr int NCV = NC * NV;
-    const int q = mesh_index(kcoord, ka, kb, lat.ndim, lat.mesh_n);
+    const int q = table_index(lat, kcoord, ka, kb);
     C M[NA][NA];
 #pragma unroll
     for (int a = 0; a < NA; ++a)
@@ -462,7 +462,7 @@ __global__ void bse_basis_kernel(LatticeDesc lat, int nb, int nbasis, const int*
     const double2* vK = evk + (static_cast<size_t>(k) * nb + v) * norb;      // coefsK
     const double2* cK2 = evkq + (static_cast<size_t>(k2) * nb + c2) * norb;  // coefsK2Q
     const double2* vK2 = evk + (static_cast<size_t>(k2) * nb + v2) * norb;   // coefsK2
-    const int q = mesh_index(kcoord, k, k2, lat.ndim, lat.mesh_n);
+    const int q = table_index(lat, kcoord, k, k2);
     const int na = lat.natoms;
     double2 r1[kMaxAtoms], r2[kMaxAtoms], x1[kMaxAtoms], x2[kMaxAtoms];
 #pragma unroll

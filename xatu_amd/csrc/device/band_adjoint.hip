@@ -84,7 +84,7 @@ __global__ __launch_bounds__(TK* TK) void band_adjoint_kernel(LatticeDesc lat, i
     const double2* cb = s_coef + static_cast<size_t>(TK + tc) * norb * NB;
     // A_{c,al} = ca[al NB + c], Q_{v,al} = ca[al NB + NC + v], B_{c,al} = cb[al NB + c], P_{v,al} = cb[al NB + NC + v]
 
-    const int q = mesh_index(kcoord, ka, kb, lat.ndim, lat.mesh_n);
+    const int q = table_index(lat, kcoord, ka, kb);
     double2 M[NA][NA];
 #pragma unroll
     for (int a = 0; a < NA; ++a)

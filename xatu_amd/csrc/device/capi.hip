@@ -37,32 +37,83 @@ int guarded(xatu_ctx* ctx, F&& f) {
 }
 
 // Centred supercell combinations within a sphere (Lattice::generateCombinations with
-// centered=true and Lattice::truncateSupercell, src/Lattice.cpp:293-341): coefficients
-// run over [0, n) - n/2 with the first one fastest; a cell is kept if |R| < radius + 1e-5.
-std::vector<int> truncate_supercell(const LatticeDesc& lat, int ncell, double radius) {
+// centered=true, Lattice::truncateSupercell and Lattice::truncateReciprocalSupercell,
+// src/Lattice.cpp:293-370): coefficients run over [0, n) - n/2 with the first one
+// fastest; a combination m of the rows of vecs is kept if |sum_j m_j vecs_j| < radius + 1e-5.
+std::vector<int> truncate_supercell(const double* vecs, int ndim, int ncell, double radius) {
     std::vector<int> out;
     const int shift = ncell / 2;
     long long total = 1;
-    for (int j = 0; j < lat.ndim; ++j) total *= ncell;
+    for (int j = 0; j < ndim; ++j) total *= ncell;
     for (long long i = 0; i < total; ++i) {
         int m[3] = {0, 0, 0};
         long long rem = i;
-        for (int j = 0; j < lat.ndim; ++j) {
+        for (int j = 0; j < ndim; ++j) {
             m[j] = static_cast<int>(rem % ncell) - shift;
             rem /= ncell;
         }
         double v[3] = {0.0, 0.0, 0.0};
-        for (int j = 0; j < lat.ndim; ++j) {
-            v[0] += m[j] * lat.bravais[3 * j + 0];
-            v[1] += m[j] * lat.bravais[3 * j + 1];
-            v[2] += m[j] * lat.bravais[3 * j + 2];
+        for (int j = 0; j < ndim; ++j) {
+            v[0] += m[j] * vecs[3 * j + 0];
+            v[1] += m[j] * vecs[3 * j + 1];
+            v[2] += m[j] * vecs[3 * j + 2];
         }
         const double d = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
         if (d < radius + 1E-5) {
-            for (int j = 0; j < lat.ndim; ++j) out.push_back(m[j]);
+            for (int j = 0; j < ndim; ++j) out.push_back(m[j]);
         }
     }
     return out;
+}
+
+// Reciprocal lattice vectors (rows): b_i . a_j = 2 pi delta_ij within the first ndim
+// Cartesian components (1D: b = 2 pi a / |a|^2), as Lattice::calculateReciprocalLattice
+// (src/Lattice.cpp:86-113) for lattices in the x(y) plane.
+void reciprocal_lattice(LatticeDesc& L) {
+    for (double& x : L.recip_vec) x = 0.0;
+    const double* a = L.bravais;
+    const double two_pi = 2.0 * M_PI;
+    if (L.ndim == 1) {
+        const double n2 = a[0] * a[0] + a[1] * a[1] + a[2] * a[2];
+        for (int x = 0; x < 3; ++x) L.recip_vec[x] = two_pi * a[x] / n2;
+        return;
+    }
+    const int n = L.ndim;
+    double A[3][3] = {}, inv[3][3] = {};
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) A[i][j] = a[3 * i + j];
+    if (n == 2) {
+        const double det = A[0][0] * A[1][1] - A[0][1] * A[1][0];
+        if (std::fabs(det) < 1e-300) throw Error(XATU_ERR_INVALID_ARGUMENT, "singular Bravais basis");
+        inv[0][0] = A[1][1] / det;
+        inv[0][1] = -A[0][1] / det;
+        inv[1][0] = -A[1][0] / det;
+        inv[1][1] = A[0][0] / det;
+    } else {
+        const double det = A[0][0] * (A[1][1] * A[2][2] - A[1][2] * A[2][1]) -
+                           A[0][1] * (A[1][0] * A[2][2] - A[1][2] * A[2][0]) +
+                           A[0][2] * (A[1][0] * A[2][1] - A[1][1] * A[2][0]);
+        if (std::fabs(det) < 1e-300) throw Error(XATU_ERR_INVALID_ARGUMENT, "singular Bravais basis");
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                const int i1 = (j + 1) % 3, i2 = (j + 2) % 3, j1 = (i + 1) % 3, j2 = (i + 2) % 3;
+                inv[i][j] = (A[i1][j1] * A[i2][j2] - A[i1][j2] * A[i2][j1]) / det;
+            }
+    }
+    // A b_i = 2 pi e_i: b_i is column i of 2 pi A^-1
+    for (int i = 0; i < n; ++i)
+        for (int x = 0; x < n; ++x) L.recip_vec[3 * i + x] = two_pi * inv[x][i];
+}
+
+// Unit cell length / area / volume (Lattice::computeUnitCellArea, src/Lattice.cpp:119-131).
+double unit_cell_measure(const LatticeDesc& L) {
+    const double* a = L.bravais;
+    if (L.ndim == 1) return std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+    const double cx = a[1] * a[5] - a[2] * a[4];
+    const double cy = a[2] * a[3] - a[0] * a[5];
+    const double cz = a[0] * a[4] - a[1] * a[3];
+    if (L.ndim == 2) return std::sqrt(cx * cx + cy * cy + cz * cz);
+    return cx * a[6] + cy * a[7] + cz * a[8];
 }
 
 PotDesc make_pot(const Context& c, int kind) {
@@ -76,6 +127,49 @@ PotDesc make_pot(const Context& c, int kind) {
     p.reg = (c.inter.regularization < 1E-10) ? c.a1norm : c.inter.regularization;
     p.cutoff = c.a1norm * c.inter.cutoff + 1E-5;
     return p;
+}
+
+// Interaction tables of the reciprocal-space mode: the sums over the reciprocal
+// vectors G of the centred n_reciprocal^ndim combinations with |G| < cutoff |b1| + 1e-5
+// (ExcitonTB::reciprocalInteractionTerm, src/ExcitonTB.cpp:562-583).
+void reciprocal_tables(Context& c) {
+    LatticeDesc& L = c.lat;
+    const xatu_interaction& in = c.inter;
+    if (in.n_reciprocal < 0) {
+        throw Error(XATU_ERR_INVALID_ARGUMENT, "setReciprocalVectors(): given number must be positive");
+    }
+    long long nt = 1;
+    for (int j = 0; j < L.ndim; ++j) nt *= 2LL * L.mesh_n - 1;
+    if (nt > (1LL << 30)) throw Error(XATU_ERR_INVALID_ARGUMENT, "mesh too large for the reciprocal-space mode");
+    L.recip = 1;
+    L.tab_n = 2 * L.mesh_n - 1;
+    L.ntab = static_cast<int>(nt);
+    const double* b1 = L.recip_vec;
+    const double b1n = std::sqrt(b1[0] * b1[0] + b1[1] * b1[1] + b1[2] * b1[2]);
+    const std::vector<int> m = truncate_supercell(L.recip_vec, L.ndim, in.n_reciprocal, in.cutoff * b1n);
+    const int ng = static_cast<int>(m.size() / L.ndim);
+    std::vector<double> g(3 * static_cast<size_t>(std::max(ng, 1)), 0.0);
+    for (int i = 0; i < ng; ++i)
+        for (int j = 0; j < L.ndim; ++j)
+            for (int x = 0; x < 3; ++x) g[3 * i + x] += m[i * L.ndim + j] * L.recip_vec[3 * j + x];
+    DevBuf<double> d_g;
+    d_g.alloc(g.size());
+    XATU_HIP(hipMemcpyAsync(d_g.p, g.data(), g.size() * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    const int na2 = L.natoms * L.natoms;
+    c.d_ft.alloc(static_cast<size_t>(na2) * L.ntab);
+    c.d_ftq.alloc(na2);
+    const double kEc = 1.6021766E-19, kEps0 = 8.8541878E-12;  // include/xatu/ExcitonTB.hpp:10-14
+    const double eps_bar = (in.eps_m + in.eps_s) / 2;
+    const double pref = kEc * 1E10 / (2 * kEps0 * eps_bar * unit_cell_measure(L) * in.total_cells);
+    EventTimer ev(c.stream);
+    ev.start();
+    launch_reciprocal_table(c, d_g.p, ng, pref, c.stream);
+    if (!in.exchange) XATU_HIP(hipMemsetAsync(c.d_ftq.p, 0, na2 * sizeof(double2), c.stream));
+    ev.stop();
+    c.tim.ms_fourier = ev.ms();  // synchronizes the stream before d_g is released
+    c.nR = 0;
+    c.have_ft = true;
+    c.built64 = c.built32 = c.prepared = false;
 }
 
 // Exchange factors and the FP64 diagonal (Davidson preconditioner), needed by
@@ -110,7 +204,9 @@ void apply_operator(Context& c, int precision, int matrix_free, const double2* X
     if (matrix_free) {
         if (!c.prepared) throw Error(XATU_ERR_LOGIC, "operator not prepared (call xatu_build_bse)");
         ev.start();
-        if (matrix_free == XATU_MATRIX_FREE_TILES) {
+        // the convolution form needs a table periodic on the mesh: the reciprocal-space
+        // mode runs its matrix-free products on recomputed tiles
+        if (matrix_free == XATU_MATRIX_FREE_TILES || c.lat.recip) {
             if (precision == XATU_FP64) launch_matfree64(c, X, Y, nvec, c.stream);
             else launch_matfree32(c, X, Y, nvec, c.stream);
         } else {
@@ -202,6 +298,10 @@ int xatu_set_lattice(xatu_ctx* ctx, int ndim, const double* bravais, int natoms,
             L.orb_begin[a + 1] = L.orb_begin[a] + norb_per_atom[a];
         }
         L.norb = L.orb_begin[natoms];
+        reciprocal_lattice(L);
+        L.recip = 0;
+        L.tab_n = mesh_n;
+        L.ntab = L.nmesh;
         c.lat = L;
         c.a1norm = std::sqrt(bravais[0] * bravais[0] + bravais[1] * bravais[1] + bravais[2] * bravais[2]);
         c.have_lattice = true;
@@ -259,10 +359,20 @@ int xatu_fourier_sums(xatu_ctx* ctx, const xatu_interaction* inter) {
         if (inter->r0 == 0) throw Error(XATU_ERR_INVALID_ARGUMENT, "Error: r0 must be non-zero");
         if (inter->ncell < 1) throw Error(XATU_ERR_INVALID_ARGUMENT, "ncell must be a positive number");
         if (inter->total_cells < 1) throw Error(XATU_ERR_INVALID_ARGUMENT, "total_cells must be positive");
+        if (inter->mode != XATU_MODE_REALSPACE && inter->mode != XATU_MODE_RECIPROCAL) {
+            throw Error(XATU_ERR_INVALID_ARGUMENT, "setMode(): mode must be either realspace or reciprocalspace");
+        }
         c.inter = *inter;
         c.pot_d = make_pot(c, inter->potential);
         c.pot_x = inter->exchange ? make_pot(c, inter->exchange_potential) : c.pot_d;
-        c.h_cells = truncate_supercell(c.lat, inter->ncell, c.a1norm * inter->cutoff);
+        if (inter->mode == XATU_MODE_RECIPROCAL) {
+            reciprocal_tables(c);
+            return;
+        }
+        c.lat.recip = 0;
+        c.lat.tab_n = c.lat.mesh_n;
+        c.lat.ntab = c.lat.nmesh;
+        c.h_cells = truncate_supercell(c.lat.bravais, c.lat.ndim, inter->ncell, c.a1norm * inter->cutoff);
         c.nR = static_cast<int>(c.h_cells.size() / c.lat.ndim);
         if (c.nR == 0) throw Error(XATU_ERR_INVALID_ARGUMENT, "no lattice vectors inside the cutoff");
         std::vector<int> padded(static_cast<size_t>(c.nR) * 3, 0);
@@ -442,6 +552,9 @@ int xatu_hf_gradients(xatu_ctx* ctx, int nstates, const double* eigvec, int npar
                       double* grad_out) {
     return guarded(ctx, [&](Context& c) {
         c.require_ft();
+        if (c.lat.recip) {
+            throw Error(XATU_ERR_LOGIC, "interaction-parameter gradients are available in the real-space mode only");
+        }
         if (nstates < 1 || nparams < 1 || !params || !grad_out) throw Error(XATU_ERR_INVALID_ARGUMENT, "bad args");
         for (int i = 0; i < nparams; ++i) {
             if (params[i] < XATU_PARAM_R0 || params[i] > XATU_PARAM_EPS_S) {
@@ -507,6 +620,7 @@ int xatu_oscillator_strengths(xatu_ctx* ctx, int nstates, const double* eigvec, 
 int xatu_get_fourier(xatu_ctx* ctx, double* ft_out, double* ftq_out) {
     return guarded(ctx, [&](Context& c) {
         c.require_ft();
+        if (c.lat.recip) throw Error(XATU_ERR_LOGIC, "the reciprocal-space mode has no lattice Fourier sums");
         const size_t na2 = static_cast<size_t>(c.lat.natoms) * c.lat.natoms;
         if (ft_out) XATU_HIP(hipMemcpy(ft_out, c.d_ft.p, na2 * c.lat.nmesh * sizeof(double2), hipMemcpyDeviceToHost));
         if (ftq_out) XATU_HIP(hipMemcpy(ftq_out, c.d_ftq.p, na2 * sizeof(double2), hipMemcpyDeviceToHost));

@@ -73,6 +73,13 @@ struct LatticeDesc {
     double bravais[9];
     double tau[8 * 3];  // atomic positions (natoms <= 8)
     int orb_begin[9];   // orbital offset of each atom, orb_begin[natoms] = norb
+    // interaction table: real-space mode indexes it by the mesh point of k_b - k_a
+    // (tab_n = mesh_n); reciprocal mode by the raw offset difference k_a - k_b + (mesh_n - 1)
+    // per axis (tab_n = 2 mesh_n - 1), because its sums over G are not periodic in q
+    int recip;
+    int tab_n;
+    int ntab;           // tab_n^ndim
+    double recip_vec[9];  // reciprocal lattice vectors (rows), 1/Angstrom
 };
 
 constexpr int kMaxAtoms = 8;
@@ -188,6 +195,9 @@ void launch_potential_table(const Context& c, const PotDesc& pot, double* out, h
 void launch_twiddles(int mesh_n, double2* out, hipStream_t s);
 void launch_fourier_mesh(const Context& c, hipStream_t s);
 void launch_fourier_q(const Context& c, hipStream_t s);
+// Reciprocal-mode tables: M_ab(q) over the raw mesh differences into d_ft and, with
+// exchange, M_ab(Q) into d_ftq; d_gvec holds ng reciprocal lattice vectors (x, y, z).
+void launch_reciprocal_table(const Context& c, const double* d_gvec, int ng, double pref, hipStream_t s);
 void launch_exchange_factors(const Context& c, hipStream_t s);
 void launch_build_tiles64(const Context& c, double2* H, hipStream_t s);
 void launch_build_tiles32(const Context& c, float2* H, hipStream_t s);

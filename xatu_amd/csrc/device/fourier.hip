@@ -202,7 +202,105 @@ __global__ __launch_bounds__(kFtThreads) void fourier_q_kernel(LatticeDesc lat, 
     }
 }
 
+// Reciprocal-space mode (src/ExcitonTB.cpp:408-426, 562-621). The direct term of a
+// pair of k-points is
+//   D = sum_ab rho^c_a M_ab(k - k') conj(rho^v'_b),
+//   M_ab(q) = sum_G W(|q + G|) exp(i (q + G).(tau_a - tau_b)),
+//   W(q) = 0 for q < |b1| / T, else pref / (q (1 + r0 q))     (keldyshFT),
+// the exchange term uses M_ab(Q). M is not periodic on the mesh (the G set is cut
+// off by |G|), so the table holds every raw difference k - k' of mesh points: offset
+// d_j - (m - 1) per axis, in units of b_j / m. M_ba = conj(M_ab) because W is real.
+struct RecipDesc {
+    double pref;  // e 1e10 / (2 eps0 eps_bar A T)
+    double r0;
+    double qmin;  // |b1| / T
+};
+
+__device__ inline double2 reciprocal_sum(const LatticeDesc& lat, const double* q, int a, int b,
+                                         const double* __restrict__ gvec, int ng, const RecipDesc& rd) {
+    const double dx = lat.tau[3 * a + 0] - lat.tau[3 * b + 0];
+    const double dy = lat.tau[3 * a + 1] - lat.tau[3 * b + 1];
+    const double dz = lat.tau[3 * a + 2] - lat.tau[3 * b + 2];
+    double re = 0.0, im = 0.0;
+    for (int g = 0; g < ng; ++g) {
+        const double x = q[0] + gvec[3 * g + 0];
+        const double y = q[1] + gvec[3 * g + 1];
+        const double z = q[2] + gvec[3 * g + 2];
+        const double qn = sqrt(x * x + y * y + z * z);
+        if (qn < rd.qmin) continue;
+        const double w = rd.pref / (qn * (1.0 + rd.r0 * qn));
+        double sn, cs;
+        sincos(x * dx + y * dy + z * dz, &sn, &cs);
+        re += w * cs;
+        im += w * sn;
+    }
+    return make_double2(re, im);
+}
+
+__device__ inline void pair_of(int p, int natoms, int& a, int& b) {
+    a = 0;
+    int rem = p;
+    while (rem >= natoms - a) {
+        rem -= natoms - a;
+        ++a;
+    }
+    b = a + rem;
+}
+
+// grid: (ceil(ntab / 256), npairs); block: 256
+__global__ void reciprocal_table_kernel(LatticeDesc lat, const double* __restrict__ gvec, int ng, RecipDesc rd,
+                                        int npairs, double2* __restrict__ ft) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int p = blockIdx.y;
+    if (t >= lat.ntab || p >= npairs) return;
+    int a, b;
+    pair_of(p, lat.natoms, a, b);
+    double q[3] = {0.0, 0.0, 0.0};
+    int rem = t;
+    for (int j = 0; j < lat.ndim; ++j) {
+        const double f = static_cast<double>(rem % lat.tab_n - (lat.mesh_n - 1)) / lat.mesh_n;
+        rem /= lat.tab_n;
+        q[0] += f * lat.recip_vec[3 * j + 0];
+        q[1] += f * lat.recip_vec[3 * j + 1];
+        q[2] += f * lat.recip_vec[3 * j + 2];
+    }
+    const double2 m = reciprocal_sum(lat, q, a, b, gvec, ng, rd);
+    const size_t base = static_cast<size_t>(t) * lat.natoms * lat.natoms;
+    ft[base + a + static_cast<size_t>(lat.natoms) * b] = m;
+    if (a != b) ft[base + b + static_cast<size_t>(lat.natoms) * a] = make_double2(m.x, -m.y);
+}
+
+// M_ab(Q) for the exchange term. grid: 1; block: 64 (npairs <= 36)
+__global__ void reciprocal_q_kernel(LatticeDesc lat, const double* __restrict__ gvec, int ng, RecipDesc rd, int npairs,
+                                    double qx, double qy, double qz, double2* __restrict__ ftq) {
+    const int p = threadIdx.x;
+    if (p >= npairs) return;
+    int a, b;
+    pair_of(p, lat.natoms, a, b);
+    const double q[3] = {qx, qy, qz};
+    const double2 m = reciprocal_sum(lat, q, a, b, gvec, ng, rd);
+    ftq[a + lat.natoms * b] = m;
+    if (a != b) ftq[b + lat.natoms * a] = make_double2(m.x, -m.y);
+}
+
 }  // namespace
+
+void launch_reciprocal_table(const Context& c, const double* d_gvec, int ng, double pref, hipStream_t s) {
+    const int npairs = c.lat.natoms * (c.lat.natoms + 1) / 2;
+    RecipDesc rd{};
+    rd.pref = pref;
+    rd.r0 = c.inter.r0;
+    const double* b1 = c.lat.recip_vec;
+    rd.qmin = sqrt(b1[0] * b1[0] + b1[1] * b1[1] + b1[2] * b1[2]) / static_cast<double>(c.inter.total_cells);
+    dim3 grid((c.lat.ntab + 255) / 256, npairs);
+    hipLaunchKernelGGL(reciprocal_table_kernel, grid, dim3(256), 0, s, c.lat, d_gvec, ng, rd, npairs, c.d_ft.p);
+    XATU_HIP(hipGetLastError());
+    if (c.inter.exchange) {
+        hipLaunchKernelGGL(reciprocal_q_kernel, dim3(1), dim3(64), 0, s, c.lat, d_gvec, ng, rd, npairs, c.inter.Q[0],
+                           c.inter.Q[1], c.inter.Q[2], c.d_ftq.p);
+        XATU_HIP(hipGetLastError());
+    }
+}
 
 void launch_potential_table(const Context& c, const PotDesc& pot, double* out, hipStream_t s) {
     const int npairs = c.lat.natoms * (c.lat.natoms + 1) / 2;

@@ -21,4 +21,18 @@ __device__ inline int mesh_index(const int* __restrict__ kcoord, int ka, int kb,
     return q;
 }
 
+// Index of the interaction table entry for the pair (k_a, k_b): the mesh point of
+// k_b - k_a in real-space mode; the raw difference k_a - k_b (offsets shifted by
+// mesh_n - 1 per axis) in reciprocal mode, whose table is not periodic on the mesh.
+template <typename Lat>
+__device__ inline int table_index(const Lat& lat, const int* __restrict__ kcoord, int ka, int kb) {
+    if (!lat.recip) return mesh_index(kcoord, ka, kb, lat.ndim, lat.mesh_n);
+    int q = 0, mul = 1;
+    for (int j = 0; j < lat.ndim; ++j) {
+        q += (kcoord[3 * ka + j] - kcoord[3 * kb + j] + lat.mesh_n - 1) * mul;
+        mul *= lat.tab_n;
+    }
+    return q;
+}
+
 }  // namespace xatu_dev

@@ -264,9 +264,6 @@ void ExcitonTB::ensureContext() {
 void ExcitonTB::initializeHamiltonian() {
     if (bands.empty()) throw std::invalid_argument("Error: Exciton object must have some bands");
     if (system->nk == 0) throw std::invalid_argument("Error: BZ mesh must be initialized first");
-    if (mode != "realspace") {
-        throw std::invalid_argument("initializeHamiltonian(): only mode 'realspace' is implemented by this build");
-    }
     if (gauge != "lattice") {
         throw std::invalid_argument("initializeHamiltonian(): only the 'lattice' gauge is implemented by this build");
     }
@@ -351,6 +348,8 @@ void ExcitonTB::initializeResultsH0() {
     in.exchange_potential = potential_id(exchange ? exchangePotential : potential);
     in.exchange = exchange ? 1 : 0;
     for (int x = 0; x < 3; ++x) in.Q[x] = Q[x];
+    in.mode = (mode == "reciprocalspace") ? XATU_MODE_RECIPROCAL : XATU_MODE_REALSPACE;
+    in.n_reciprocal = nReciprocalVectors;
     check(xatu_fourier_sums(ctx_, &in));
     built_ = built32_ = false;
 }
