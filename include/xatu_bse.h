@@ -97,7 +97,14 @@ typedef struct xatu_solve_options {
     int max_iter;    /* 0 selects a default */
     double tol;      /* FP64 residual 2-norm tolerance in eV (0 selects 1e-5); the FP32 form iterates
                         to max(tol/2, 2e-5) with FP32 products, then refines with FP64 products */
+    int fp32_arith;  /* products with the FP32 matrix: XATU_ARITH_SPLIT16 (0, default) runs them on the
+                        FP16 matrix cores with every FP32 operand split into two FP16 parts (22-bit
+                        operands, FP32 accumulation) when N is even, else on the FP32 matrix cores;
+                        XATU_ARITH_FP32 (1) always uses the FP32 matrix cores */
 } xatu_solve_options;
+
+#define XATU_ARITH_SPLIT16 0
+#define XATU_ARITH_FP32 1
 
 typedef struct xatu_solve_report {
     int iterations;          /* Davidson iterations (both phases) */
@@ -173,7 +180,8 @@ int xatu_solve(xatu_ctx* ctx, const xatu_solve_options* opt, double* eigval_out,
                xatu_solve_report* report);
 
 /* y = H x for nvec vectors (complex128, N x nvec column-major) with the resident matrix of
- * the given precision, or matrix-free (matrix_free: XATU_MATRIX_FREE_CONV or _TILES). */
+ * the given precision, or matrix-free (matrix_free: XATU_MATRIX_FREE_CONV or _TILES).
+ * FP32 products use the default arithmetic (XATU_ARITH_SPLIT16). */
 int xatu_bse_matvec(xatu_ctx* ctx, int precision, int matrix_free, int nvec, const double* x, double* y);
 
 /* Hellmann-Feynman gradients dE_n/dtheta = psi_n^H (dH/dtheta) psi_n for the screening

@@ -112,8 +112,9 @@ def test_davidson_fp64(name, nstates, request):
     ctx.close()
 
 
+@pytest.mark.parametrize("arith", [0, 1])  # XATU_ARITH_SPLIT16 (FP16 matrix cores, split operands), XATU_ARITH_FP32
 @pytest.mark.parametrize("name,nstates", [("hbn30_cut20_x", 8), ("mos2_q_x", 4), ("random18", 16), ("random36", 8)])
-def test_fp32_build_and_mixed_solve(name, nstates, request):
+def test_fp32_build_and_mixed_solve(name, nstates, arith, request):
     ex, H = get_case(name, request)
     ctx = gpu_context(ex)
     ctx.build(0, scissor=ex.scissor)
@@ -124,9 +125,9 @@ def test_fp32_build_and_mixed_solve(name, nstates, request):
     blk, mx = ctx.compare_builds(4)
     assert mx == pytest.approx(err.max(), rel=1e-6, abs=1e-12)
     print(f"[{name}] FP32 block errors: max {mx:.3e}, median {np.median(blk):.3e}")
-    w, v, rep = ctx.solve(method=1, nstates=nstates, precision=1, tol=1e-6)
+    w, v, rep = ctx.solve(method=1, nstates=nstates, precision=1, tol=1e-6, fp32_arith=arith)
     wr = np.linalg.eigvalsh(H)[:nstates]
-    print(f"[{name}] mixed: per-eigenvalue error (meV): {np.round((w - wr) * 1e3, 6)}")
+    print(f"[{name}] mixed (arith {arith}): per-eigenvalue error (meV): {np.round((w - wr) * 1e3, 6)}")
     assert np.abs(w - wr).max() < 1e-4
     assert rep.refine_iterations >= 1
     ctx.close()
@@ -143,6 +144,29 @@ def test_matvec(name, request):
     y = H @ x
     assert rel_err(ctx.matvec(x, precision=0), y) < 1e-12
     assert rel_err(ctx.matvec(x, precision=1), y) < 1e-5
+    ctx.close()
+
+
+@pytest.mark.parametrize("name", ["hbn20", "random36"])
+def test_split_product_scaling(name, request):
+    # per-column power-of-two scales: the relative error does not depend on the magnitude
+    # of a column, zero columns stay zero, and fewer than 32 vectors are handled
+    ex, H = get_case(name, request)
+    ctx = gpu_context(ex)
+    ctx.build(1, scissor=ex.scissor)
+    rng = np.random.default_rng(5)
+    x = rng.normal(size=(ex.N, 7)) + 1j * rng.normal(size=(ex.N, 7))
+    x[:, 1] *= 1e-30
+    x[:, 2] *= 1e30
+    x[:, 3] = 0.0
+    x[::3, 4] = 0.0
+    y = ctx.matvec(x, precision=1)
+    ref = H @ x
+    for j in range(7):
+        if j == 3:
+            assert np.all(y[:, j] == 0)
+        else:
+            assert rel_err(y[:, j], ref[:, j]) < 1e-5, j
     ctx.close()
 
 

@@ -38,6 +38,7 @@ DEVICE_SOURCES = [
     "device/hf.hip",
     "device/band_adjoint.hip",
     "device/convolution.hip",
+    "device/split_product.hip",
     "device/capi.hip",
     "device/davidson.cpp",
 ]

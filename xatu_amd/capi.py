@@ -25,6 +25,7 @@ XATU_MATRIX_FREE_CONV = 1    # matrix-free product forms (solve / matvec)
 XATU_MATRIX_FREE_TILES = 2
 XATU_POTENTIAL_KELDYSH, XATU_POTENTIAL_COULOMB = 0, 1
 XATU_MODE_REALSPACE, XATU_MODE_RECIPROCAL = 0, 1
+XATU_ARITH_SPLIT16, XATU_ARITH_FP32 = 0, 1
 XATU_SOLVE_DIAG, XATU_SOLVE_DAVIDSON = 0, 1
 XATU_PARAM_R0, XATU_PARAM_EPS_M, XATU_PARAM_EPS_S = 0, 1, 2
 
@@ -49,7 +50,7 @@ class Interaction(ctypes.Structure):
 class SolveOptions(ctypes.Structure):
     _fields_ = [("method", ctypes.c_int), ("precision", ctypes.c_int), ("matrix_free", ctypes.c_int),
                 ("nstates", ctypes.c_int), ("block", ctypes.c_int), ("max_basis", ctypes.c_int),
-                ("max_iter", ctypes.c_int), ("tol", ctypes.c_double)]
+                ("max_iter", ctypes.c_int), ("tol", ctypes.c_double), ("fp32_arith", ctypes.c_int)]
 
 
 class SolveReport(ctypes.Structure):
@@ -199,8 +200,9 @@ class Context:
         self._check(self.L.xatu_free_bse(self._h, precision))
 
     def solve(self, method=XATU_SOLVE_DAVIDSON, nstates=8, precision=XATU_FP64, matrix_free=False, block=0,
-              max_basis=0, max_iter=0, tol=0.0, want_vectors=True):
-        opt = SolveOptions(method, precision, int(matrix_free), nstates, block, max_basis, max_iter, tol)
+              max_basis=0, max_iter=0, tol=0.0, want_vectors=True, fp32_arith=0):
+        opt = SolveOptions(method, precision, int(matrix_free), nstates, block, max_basis, max_iter, tol,
+                           int(fp32_arith))
         n_out = self.N if method == XATU_SOLVE_DIAG and nstates <= 0 else nstates
         w = np.zeros(max(n_out, 1))
         v = np.zeros((self.N, n_out), dtype=np.complex128, order="F") if want_vectors else None

@@ -189,6 +189,7 @@ void prepare_operator(Context& c) {
     c.h_diag.resize(c.N);
     XATU_HIP(hipMemcpyAsync(c.h_diag.data(), c.d_diag.p, c.N * sizeof(double), hipMemcpyDeviceToHost, c.stream));
     XATU_HIP(hipStreamSynchronize(c.stream));
+    split_product_scale(c);
     c.prepared = true;
     ++c.prep_gen;
 }
@@ -433,7 +434,10 @@ int xatu_build_bse(xatu_ctx* ctx, int precision, double scissor, void* hbs_out) 
             launch_build_tiles64(c, c.d_H64.p, c.stream);
         } else if (precision == XATU_FP32) {
             c.built32 = false;
-            c.d_H32.alloc(nn);
+            // 32 zero entries past the end: the split product reads up to 31 entries past
+            // the last column (multiplied by zero vector padding)
+            c.d_H32.alloc(nn + 32);
+            XATU_HIP(hipMemsetAsync(c.d_H32.p + nn, 0, 32 * sizeof(float2), c.stream));
             launch_build_tiles32(c, c.d_H32.p, c.stream);
         }
         evk.stop();
@@ -507,6 +511,9 @@ int xatu_solve(xatu_ctx* ctx, const xatu_solve_options* opt, double* eigval_out,
                xatu_solve_report* report) {
     return guarded(ctx, [&](Context& c) {
         if (!opt || !eigval_out) throw Error(XATU_ERR_INVALID_ARGUMENT, "null argument");
+        if (opt->fp32_arith != XATU_ARITH_SPLIT16 && opt->fp32_arith != XATU_ARITH_FP32) {
+            throw Error(XATU_ERR_INVALID_ARGUMENT, "fp32_arith must be XATU_ARITH_SPLIT16 or XATU_ARITH_FP32");
+        }
         xatu_solve_report rep{};
         c.tim.ms_matvec_kernel = 0.0;
         c.tim.n_matvec_kernel = 0;

@@ -146,7 +146,12 @@ struct Context {
 
     // scratch for products and projections
     DevBuf<double2> s_xt64, s_yp64;   // staged vectors / split partial sums, FP64 dense product
-    DevBuf<float2> s_xt32, s_yp32;    // same, FP32 dense product
+    DevBuf<float2> s_xt32, s_yp32;    // same, FP32 dense product (partial sums also of the split product)
+    DevBuf<float4> s_xs16;            // vectors staged as split-FP16 fragments
+    DevBuf<float> s_colmax, s_xscale; // column maxima and power-of-two scales of the staged vectors
+    DevBuf<double> s_xinv;            // inverse combined scales applied by the split-product reduction
+    float h16_scale = 0.0f;           // power-of-two scale of H for the split product (0: unavailable)
+    bool fp32_arith_only = false;     // products with the FP32 matrix stay on the FP32 matrix cores
     DevBuf<double2> s_gram;           // split partial sums of the Gram kernel
 
     // last solve
@@ -210,6 +215,13 @@ void launch_compare(const Context& c, int kblock, double* d_blockerr, hipStream_
 // FP32 matrix (the FP32 product stages and accumulates in FP32).
 void launch_matvec64(Context& c, const double2* X, double2* Y, int nvec, hipStream_t s);
 void launch_matvec32(Context& c, const double2* X, double2* Y, int nvec, hipStream_t s);
+// Products with the resident FP32 matrix on the FP16 matrix cores, operands split into
+// hi + lo FP16 parts (split_product.hip). split_product_scale sets h16_scale from bounds
+// on |H| (after the tables, bands and diagonal are ready); the split product needs an
+// even N (16-byte aligned columns) and the FP32 matrix with 32 zero entries of padding.
+void split_product_scale(Context& c);
+bool split_product_available(const Context& c);
+void launch_matvec_split16(Context& c, const double2* X, double2* Y, int nvec, hipStream_t s);
 // C (m x n, leading dimension ldc) = A^H B over the N rows of A (lda) and B (ldb), FP64.
 void launch_gram(Context& c, const double2* A, int lda, const double2* B, int ldb, int m, int n, double2* C, int ldc,
                  hipStream_t s);
@@ -227,8 +239,11 @@ void launch_matfree32(Context& c, const double2* X, double2* Y, int nvec, hipStr
 // product's precision, and the fixed-order sum Y[row + n N] = sum_s Yp[s][row][n].
 void launch_stage_vectors(const double2* X, int64_t N, int nvec, double2* Xt, hipStream_t s);
 void launch_stage_vectors(const double2* X, int64_t N, int nvec, float2* Xt, hipStream_t s);
-void launch_reduce_partials(const double2* Yp, int64_t N, int nvec, int splits, double2* Y, hipStream_t s);
-void launch_reduce_partials(const float2* Yp, int64_t N, int nvec, int splits, double2* Y, hipStream_t s);
+// scale (nullable, nvec entries): Y[row + n N] = scale[n] sum_s Yp[s][row][n].
+void launch_reduce_partials(const double2* Yp, int64_t N, int nvec, int splits, double2* Y, hipStream_t s,
+                            const double* scale = nullptr);
+void launch_reduce_partials(const float2* Yp, int64_t N, int nvec, int splits, double2* Y, hipStream_t s,
+                            const double* scale = nullptr);
 
 // Solver entry points (davidson.cpp).
 void solve_dense_diag(Context& c, int nstates, double* eigval_out, double* eigvec_out);

@@ -344,6 +344,13 @@ void solve_davidson(Context& c, const xatu_solve_options& opt, double* eigval_ou
     const double tol = opt.tol > 0 ? opt.tol : 1e-5;
 
     const double t0 = now_ms();
+    // arithmetic of the products with the FP32 matrix, for this solve only
+    struct ArithScope {
+        Context& c;
+        bool saved;
+        ArithScope(Context& cc, bool fp32) : c(cc), saved(cc.fp32_arith_only) { c.fp32_arith_only = fp32; }
+        ~ArithScope() { c.fp32_arith_only = saved; }
+    } arith(c, opt.fp32_arith == XATU_ARITH_FP32);
     std::unique_ptr<Davidson> d32;
     std::unique_ptr<Davidson> dav;
     bool ok;

@@ -173,20 +173,45 @@ __global__ __launch_bounds__(256, 3) void dense_product_kernel(const cx_t<R>* __
         }
 }
 
-// Y[r + n N] = sum_s Yp[s][r][n] for n < nvec (written in the output precision).
+// Y[r + n N] = scale[n] sum_s Yp[s][r][n] for n < nvec (scale null: 1), summed in the
+// output precision in a fixed order. A workgroup owns 64 rows: the partial rows are read
+// with consecutive threads on consecutive (row, vector) pairs, the sums are transposed
+// through LDS and written column by column.
 template <typename R, typename Rout>
-__global__ void reduce_product_kernel(const cx_t<R>* __restrict__ Yp, int64_t N, int nvec, int splits,
-                                      cx_t<Rout>* __restrict__ Y) {
-    const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    const int n = blockIdx.y;
-    if (idx >= N || n >= nvec) return;
-    Rout sr = 0, si = 0;
-    for (int s = 0; s < splits; ++s) {
-        const cx_t<R> v = Yp[(static_cast<int64_t>(s) * N + idx) * kVec + n];
-        sr += static_cast<Rout>(v.x);
-        si += static_cast<Rout>(v.y);
+__global__ __launch_bounds__(256) void reduce_product_kernel(const cx_t<R>* __restrict__ Yp, int64_t N, int nvec,
+                                                             int splits, const double* __restrict__ scale,
+                                                             cx_t<Rout>* __restrict__ Y) {
+    constexpr int kRows = 64;
+    __shared__ cx_t<Rout> tile[kRows][kVec + 1];
+    const int64_t r0 = static_cast<int64_t>(blockIdx.x) * kRows;
+#pragma unroll
+    for (int j = 0; j < kRows * kVec / 256; ++j) {
+        const int f = threadIdx.x + 256 * j;
+        const int rr = f / kVec, n = f % kVec;
+        const int64_t r = r0 + rr;
+        Rout sr = 0, si = 0;
+        if (r < N && n < nvec) {
+            for (int s = 0; s < splits; ++s) {
+                const cx_t<R> v = Yp[(static_cast<int64_t>(s) * N + r) * kVec + n];
+                sr += static_cast<Rout>(v.x);
+                si += static_cast<Rout>(v.y);
+            }
+            if (scale) {
+                const Rout sc = static_cast<Rout>(scale[n]);
+                sr *= sc;
+                si *= sc;
+            }
+        }
+        tile[rr][n] = mk(sr, si);
     }
-    Y[idx + static_cast<int64_t>(n) * N] = mk(sr, si);
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kRows * kVec / 256; ++j) {
+        const int f = threadIdx.x + 256 * j;
+        const int rr = f % kRows, n = f / kRows;
+        const int64_t r = r0 + rr;
+        if (r < N && n < nvec) Y[r + static_cast<int64_t>(n) * N] = tile[rr][n];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -299,8 +324,8 @@ void dense_product(const Context& c, const cx_t<R>* H, const cx_t<Rin>* X, cx_t<
                            xt.p);
         hipLaunchKernelGGL((dense_product_kernel<R>), dim3(row_blocks * splits), dim3(256), 0, s, H, xt.p, N, kchunk,
                            splits, yp.p);
-        hipLaunchKernelGGL((reduce_product_kernel<R, Rout>), dim3((N + 255) / 256, nv), dim3(256), 0, s, yp.p, N, nv,
-                           splits, Y + v0 * N);
+        hipLaunchKernelGGL((reduce_product_kernel<R, Rout>), dim3((N + 63) / 64), dim3(256), 0, s, yp.p, N, nv,
+                           splits, nullptr, Y + v0 * N);
     }
     XATU_HIP(hipGetLastError());
 }
@@ -311,8 +336,14 @@ void launch_matvec64(Context& c, const double2* X, double2* Y, int nvec, hipStre
     dense_product<double, double, double>(c, c.d_H64.p, X, Y, nvec, s, c.s_xt64, c.s_yp64);
 }
 
-// FP32 product of FP64 vectors: staged in FP32, accumulated in FP32, widened on output.
+// Product with the FP32 matrix: on the FP16 matrix cores with split operands where the
+// matrix layout allows it (even N), else staged in FP32 and accumulated in FP32 on the
+// FP32 matrix cores; widened to FP64 on output either way.
 void launch_matvec32(Context& c, const double2* X, double2* Y, int nvec, hipStream_t s) {
+    if (split_product_available(c)) {
+        launch_matvec_split16(c, X, Y, nvec, s);
+        return;
+    }
     dense_product<double, float, double>(c, c.d_H32.p, X, Y, nvec, s, c.s_xt32, c.s_yp32);
 }
 
@@ -324,13 +355,15 @@ void launch_stage_vectors(const double2* X, int64_t N, int nvec, float2* Xt, hip
     hipLaunchKernelGGL((stage_vectors_kernel<double, float>), dim3((N * kVec + 255) / 256), dim3(256), 0, s, X, N,
                        nvec, Xt);
 }
-void launch_reduce_partials(const double2* Yp, int64_t N, int nvec, int splits, double2* Y, hipStream_t s) {
-    hipLaunchKernelGGL((reduce_product_kernel<double, double>), dim3((N + 255) / 256, nvec), dim3(256), 0, s, Yp, N,
-                       nvec, splits, Y);
+void launch_reduce_partials(const double2* Yp, int64_t N, int nvec, int splits, double2* Y, hipStream_t s,
+                            const double* scale) {
+    hipLaunchKernelGGL((reduce_product_kernel<double, double>), dim3((N + 63) / 64), dim3(256), 0, s, Yp, N, nvec,
+                       splits, scale, Y);
 }
-void launch_reduce_partials(const float2* Yp, int64_t N, int nvec, int splits, double2* Y, hipStream_t s) {
-    hipLaunchKernelGGL((reduce_product_kernel<float, double>), dim3((N + 255) / 256, nvec), dim3(256), 0, s, Yp, N,
-                       nvec, splits, Y);
+void launch_reduce_partials(const float2* Yp, int64_t N, int nvec, int splits, double2* Y, hipStream_t s,
+                            const double* scale) {
+    hipLaunchKernelGGL((reduce_product_kernel<float, double>), dim3((N + 63) / 64), dim3(256), 0, s, Yp, N, nvec,
+                       splits, scale, Y);
 }
 
 void launch_gram(Context& c, const double2* A, int lda, const double2* B, int ldb, int m, int n, double2* C, int ldc,

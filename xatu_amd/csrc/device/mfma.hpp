@@ -27,6 +27,15 @@ struct Mfma<double> {
     __device__ static int out_row(int lk, int v) { return lk + 4 * v; }
 };
 
+// 16x16x32 FP16 product with FP32 accumulation (gfx950): lane l supplies 8 values
+// A[l % 16][k], B[k][l % 16] for one group of 8 k indices selected by l / 16 (the same
+// group for A and B); the output has the FP32 16x16x4 layout, row = 4 (l / 16) + v
+// (checked on gfx950).
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+__device__ inline f32x4 mma_f16(f16x8 a, f16x8 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+}
+
 // Vectors per product launch (two 16-column MFMA tiles).
 constexpr int kBlockVectors = 32;
 
