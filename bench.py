@@ -148,6 +148,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the FP64 plain path")
     ap.add_argument("--max-basis", type=int, default=0, help="Davidson basis size before restart (0: library default)")
+    ap.add_argument("--block", type=int, default=0, help="Davidson block size (0: the workload's)")
     args = ap.parse_args()
     # exactly one JSON line on stdout: everything else (including what the native library
     # prints) goes to stderr
@@ -164,6 +165,8 @@ def main():
 
     cfg = CONFIGS[args.config]
     natoms, orb, n, nb, r0, cutoff, nstates, block, mf = cfg
+    if args.block > 0:
+        block = args.block
     tmp = Path(tempfile.mkdtemp(prefix="xatu_bench_"))
     model = synthetic.write_model(tmp / "model.model", natoms=natoms, orb_per_atom=orb)
 

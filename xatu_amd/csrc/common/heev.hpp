@@ -38,17 +38,28 @@ struct Tridiag {
     std::vector<char> vused;
 };
 
+// sqrt(a^2 + b^2) without the cost of std::hypot where no scaling is needed.
+inline double fast_hypot(double a, double b) {
+    a = std::fabs(a);
+    b = std::fabs(b);
+    const double m = std::max(a, b);
+    if (m > 1e150 || (m < 1e-150 && m > 0.0)) return std::hypot(a, b);
+    return std::sqrt(a * a + b * b);
+}
+
+// Householder reduction on the lower triangle only (as LAPACK zhetd2 with uplo = L):
+// step k reflects column k below the diagonal, forms p = B v with B the trailing
+// Hermitian block (one pass over its lower triangle) and applies the rank-2 update
+// B -= 2 (v w^H + w v^H), w = p - (v^H p) v, to the lower triangle.
 inline void tridiagonalize(int n, const cd* Ain, int lda, Tridiag& t) {
     t.n = n;
-    // split storage, full Hermitian matrix, column-major
+    // split storage of the lower triangle, column-major
     std::vector<double> ar(static_cast<size_t>(n) * n), ai(static_cast<size_t>(n) * n);
     for (int j = 0; j < n; ++j)
         for (int i = j; i < n; ++i) {
             const cd v = Ain[i + static_cast<size_t>(j) * lda];
             ar[i + static_cast<size_t>(j) * n] = v.real();
             ai[i + static_cast<size_t>(j) * n] = (i == j) ? 0.0 : v.imag();
-            ar[j + static_cast<size_t>(i) * n] = v.real();
-            ai[j + static_cast<size_t>(i) * n] = (i == j) ? 0.0 : -v.imag();
         }
     t.d.assign(n, 0.0);
     t.e.assign(n, 0.0);
@@ -57,6 +68,8 @@ inline void tridiagonalize(int n, const cd* Ain, int lda, Tridiag& t) {
     t.vused.assign(n, 0);
     t.vre.clear();
     t.vim.clear();
+    t.vre.reserve(static_cast<size_t>(n) * n / 2 + n);
+    t.vim.reserve(static_cast<size_t>(n) * n / 2 + n);
     std::vector<double> pr(n), pi(n), wr(n), wi(n);
     for (int k = 0; k + 2 < n; ++k) {
         const int m = n - k - 1;
@@ -93,19 +106,26 @@ inline void tridiagonalize(int n, const cd* Ain, int lda, Tridiag& t) {
             vi[i] *= inv;
         }
         t.vused[k] = 1;
-        // p = B v with B = A[k+1:, k+1:]: p_i = sum_j conj(B_ji) v_j (column i of B is contiguous)
+        // p = B v over the lower triangle of B = A[k+1:, k+1:]: the stored B_ij (i > j)
+        // contributes B_ij v_j to p_i and conj(B_ij) v_i to p_j
         const int base = k + 1;
         for (int i = 0; i < m; ++i) {
-            const double* br = ar.data() + static_cast<size_t>(base + i) * n + base;
-            const double* bi = ai.data() + static_cast<size_t>(base + i) * n + base;
-            double sr = 0.0, si = 0.0;
-            for (int j = 0; j < m; ++j) {
-                // conj(b) * v
-                sr += br[j] * vr[j] + bi[j] * vi[j];
-                si += br[j] * vi[j] - bi[j] * vr[j];
+            pr[i] = 0.0;
+            pi[i] = 0.0;
+        }
+        for (int j = 0; j < m; ++j) {
+            const double* br = ar.data() + static_cast<size_t>(base + j) * n + base;
+            const double* bi = ai.data() + static_cast<size_t>(base + j) * n + base;
+            const double vjr = vr[j], vji = vi[j];
+            double sr = br[j] * vjr, si = br[j] * vji;  // real diagonal
+            for (int i = j + 1; i < m; ++i) {
+                pr[i] += br[i] * vjr - bi[i] * vji;
+                pi[i] += br[i] * vji + bi[i] * vjr;
+                sr += br[i] * vr[i] + bi[i] * vi[i];
+                si += br[i] * vi[i] - bi[i] * vr[i];
             }
-            pr[i] = sr;
-            pi[i] = si;
+            pr[j] += sr;
+            pi[j] += si;
         }
         double cr = 0.0;  // c = v^H p (real)
         for (int i = 0; i < m; ++i) cr += vr[i] * pr[i] + vi[i] * pi[i];
@@ -113,18 +133,17 @@ inline void tridiagonalize(int n, const cd* Ain, int lda, Tridiag& t) {
             wr[i] = pr[i] - cr * vr[i];
             wi[i] = pi[i] - cr * vi[i];
         }
-        // B -= 2 (v w^H + w v^H)
+        // lower triangle of B -= 2 (v w^H + w v^H)
         for (int j = 0; j < m; ++j) {
             double* br = ar.data() + static_cast<size_t>(base + j) * n + base;
             double* bi = ai.data() + static_cast<size_t>(base + j) * n + base;
-            const double wjr = wr[j], wji = -wi[j];  // conj(w_j)
-            const double vjr = vr[j], vji = -vi[j];  // conj(v_j)
-            for (int i = 0; i < m; ++i) {
-                const double r = vr[i] * wjr - vi[i] * wji + wr[i] * vjr - wi[i] * vji;
-                const double im = vr[i] * wji + vi[i] * wjr + wr[i] * vji + wi[i] * vjr;
-                br[i] -= 2.0 * r;
-                bi[i] -= 2.0 * im;
+            const double wjr = 2.0 * wr[j], wji = -2.0 * wi[j];  // 2 conj(w_j)
+            const double vjr = 2.0 * vr[j], vji = -2.0 * vi[j];  // 2 conj(v_j)
+            for (int i = j; i < m; ++i) {
+                br[i] -= vr[i] * wjr - vi[i] * wji + wr[i] * vjr - wi[i] * vji;
+                bi[i] -= vr[i] * wji + vi[i] * wjr + wr[i] * vji + wi[i] * vjr;
             }
+            bi[j] = 0.0;  // the diagonal stays real
         }
         ec[k] = cd(alr, ali);
         t.d[k] = ar[static_cast<size_t>(k) * n + k];
@@ -157,14 +176,14 @@ inline void tql(int n, std::vector<double>& d, std::vector<double> e, double* zr
             if (m != l) {
                 if (iter++ == 200) throw std::runtime_error("heev: QL iteration did not converge");
                 double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
-                double r = std::hypot(g, 1.0);
+                double r = fast_hypot(g, 1.0);
                 g = d[m] - d[l] + e[l] / (g + std::copysign(r, g));
                 double s = 1.0, c = 1.0, pp = 0.0;
                 bool deflated = false;
                 for (int i = m - 1; i >= l; --i) {
                     const double f = s * e[i];
                     const double b = c * e[i];
-                    r = std::hypot(f, g);
+                    r = fast_hypot(f, g);
                     e[i + 1] = r;
                     if (r == 0.0) {
                         d[i + 1] -= pp;
@@ -219,6 +238,43 @@ inline void back_transform(const Tridiag& t, const double* z, cd* x) {
             x[k + 1 + i] -= cd(sr * vr[i] - si * vi[i], sr * vi[i] + si * vr[i]);
         }
     }
+}
+
+// The same for nz vectors at once (z: n x nz, x: n x nz, column-major): each reflector is
+// applied to all columns while it is in cache.
+inline void back_transform_block(const Tridiag& t, const double* z, int nz, cd* x) {
+    const int n = t.n;
+    std::vector<double> xr(static_cast<size_t>(n) * nz), xi(static_cast<size_t>(n) * nz);
+    for (int c = 0; c < nz; ++c)
+        for (int i = 0; i < n; ++i) {
+            const cd v = t.ph[i] * z[i + static_cast<size_t>(c) * n];
+            xr[i + static_cast<size_t>(c) * n] = v.real();
+            xi[i + static_cast<size_t>(c) * n] = v.imag();
+        }
+    for (int k = n - 3; k >= 0; --k) {
+        if (!t.vused[k]) continue;
+        const int m = n - k - 1;
+        const double* vr = t.vre.data() + t.voff[k];
+        const double* vi = t.vim.data() + t.voff[k];
+        for (int c = 0; c < nz; ++c) {
+            double* ar = xr.data() + static_cast<size_t>(c) * n + k + 1;
+            double* ai = xi.data() + static_cast<size_t>(c) * n + k + 1;
+            double sr = 0.0, si = 0.0;  // s = v^H x
+            for (int i = 0; i < m; ++i) {
+                sr += vr[i] * ar[i] + vi[i] * ai[i];
+                si += vr[i] * ai[i] - vi[i] * ar[i];
+            }
+            sr *= 2.0;
+            si *= 2.0;
+            for (int i = 0; i < m; ++i) {
+                ar[i] -= sr * vr[i] - si * vi[i];
+                ai[i] -= sr * vi[i] + si * vr[i];
+            }
+        }
+    }
+    for (int c = 0; c < nz; ++c)
+        for (int i = 0; i < n; ++i)
+            x[i + static_cast<size_t>(c) * n] = cd(xr[i + static_cast<size_t>(c) * n], xi[i + static_cast<size_t>(c) * n]);
 }
 
 // Eigenvector of the real symmetric tridiagonal (d, e) for eigenvalue lam by inverse
@@ -349,7 +405,7 @@ inline void heev_lowest(int n, const cd* Ain, int lda, int nwant, double* w, cd*
         detail::inverse_iteration(t.d, t.e, lam, tnorm, prev, static_cast<unsigned>(i + 1),
                                   zs.data() + static_cast<size_t>(i) * n);
     }
-    for (int c = 0; c < nwant; ++c) detail::back_transform(t, zs.data() + static_cast<size_t>(c) * n, Z + static_cast<size_t>(c) * n);
+    detail::back_transform_block(t, zs.data(), nwant, Z);
 }
 
 }  // namespace xatu_linalg

@@ -198,13 +198,11 @@ void prepare_operator(Context& c) {
 
 namespace xatu_dev {
 
-// Product with the resident matrix of the given precision (or matrix-free).
-void apply_operator(Context& c, int precision, int matrix_free, const double2* X, double2* Y, int nvec,
-                    double* ms_kernel) {
-    EventTimer ev(c.stream);
+// Product with the resident matrix of the given precision (or matrix-free), launched
+// on the context stream without waiting.
+void launch_operator(Context& c, int precision, int matrix_free, const double2* X, double2* Y, int nvec) {
     if (matrix_free) {
         if (!c.prepared) throw Error(XATU_ERR_LOGIC, "operator not prepared (call xatu_build_bse)");
-        ev.start();
         // the convolution form needs a table periodic on the mesh: the reciprocal-space
         // mode runs its matrix-free products on recomputed tiles
         if (matrix_free == XATU_MATRIX_FREE_TILES || c.lat.recip) {
@@ -214,25 +212,33 @@ void apply_operator(Context& c, int precision, int matrix_free, const double2* X
             if (precision == XATU_FP64) launch_conv64(c, X, Y, nvec, c.stream);
             else launch_conv32(c, X, Y, nvec, c.stream);
         }
-        ev.stop();
     } else if (precision == XATU_FP64) {
         if (!c.built64) throw Error(XATU_ERR_LOGIC, "FP64 BSE matrix is not resident");
-        ev.start();
         launch_matvec64(c, X, Y, nvec, c.stream);
-        ev.stop();
     } else {
         if (!c.built32) throw Error(XATU_ERR_LOGIC, "FP32 BSE matrix is not resident");
-        ev.start();
         launch_matvec32(c, X, Y, nvec, c.stream);
-        ev.stop();
     }
+}
+
+// Product statistics (dense products only).
+void record_product(Context& c, int matrix_free, double ms, int nvec) {
+    if (matrix_free) return;
+    c.tim.ms_matvec_kernel += ms;
+    c.tim.n_matvec_kernel += 1;
+    c.tim.n_matvec_vectors += nvec;
+}
+
+// Timed product (waits for completion).
+void apply_operator(Context& c, int precision, int matrix_free, const double2* X, double2* Y, int nvec,
+                    double* ms_kernel) {
+    EventTimer ev(c.stream);
+    ev.start();
+    launch_operator(c, precision, matrix_free, X, Y, nvec);
+    ev.stop();
     const double ms = ev.ms();
     if (ms_kernel) *ms_kernel = ms;
-    if (!matrix_free) {
-        c.tim.ms_matvec_kernel += ms;
-        c.tim.n_matvec_kernel += 1;
-        c.tim.n_matvec_vectors += nvec;
-    }
+    record_product(c, matrix_free, ms, nvec);
 }
 
 }  // namespace xatu_dev

@@ -251,6 +251,8 @@ void solve_davidson(Context& c, const xatu_solve_options& opt, double* eigval_ou
                     xatu_solve_report* rep);
 void apply_operator(Context& c, int precision, int matrix_free, const double2* X, double2* Y, int nvec,
                     double* ms_kernel);
+void launch_operator(Context& c, int precision, int matrix_free, const double2* X, double2* Y, int nvec);
+void record_product(Context& c, int matrix_free, double ms, int nvec);
 
 // Hellmann-Feynman gradients (hf.hip / capi.cpp).
 void hf_gradients(Context& c, int nstates, const double2* d_vec, int nparams, const int* params,

@@ -141,22 +141,24 @@ private:
     void append_block(int nb) {
         double2* Vn = ws_.V.p + static_cast<size_t>(m_) * N_;
         double2* Wn = ws_.W.p + static_cast<size_t>(m_) * N_;
-        double ms = 0.0;
-        apply_operator(c_, prec_, mf_, Vn, Wn, nb, &ms);
-        prod_ms_ += ms;
+        // event pairs are read after the synchronisation below, so that timing adds no waits
+        EventTimer evp(c_.stream), evg(c_.stream);
+        evp.start();
+        launch_operator(c_, prec_, mf_, Vn, Wn, nb);
+        evp.stop();
         rep_->products += nb;
         const int mn = m_ + nb;
         // P = V[:, 0:mn]^H W_new  (mn x nb)
-        {
-            EventTimer ev(c_.stream);
-            ev.start();
-            launch_gram(c_, ws_.V.p, static_cast<int>(N_), Wn, static_cast<int>(N_), mn, nb, ws_.P.p, mn, c_.stream);
-            ev.stop();
-            proj_ms_ += ev.ms();
-        }
+        evg.start();
+        launch_gram(c_, ws_.V.p, static_cast<int>(N_), Wn, static_cast<int>(N_), mn, nb, ws_.P.p, mn, c_.stream);
+        evg.stop();
         std::vector<cd> P(static_cast<size_t>(mn) * nb);
         XATU_HIP(hipMemcpyAsync(P.data(), ws_.P.p, P.size() * sizeof(cd), hipMemcpyDeviceToHost, c_.stream));
         XATU_HIP(hipStreamSynchronize(c_.stream));
+        const double pms = evp.ms();
+        prod_ms_ += pms;
+        record_product(c_, mf_, pms, nb);
+        proj_ms_ += evg.ms();
         // rows of the existing basis
         for (int j = 0; j < nb; ++j) {
             for (int i = 0; i < m_; ++i) {
@@ -223,62 +225,73 @@ private:
         rep_->restarts++;
     }
 
-    // Orthogonalise the block Tb (N x nb) against V[:, 0:m] (two passes).
-    void project_out(double2* Tb, int nb) {
+    // Orthonormalise the nb vectors in the basis slot V[:, m:m+nb] against V[:, 0:m] and
+    // among themselves; returns the number kept, written to V[:, m:m+kept].
+    // Two passes of classical Gram-Schmidt against the basis; the Gram kernel of each
+    // pass forms [V, T]^H T, so the second pass also yields the block's own Gram matrix
+    // T^H T (its change by the second, O(eps) projection is O(eps^2)). The block is then
+    // whitened through the eigen-decomposition of its column-normalised Gram matrix,
+    // dropping dependent directions. An ill-conditioned block (kept eigenvalues spanning
+    // more than 1e8) repeats the whole procedure once on the whitened vectors.
+    int orthonormalise_slot(int nb, int round = 0) {
         const int n = static_cast<int>(N_);
+        const int mn = m_ + nb;
+        double2* T = ws_.V.p + static_cast<size_t>(m_) * N_;
         EventTimer ev(c_.stream);
         ev.start();
         for (int pass = 0; pass < 2; ++pass) {
-            launch_gram(c_, ws_.V.p, n, Tb, n, m_, nb, ws_.P.p, m_, c_.stream);
-            XATU_BLAS(rocblas_zgemm(c_.blas, rocblas_operation_none, rocblas_operation_none, n, nb, m_, &kMinusOne,
-                                    bz(ws_.V.p), n, bz(ws_.P.p), m_, &kOne, bz(Tb), n));
+            launch_gram(c_, ws_.V.p, n, T, n, mn, nb, ws_.P.p, mn, c_.stream);
+            if (m_ > 0) {
+                XATU_BLAS(rocblas_zgemm(c_.blas, rocblas_operation_none, rocblas_operation_none, n, nb, m_,
+                                        &kMinusOne, bz(ws_.V.p), n, bz(ws_.P.p), mn, &kOne, bz(T), n));
+            }
         }
         ev.stop();
-        proj_ms_ += ev.ms();
-    }
-
-    // Orthonormalise the block Tb in place via its Gram matrix; returns the kept count
-    // and writes the orthonormal vectors to V[:, m:m+kept].
-    int orthonormalise(double2* Tb, int nb) {
-        const int n = static_cast<int>(N_);
-        std::vector<cd> G(static_cast<size_t>(nb) * nb);
-        launch_gram(c_, Tb, n, Tb, n, nb, nb, ws_.P.p, nb, c_.stream);
-        XATU_HIP(hipMemcpyAsync(G.data(), ws_.P.p, G.size() * sizeof(cd), hipMemcpyDeviceToHost, c_.stream));
+        std::vector<cd> P(static_cast<size_t>(mn) * nb);
+        XATU_HIP(hipMemcpyAsync(P.data(), ws_.P.p, P.size() * sizeof(cd), hipMemcpyDeviceToHost, c_.stream));
         XATU_HIP(hipStreamSynchronize(c_.stream));
+        proj_ms_ += ev.ms();
+        // column-normalised Gram matrix of the block
+        std::vector<cd> G(static_cast<size_t>(nb) * nb);
+        std::vector<double> dsc(nb);
+        for (int j = 0; j < nb; ++j) {
+            const double g = P[(m_ + j) + static_cast<size_t>(j) * mn].real();
+            dsc[j] = g > 0.0 ? 1.0 / std::sqrt(g) : 0.0;
+        }
+        for (int j = 0; j < nb; ++j)
+            for (int i = 0; i < nb; ++i) G[i + static_cast<size_t>(j) * nb] = P[(m_ + i) + static_cast<size_t>(j) * mn] * dsc[i] * dsc[j];
         std::vector<double> lam(nb);
         std::vector<cd> U(static_cast<size_t>(nb) * nb);
         xatu_linalg::heev(nb, G.data(), nb, lam.data(), U.data());
         const double lmax = std::max(lam[nb - 1], 0.0);
         std::vector<cd> S;
         int kept = 0;
+        double lmin_kept = lmax;
         for (int j = nb - 1; j >= 0; --j) {
             if (lam[j] > 1e-20 && lam[j] > 1e-14 * lmax) {
                 const double sc = 1.0 / std::sqrt(lam[j]);
-                for (int i = 0; i < nb; ++i) S.push_back(U[i + static_cast<size_t>(j) * nb] * sc);
+                for (int i = 0; i < nb; ++i) S.push_back(U[i + static_cast<size_t>(j) * nb] * (dsc[i] * sc));
+                lmin_kept = lam[j];
                 ++kept;
             }
         }
         if (kept == 0) return 0;
+        // V[:, m:m+kept] = T S (through the scratch block: the product may not overlap its input)
         XATU_HIP(hipMemcpyAsync(ws_.P.p, S.data(), S.size() * sizeof(cd), hipMemcpyHostToDevice, c_.stream));
-        double2* dst = ws_.V.p + static_cast<size_t>(m_) * N_;
-        XATU_BLAS(rocblas_zgemm(c_.blas, rocblas_operation_none, rocblas_operation_none, n, kept, nb, &kOne, bz(Tb),
-                                n, bz(ws_.P.p), nb, &kZero, bz(dst), n));
+        XATU_BLAS(rocblas_zgemm(c_.blas, rocblas_operation_none, rocblas_operation_none, n, kept, nb, &kOne, bz(T), n,
+                                bz(ws_.P.p), nb, &kZero, bz(ws_.Tb.p), n));
+        XATU_HIP(hipMemcpyAsync(T, ws_.Tb.p, static_cast<size_t>(N_) * kept * sizeof(double2), hipMemcpyDeviceToDevice,
+                                c_.stream));
+        if (round == 0 && lmin_kept < 1e-8 * lmax) return orthonormalise_slot(kept, 1);
         return kept;
     }
 
     bool add_corrections() {
         const int nb = std::min(b_, mmax_ - m_);
         if (nb <= 0) return false;
-        launch_precondition(ws_.R.p, ws_.theta.p, c_.d_diag.p, N_, nb, ws_.Tb.p, c_.stream);
-        project_out(ws_.Tb.p, nb);
-        int kept = orthonormalise(ws_.Tb.p, nb);
-        if (kept == 0) return false;
-        // second pass on the orthonormal block for numerical safety
-        double2* Vn = ws_.V.p + static_cast<size_t>(m_) * N_;
-        XATU_HIP(hipMemcpyAsync(ws_.Tb.p, Vn, static_cast<size_t>(N_) * kept * sizeof(double2),
-                                hipMemcpyDeviceToDevice, c_.stream));
-        project_out(ws_.Tb.p, kept);
-        kept = orthonormalise(ws_.Tb.p, kept);
+        double2* T = ws_.V.p + static_cast<size_t>(m_) * N_;
+        launch_precondition(ws_.R.p, ws_.theta.p, c_.d_diag.p, N_, nb, T, c_.stream);
+        const int kept = orthonormalise_slot(nb);
         if (kept == 0) return false;
         append_block(kept);
         return true;
