@@ -51,7 +51,8 @@ def measured_traffic(config, kind):
     for line in path.read_text().splitlines():
         if line.startswith("traffic_bytes_per_launch:"):
             vals = dict(kv.split("=") for kv in line.split(":", 1)[1].split())
-            return float(vals[kind]) if kind in vals else None
+            v = float(vals[kind]) if kind in vals else float("nan")
+            return v if np.isfinite(v) else None
     return None
 
 
