@@ -40,17 +40,31 @@ __host__ __device__ inline C cmul(C a, C b) {
     r.y = a.x * b.y + a.y * b.x;
     return r;
 }
-// acc += a * b
-template <typename C>
-__host__ __device__ inline void cfma(C& acc, C a, C b) {
-    acc.x += a.x * b.x - a.y * b.y;
-    acc.y += a.x * b.y + a.y * b.x;
+// acc += a * b as two fused multiply-adds per component
+__host__ __device__ inline void cfma(double2& acc, double2 a, double2 b) {
+    acc.x = fma(a.x, b.x, acc.x);
+    acc.x = fma(-a.y, b.y, acc.x);
+    acc.y = fma(a.x, b.y, acc.y);
+    acc.y = fma(a.y, b.x, acc.y);
+}
+__host__ __device__ inline void cfma(float2& acc, float2 a, float2 b) {
+    acc.x = fmaf(a.x, b.x, acc.x);
+    acc.x = fmaf(-a.y, b.y, acc.x);
+    acc.y = fmaf(a.x, b.y, acc.y);
+    acc.y = fmaf(a.y, b.x, acc.y);
 }
 // acc += conj(a) * b
-template <typename C>
-__host__ __device__ inline void cfma_conj(C& acc, C a, C b) {
-    acc.x += a.x * b.x + a.y * b.y;
-    acc.y += a.x * b.y - a.y * b.x;
+__host__ __device__ inline void cfma_conj(double2& acc, double2 a, double2 b) {
+    acc.x = fma(a.x, b.x, acc.x);
+    acc.x = fma(a.y, b.y, acc.x);
+    acc.y = fma(a.x, b.y, acc.y);
+    acc.y = fma(-a.y, b.x, acc.y);
+}
+__host__ __device__ inline void cfma_conj(float2& acc, float2 a, float2 b) {
+    acc.x = fmaf(a.x, b.x, acc.x);
+    acc.x = fmaf(a.y, b.y, acc.x);
+    acc.y = fmaf(a.x, b.y, acc.y);
+    acc.y = fmaf(-a.y, b.x, acc.y);
 }
 template <typename C>
 __host__ __device__ inline C cconj(C a) {
