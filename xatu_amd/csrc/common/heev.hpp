@@ -22,6 +22,20 @@
 #include <stdexcept>
 #include <vector>
 
+// The O(n^3) loops below are also compiled for AVX2/FMA (x86-64 hosts of this GPU have
+// it); the loader picks the clone the host CPU supports.
+#if defined(__x86_64__) && !defined(__HIP_DEVICE_COMPILE__)
+#define XATU_HOST_CLONES __attribute__((target_clones("arch=haswell", "default")))
+#else
+#define XATU_HOST_CLONES
+#endif
+// The dot products of these loops may be summed in any order (vectorised partial sums).
+#if defined(__clang__)
+#define XATU_REASSOC _Pragma("clang fp reassociate(on)")
+#else
+#define XATU_REASSOC
+#endif
+
 namespace xatu_linalg {
 
 using cd = std::complex<double>;
@@ -45,6 +59,75 @@ inline double fast_hypot(double a, double b) {
     const double m = std::max(a, b);
     if (m > 1e150 || (m < 1e-150 && m > 0.0)) return std::hypot(a, b);
     return std::sqrt(a * a + b * b);
+}
+
+// Two-sided reflection of the trailing Hermitian block B (m x m, lower triangle stored
+// split in br/bi with leading dimension ld) by I - 2 v v^H: p = B v over the lower
+// triangle (the stored B_ij, i > j, contributes B_ij v_j to p_i and conj(B_ij) v_i to
+// p_j), w = p - (v^H p) v, then the lower triangle -= 2 (v w^H + w v^H).
+XATU_HOST_CLONES static void reflect_trailing(int m, int ld, double* __restrict__ ar, double* __restrict__ ai,
+                                              const double* __restrict__ vr, const double* __restrict__ vi,
+                                              double* __restrict__ pr, double* __restrict__ pi,
+                                              double* __restrict__ wr, double* __restrict__ wi) {
+    XATU_REASSOC
+    for (int i = 0; i < m; ++i) {
+        pr[i] = 0.0;
+        pi[i] = 0.0;
+    }
+    for (int j = 0; j < m; ++j) {
+        const double* br = ar + static_cast<size_t>(j) * ld;
+        const double* bi = ai + static_cast<size_t>(j) * ld;
+        const double vjr = vr[j], vji = vi[j];
+        double sr = br[j] * vjr, si = br[j] * vji;  // real diagonal
+        for (int i = j + 1; i < m; ++i) {
+            pr[i] += br[i] * vjr - bi[i] * vji;
+            pi[i] += br[i] * vji + bi[i] * vjr;
+            sr += br[i] * vr[i] + bi[i] * vi[i];
+            si += br[i] * vi[i] - bi[i] * vr[i];
+        }
+        pr[j] += sr;
+        pi[j] += si;
+    }
+    double cr = 0.0;  // c = v^H p (real)
+    for (int i = 0; i < m; ++i) cr += vr[i] * pr[i] + vi[i] * pi[i];
+    for (int i = 0; i < m; ++i) {
+        wr[i] = pr[i] - cr * vr[i];
+        wi[i] = pi[i] - cr * vi[i];
+    }
+    for (int j = 0; j < m; ++j) {
+        double* br = ar + static_cast<size_t>(j) * ld;
+        double* bi = ai + static_cast<size_t>(j) * ld;
+        const double wjr = 2.0 * wr[j], wji = -2.0 * wi[j];  // 2 conj(w_j)
+        const double vjr = 2.0 * vr[j], vji = -2.0 * vi[j];  // 2 conj(v_j)
+        for (int i = j; i < m; ++i) {
+            br[i] -= vr[i] * wjr - vi[i] * wji + wr[i] * vjr - wi[i] * vji;
+            bi[i] -= vr[i] * wji + vi[i] * wjr + wr[i] * vji + wi[i] * vjr;
+        }
+        bi[j] = 0.0;  // the diagonal stays real
+    }
+}
+
+// x (n x nz block, split, leading dimension ld) -= 2 v (v^H x) for a unit reflector v of
+// length m applied to rows [0, m) of x.
+XATU_HOST_CLONES static void reflect_block(int m, int nz, int ld, const double* __restrict__ vr,
+                                           const double* __restrict__ vi, double* __restrict__ xr,
+                                           double* __restrict__ xi) {
+    XATU_REASSOC
+    for (int c = 0; c < nz; ++c) {
+        double* ar = xr + static_cast<size_t>(c) * ld;
+        double* ai = xi + static_cast<size_t>(c) * ld;
+        double sr = 0.0, si = 0.0;  // s = v^H x
+        for (int i = 0; i < m; ++i) {
+            sr += vr[i] * ar[i] + vi[i] * ai[i];
+            si += vr[i] * ai[i] - vi[i] * ar[i];
+        }
+        sr *= 2.0;
+        si *= 2.0;
+        for (int i = 0; i < m; ++i) {
+            ar[i] -= sr * vr[i] - si * vi[i];
+            ai[i] -= sr * vi[i] + si * vr[i];
+        }
+    }
 }
 
 // Householder reduction on the lower triangle only (as LAPACK zhetd2 with uplo = L):
@@ -106,45 +189,8 @@ inline void tridiagonalize(int n, const cd* Ain, int lda, Tridiag& t) {
             vi[i] *= inv;
         }
         t.vused[k] = 1;
-        // p = B v over the lower triangle of B = A[k+1:, k+1:]: the stored B_ij (i > j)
-        // contributes B_ij v_j to p_i and conj(B_ij) v_i to p_j
-        const int base = k + 1;
-        for (int i = 0; i < m; ++i) {
-            pr[i] = 0.0;
-            pi[i] = 0.0;
-        }
-        for (int j = 0; j < m; ++j) {
-            const double* br = ar.data() + static_cast<size_t>(base + j) * n + base;
-            const double* bi = ai.data() + static_cast<size_t>(base + j) * n + base;
-            const double vjr = vr[j], vji = vi[j];
-            double sr = br[j] * vjr, si = br[j] * vji;  // real diagonal
-            for (int i = j + 1; i < m; ++i) {
-                pr[i] += br[i] * vjr - bi[i] * vji;
-                pi[i] += br[i] * vji + bi[i] * vjr;
-                sr += br[i] * vr[i] + bi[i] * vi[i];
-                si += br[i] * vi[i] - bi[i] * vr[i];
-            }
-            pr[j] += sr;
-            pi[j] += si;
-        }
-        double cr = 0.0;  // c = v^H p (real)
-        for (int i = 0; i < m; ++i) cr += vr[i] * pr[i] + vi[i] * pi[i];
-        for (int i = 0; i < m; ++i) {
-            wr[i] = pr[i] - cr * vr[i];
-            wi[i] = pi[i] - cr * vi[i];
-        }
-        // lower triangle of B -= 2 (v w^H + w v^H)
-        for (int j = 0; j < m; ++j) {
-            double* br = ar.data() + static_cast<size_t>(base + j) * n + base;
-            double* bi = ai.data() + static_cast<size_t>(base + j) * n + base;
-            const double wjr = 2.0 * wr[j], wji = -2.0 * wi[j];  // 2 conj(w_j)
-            const double vjr = 2.0 * vr[j], vji = -2.0 * vi[j];  // 2 conj(v_j)
-            for (int i = j; i < m; ++i) {
-                br[i] -= vr[i] * wjr - vi[i] * wji + wr[i] * vjr - wi[i] * vji;
-                bi[i] -= vr[i] * wji + vi[i] * wjr + wr[i] * vji + wi[i] * vjr;
-            }
-            bi[j] = 0.0;  // the diagonal stays real
-        }
+        const size_t base = static_cast<size_t>(k + 1) * n + k + 1;
+        reflect_trailing(m, n, ar.data() + base, ai.data() + base, vr, vi, pr.data(), pi.data(), wr.data(), wi.data());
         ec[k] = cd(alr, ali);
         t.d[k] = ar[static_cast<size_t>(k) * n + k];
     }
@@ -254,23 +300,8 @@ inline void back_transform_block(const Tridiag& t, const double* z, int nz, cd* 
     for (int k = n - 3; k >= 0; --k) {
         if (!t.vused[k]) continue;
         const int m = n - k - 1;
-        const double* vr = t.vre.data() + t.voff[k];
-        const double* vi = t.vim.data() + t.voff[k];
-        for (int c = 0; c < nz; ++c) {
-            double* ar = xr.data() + static_cast<size_t>(c) * n + k + 1;
-            double* ai = xi.data() + static_cast<size_t>(c) * n + k + 1;
-            double sr = 0.0, si = 0.0;  // s = v^H x
-            for (int i = 0; i < m; ++i) {
-                sr += vr[i] * ar[i] + vi[i] * ai[i];
-                si += vr[i] * ai[i] - vi[i] * ar[i];
-            }
-            sr *= 2.0;
-            si *= 2.0;
-            for (int i = 0; i < m; ++i) {
-                ar[i] -= sr * vr[i] - si * vi[i];
-                ai[i] -= sr * vi[i] + si * vr[i];
-            }
-        }
+        reflect_block(m, nz, n, t.vre.data() + t.voff[k], t.vim.data() + t.voff[k], xr.data() + k + 1,
+                      xi.data() + k + 1);
     }
     for (int c = 0; c < nz; ++c)
         for (int i = 0; i < n; ++i)
