@@ -588,3 +588,39 @@ def bse_reciprocal(ex, n_reciprocal):
         H[i, i] = h + h
     H = np.triu(H, 1) + np.triu(H, 1).conj().T + np.diag(np.real(np.diag(H)))
     return H
+
+
+# ------------------------------------------------------------------ result observables
+
+def exciton_spin(ex, x):
+    """ResultTB::spinX (src/ResultTB.cpp:21-99): <S_z> of the state x as {total, hole,
+    electron}. Orbital o of an atom carries s_z = +1/2 (o even) or -1/2 (o odd); per k the
+    hole matrix is Sh(i, j) = sum_a conj(V_j(a)) s_a V_i(a) (k stack, valence) and the
+    electron matrix Se(i, j) = sum_a conj(C_i(a)) s_a C_j(a) (k+Q stack, conduction);
+    hole = -x^H (1 (x) Sh) x, electron = x^H (Se (x) 1) x, total = Re(hole + electron)."""
+    s = ex.sys
+    if any(n % 2 for n in s.orb_per_atom):
+        raise ValueError("spinX needs pairs of spin orbitals on every atom")
+    sz = np.concatenate([np.tile([0.5, -0.5], n // 2) for n in s.orb_per_atom])
+    V = ex.eigvec_k[:, :ex.nv, :]
+    C = ex.eigvec_kq[:, ex.nv:, :]
+    X = np.asarray(x).reshape(ex.nk, ex.nc, ex.nv)
+    Sh = np.einsum("ajk,a,aik->kij", V.conj(), sz, V)
+    Se = np.einsum("aik,a,ajk->kij", C.conj(), sz, C)
+    hole = -np.einsum("kci,kij,kcj->", X.conj(), Sh, X)
+    electron = np.einsum("kiv,kij,kjv->", X.conj(), Se, X)
+    return np.array([(hole + electron).real, hole, electron])
+
+
+def real_space_wavefunction(ex, x, electron_index, hole_index, e_cell, h_cell):
+    """ResultTB::realSpaceWavefunction (src/ResultTB.cpp:310-368, 470-482): with
+    R = e_cell - h_cell, sum over orbitals alpha of the electron atom and beta of the hole
+    atom of |sum_{kcv} C_kc(alpha) conj(V_kv(beta)) e^{i k.R} x_kcv|^2."""
+    s = ex.sys
+    off = np.cumsum([0] + list(s.orb_per_atom))
+    ph = np.exp(1j * (s.kpoints @ (np.asarray(e_cell, float) - np.asarray(h_cell, float))))
+    X = (np.asarray(x).reshape(ex.nk, ex.nc * ex.nv) * ph[:, None]).reshape(ex.nk, ex.nc, ex.nv)
+    C = ex.eigvec_kq[off[electron_index]:off[electron_index + 1], ex.nv:, :]
+    V = ex.eigvec_k[off[hole_index]:off[hole_index + 1], :ex.nv, :]
+    amp = np.einsum("ack,bvk,kcv->ab", C, V.conj(), X)
+    return float(np.sum(np.abs(amp) ** 2))

@@ -368,3 +368,58 @@ def test_reciprocal_without_g_vectors_is_band_differences():
     e = np.array([ex.scissor + ex.eigval_kq[ex.nv + c, k] - ex.eigval_k[v, k]
                   for k in range(ex.nk) for c in range(ex.nc) for v in range(ex.nv)])
     assert np.array_equal(H, np.diag(e))
+
+
+# ------------------------------------------------------------------ result observables
+
+def test_real_space_wavefunction_reference_hash():
+    # test/tests.cpp:323-366: hole on atom 1 of cell 0, electron over the cells within
+    # 20 |a1|, summed over the degenerate lowest pair
+    ex = oracle_case(HBN, 20, 1, 0, (1, 1, 10))
+    w, v = np.linalg.eigh(ex.bse())
+    s = ex.sys
+    cells = s.truncate_supercell(20, np.linalg.norm(s.bravais[0]) * 20)
+    rswf = np.zeros(len(cells) * s.natoms)
+    for n in range(2):
+        it = 0
+        for cell in cells:
+            for a in range(s.natoms):
+                rswf[it] += O.real_space_wavefunction(ex, v[:, n], a, 1, cell, np.zeros(3))
+                it += 1
+    assert abs(O.array2hash(rswf) - 3.4185866144) < 1e-5
+
+
+def test_real_space_wavefunction_normalisation():
+    # summed over every electron site of the supercell and every hole orbital, the
+    # probability of a normalised state is the number of hole orbitals... for one hole
+    # site: sum over the n x n cells of the electron and its orbitals equals
+    # nk * sum_beta sum_k |...|: by Parseval over the mesh, sum_R |f(R)|^2 = nk sum_k |g(k)|^2
+    ex = oracle_case(MOS2, 4, 2, 0, (1., 4., 13.55))
+    rng = np.random.default_rng(2)
+    x = rng.normal(size=ex.N) + 1j * rng.normal(size=ex.N)
+    s = ex.sys
+    cells = s.combinations(4, centered=True) @ s.bravais[:s.ndim]
+    for h in range(s.natoms):
+        tot = sum(O.real_space_wavefunction(ex, x, e, h, c, np.zeros(3)) for c in cells for e in range(s.natoms))
+        off = np.cumsum([0] + list(s.orb_per_atom))
+        X = x.reshape(ex.nk, ex.nc, ex.nv)
+        g = np.einsum("ack,bvk,kcv->kab", ex.eigvec_kq[:, ex.nv:, :], ex.eigvec_k[off[h]:off[h + 1], :ex.nv, :].conj(), X)
+        assert tot == pytest.approx(ex.nk * np.sum(np.abs(g) ** 2), rel=1e-10)
+
+
+SPIN_MOS2 = np.array([[-9.915e-01, -5.000e-01, -4.915e-01], [-6.966e-04, -5.000e-01, 4.993e-01],
+                      [8.297e-03, 4.998e-01, -4.915e-01], [9.991e-01, 4.998e-01, 4.993e-01]])
+
+
+def test_spin_reference_values():
+    # test/tests.cpp:627-656 (MoS2, 12x12 sub-mesh of a 24x24 mesh, shifted; the four
+    # lowest states are non-degenerate, so the per-state values are gauge invariant)
+    ex = oracle_case(MOS2, 12, 2, 0, (1., 4., 13.55), factor=2, shift=(0.6628, -1.1480, 0))
+    w, v = np.linalg.eigh(ex.bse())
+    assert np.min(np.diff(w[:5])) > 1e-3
+    spin = np.array([O.exciton_spin(ex, v[:, n]).real for n in range(4)])
+    assert np.abs(spin - SPIN_MOS2).max() < 1e-4
+    # the total is the sum of the hole and electron parts; a spin-up/down basis state has
+    # hole and electron spins of +-1/2
+    assert np.allclose(spin[:, 0], spin[:, 1] + spin[:, 2], atol=1e-12)
+    assert np.all(np.abs(spin[:, 1:]) <= 0.5 + 1e-12)

@@ -48,6 +48,7 @@ HOST_SOURCES = [
     "host/system.cpp",
     "host/exciton.cpp",
     "host/optics.cpp",
+    "host/results.cpp",
     "host/pybind.cpp",
 ]
 

@@ -158,6 +158,31 @@ public:
     std::vector<double> reciprocalAmplitude(int stateindex) const;
     void writeReciprocalAmplitude(const std::vector<cdouble>& coefs, FILE* textfile) const;
     void writeReciprocalAmplitude(int stateindex, FILE* textfile) const;
+    void writeExtendedReciprocalAmplitude(const std::vector<cdouble>& coefs, FILE* textfile) const;
+    void writeExtendedReciprocalAmplitude(int stateindex, FILE* textfile) const;
+    void writePhase(const std::vector<cdouble>& coefs, FILE* textfile) const;
+    void writePhase(int stateindex, FILE* textfile) const;
+    void writeExtendedPhase(const std::vector<cdouble>& coefs, FILE* textfile) const;
+    void writeExtendedPhase(int stateindex, FILE* textfile) const;
+    // Spin <S_z>: {total, hole, electron} (src/ResultTB.cpp:21-99, Result.tpp:145-150, 456-470)
+    std::vector<cdouble> spinX(const std::vector<cdouble>& coefs) const;
+    std::vector<cdouble> spinX(int stateindex) const;
+    void writeSpin(int n, FILE* textfile) const;
+    // Real-space electron probability with a fixed hole (src/ResultTB.cpp:310-411);
+    // cells and positions are Cartesian 3-vectors.
+    double realSpaceWavefunction(const std::vector<cdouble>& coefs, int electronIndex, int holeIndex,
+                                 const std::vector<double>& eCell, const std::vector<double>& hCell) const;
+    void writeRealspaceAmplitude(const std::vector<cdouble>& coefs, int holeIndex, const std::vector<double>& holeCell,
+                                 FILE* textfile, int ncells) const;
+    void writeRealspaceAmplitude(int stateindex, int holeIndex, const std::vector<double>& holeCell, FILE* textfile,
+                                 int ncells) const;
+    // Result.tpp:66-139, 479-499
+    double bindingEnergy(int stateindex, double gap = -1) const;
+    double determineGap() const;
+    int findExcitonPeak(int stateindex) const;
+    double boundingBoxBZ() const;
+    double ftExcitonEnvelope(int stateindex, const std::vector<double>& electronPosition,
+                             const std::vector<double>& holePosition) const;
 };
 
 // Single-particle velocity matrix elements of the k stack, V[((k*3 + j)*nb + n)*nb + m] (a.u.).

@@ -167,7 +167,8 @@ PYBIND11_MODULE(_xatu, m) {
         .def_readonly("ms_rayleigh_ritz", &xatu_solve_report::ms_rayleigh_ritz)
         .def_readonly("ms_projections", &xatu_solve_report::ms_projections);
 
-    py::class_<ResultTB>(m, "ResultTB")
+    auto result = py::class_<ResultTB>(m, "ResultTB");
+    result
         .def_property_readonly("eigval", [](const ResultTB& r) { return py::array_t<double>(r.eigval.size(), r.eigval.data()); })
         .def_property_readonly("eigvec", [](const ResultTB& r) { return fortran(r.eigvec, {r.nrows, r.ncols}); })
         .def("excitonOscillatorStrength",
@@ -207,7 +208,69 @@ PYBIND11_MODULE(_xatu, m) {
                  r.writeStates(f, n);
                  std::fclose(f);
              },
-             py::arg("filename"), py::arg("n") = 0);
+             py::arg("filename"), py::arg("n") = 0)
+        .def("spinX",
+             [](const ResultTB& r, py::object state) {
+                 std::vector<cdouble> sp = py::isinstance<py::int_>(state)
+                                               ? r.spinX(state.cast<int>())
+                                               : r.spinX(state.cast<std::vector<cdouble>>());
+                 return py::array_t<cdouble>(sp.size(), sp.data());
+             },
+             py::arg("state"))
+        .def("writeSpin",
+             [](const ResultTB& r, int n, const std::string& name) {
+                 FILE* f = open_or_throw(name);
+                 try {
+                     r.writeSpin(n, f);
+                 } catch (...) {
+                     std::fclose(f);
+                     throw;
+                 }
+                 std::fclose(f);
+             },
+             py::arg("n"), py::arg("filename"))
+        .def("realSpaceWavefunction",
+             [](const ResultTB& r, const std::vector<cdouble>& coefs, int e, int h, const std::vector<double>& ec,
+                const std::vector<double>& hc) { return r.realSpaceWavefunction(coefs, e, h, ec, hc); },
+             py::arg("coefs"), py::arg("electronIndex"), py::arg("holeIndex"), py::arg("eCell"), py::arg("hCell"))
+        .def("writeRealspaceAmplitude",
+             [](const ResultTB& r, py::object state, int hole, const std::vector<double>& hcell,
+                const std::string& name, int ncells) {
+                 FILE* f = open_or_throw(name);
+                 try {
+                     if (py::isinstance<py::int_>(state)) r.writeRealspaceAmplitude(state.cast<int>(), hole, hcell, f, ncells);
+                     else r.writeRealspaceAmplitude(state.cast<std::vector<cdouble>>(), hole, hcell, f, ncells);
+                 } catch (...) {
+                     std::fclose(f);
+                     throw;
+                 }
+                 std::fclose(f);
+             },
+             py::arg("state"), py::arg("holeIndex"), py::arg("holeCell"), py::arg("filename"), py::arg("ncells"))
+        .def("bindingEnergy", &ResultTB::bindingEnergy, py::arg("stateindex"), py::arg("gap") = -1.0)
+        .def("determineGap", &ResultTB::determineGap)
+        .def("findExcitonPeak", &ResultTB::findExcitonPeak)
+        .def("boundingBoxBZ", &ResultTB::boundingBoxBZ)
+        .def("ftExcitonEnvelope", &ResultTB::ftExcitonEnvelope, py::arg("stateindex"), py::arg("electron_position"),
+             py::arg("hole_position"));
+    auto def_state_writer = [&](const char* name, void (ResultTB::*by_coefs)(const std::vector<cdouble>&, FILE*) const,
+                                void (ResultTB::*by_index)(int, FILE*) const) {
+        result.def(name, [by_coefs, by_index](const ResultTB& r, py::object state, const std::string& fname) {
+            FILE* f = open_or_throw(fname);
+            try {
+                if (py::isinstance<py::int_>(state)) (r.*by_index)(state.cast<int>(), f);
+                else (r.*by_coefs)(state.cast<std::vector<cdouble>>(), f);
+            } catch (...) {
+                std::fclose(f);
+                throw;
+            }
+            std::fclose(f);
+        }, py::arg("state"), py::arg("filename"));
+    };
+    def_state_writer("writeExtendedReciprocalAmplitude", &ResultTB::writeExtendedReciprocalAmplitude,
+                     &ResultTB::writeExtendedReciprocalAmplitude);
+    def_state_writer("writePhase", &ResultTB::writePhase, &ResultTB::writePhase);
+    def_state_writer("writeExtendedPhase", &ResultTB::writeExtendedPhase, &ResultTB::writeExtendedPhase);
 
     py::class_<ExcitonTB>(m, "ExcitonTB")
         .def(py::init<const SystemConfiguration&, int, int, int, const std::vector<double>&, const std::vector<double>&>(),
