@@ -27,6 +27,8 @@ from __future__ import annotations
 import ctypes
 from pathlib import Path
 
+import re
+
 import numpy as np
 
 _HERE = Path(__file__).resolve().parent
@@ -177,6 +179,11 @@ class System:
         self.H = np.asarray(model["H"], complex)
         self.S = model.get("S")
         self.filling = model["filling"]
+        # SystemTB.cpp:30 / utils.cpp:176-178: H(0) holding only one triangle marks the
+        # matrices as triangles (H(k) = T(k) + T(k)^H - diag T(k))
+        H0 = self.H[0]
+        self.triangular = bool(np.all(np.tril(H0, -1) == 0) != np.all(np.triu(H0, 1) == 0))
+        self.au = False
         self.fermi = self.filling - 1
         self.orb_per_atom = [self.orbitals[int(s)] for s in self.motif[:, 3]]
         self.basisdim = sum(self.orb_per_atom)
@@ -224,15 +231,25 @@ class System:
         n = lib().oracle_truncate_supercell(self.ndim, _d(self.bravais), ncell, radius, _d(out), len(out))
         return out[:n].copy()
 
-    def hamiltonian(self, k):
+    def _bloch(self, k, M):
         k = np.atleast_2d(k)
         ph = np.exp(1j * (k @ self.cells.T))  # nk x ncells
-        return np.einsum("kr,rij->kij", ph, self.H)
+        h = np.einsum("kr,rij->kij", ph, M)
+        if self.triangular:  # SystemTB.cpp:69-72
+            d = np.einsum("kii->ki", h)
+            h = h - np.einsum("ki,ij->kij", d / 2, np.eye(h.shape[-1]))
+            h = h + np.conj(np.swapaxes(h, -1, -2))
+        return h
+
+    def hamiltonian(self, k):
+        return self._bloch(k, self.H)
 
     def solve_bands(self, k):
         h = self.hamiltonian(k)
         if self.S is not None and len(self.S):
-            s = np.einsum("kr,rij->kij", np.exp(1j * (np.atleast_2d(k) @ self.cells.T)), self.S)
+            if self.au:  # SystemTB.cpp:117-122 (Hartree -> eV with the reference's 27.2)
+                h = h * 27.2
+            s = self._bloch(k, self.S)
             w, v = np.linalg.eigh(s)
             root = np.einsum("kij,kj,klj->kil", v, 1 / np.sqrt(w), v.conj())
             h = root @ h @ root
@@ -624,3 +641,123 @@ def real_space_wavefunction(ex, x, electron_index, hole_index, e_cell, h_cell):
     V = ex.eigvec_k[off[hole_index]:off[hole_index + 1], :ex.nv, :]
     amp = np.einsum("ack,bvk,kcv->ab", C, V.conj(), X)
     return float(np.sum(np.abs(amp) ** 2))
+
+
+# ------------------------------------------------------------------ CRYSTAL outputs
+
+def parse_crystal(path, ncells=20):
+    """CRYSTALConfiguration (src/CRYSTALConfiguration.cpp:18-464): lattice (dimension
+    lowered for (0, 0, 500) / (0, 500, 0) or zero vectors), motif with species in order of
+    first appearance, orbitals per species from the basis-set section, and the overlap and
+    Fock matrices (printed lower triangles, column blocks) of the cells with index <= ncells.
+    Returns the model dictionary System() takes, plus the raw counts."""
+    lines = Path(path).read_text().split("\n")
+    if lines and lines[-1] == "":
+        lines.pop()
+    pos = 0
+    st = dict(natoms=0, norbitals=0, nsh=0, electrons=0, core=0, ndim=0)
+    bravais, motif, shells, orbs = None, None, [], []
+    cells, S, F = [], [], []
+
+    def numbers_after(line, key, typ=int):
+        rest = line[line.index(key) + len(key):]
+        tok = rest.split()
+        return typ(tok[0])
+
+    def cell_header(line, key):
+        rest = line[line.index(key) + len(key):]
+        idx, rest = rest.split("(", 1)
+        x, y, z = rest.split(")")[0].split()
+        return int(idx), (int(x), int(y), int(z))
+
+    def matrix(n):
+        nonlocal pos
+        m = np.zeros((n, n))
+        cols, row, col = [], 0, 0
+        while pos < len(lines):
+            line = lines[pos]; pos += 1
+            if line == "":
+                cols = [int(t) for t in lines[pos].split()]; pos += 1
+                line = lines[pos]; pos += 1
+                if line == "":
+                    continue
+            tok = line.split()
+            row = int(tok[0])
+            for i, t in enumerate(tok[1:]):
+                col = cols[i]
+                m[row - 1, col - 1] = float(t)
+            if row == n and col == n:
+                return m
+        return m
+
+    def count_numbers(line):
+        n = 0
+        for t in re.findall(r"[-+]?(?:\d+\.?\d*|\.\d+)(?:[eE][-+]?\d+)?|\S+", line):
+            try:
+                float(t)
+            except ValueError:
+                break
+            n += 1
+        return n
+
+    while pos < len(lines):
+        line = lines[pos]; pos += 1
+        if "DIRECT LATTICE VECTOR COMPONENTS (ANGSTROM)" in line:
+            vec = np.array([[float(x) for x in lines[pos + i].split()[:3]] for i in range(3)])
+            pos += 3
+            rows = 3
+            if np.all(np.abs(vec[2] - [0, 0, 500]) <= 0.1) or np.all(np.abs(vec[2]) <= 0.1):
+                rows = 2
+                if np.all(np.abs(vec[1] - [0, 500, 0]) <= 0.1) or np.all(np.abs(vec[1]) <= 0.1):
+                    rows = 1
+            bravais, st["ndim"] = vec[:rows], rows
+        elif "N. OF ATOMS PER CELL" in line:
+            st["natoms"] = numbers_after(line, "N. OF ATOMS PER CELL")
+        elif "NUMBER OF SHELLS" in line:
+            st["nsh"] = numbers_after(line, "NUMBER OF SHELLS")
+        elif "NUMBER OF AO" in line:
+            st["norbitals"] = numbers_after(line, "NUMBER OF AO")
+        elif "N. OF ELECTRONS PER CELL" in line:
+            st["electrons"] = numbers_after(line, "N. OF ELECTRONS PER CELL")
+        elif "CORE ELECTRONS PER CELL" in line:
+            st["core"] = numbers_after(line, "CORE ELECTRONS PER CELL")
+        elif "ATOM" in line and "SHELL" in line:
+            pos += 1
+            species, motif, shells = [], [], []
+            for _ in range(st["natoms"]):
+                tok = lines[pos].split(); pos += 1
+                if tok[2] not in species:
+                    species.append(tok[2]); shells.append(int(tok[3]))
+                motif.append([float(tok[4]), float(tok[5]), float(tok[6]), species.index(tok[2])])
+            motif = np.array(motif)
+        elif "LOCAL ATOMIC FUNCTIONS BASIS SET" in line:
+            pos += 3
+            seen, total, ns, last = [], 0, 0, 0
+            for a in range(st["natoms"]):
+                tok = lines[pos].split(); pos += 1
+                if tok[1] in seen:
+                    total += orbs[int(motif[a, 3])]
+                    continue
+                seen.append(tok[1])
+                for _ in range(shells[ns]):
+                    head = lines[pos].replace("-", " - ", 1).split(); pos += 1
+                    last = int(head[2]) if head[1] == "-" else int(head[0])
+                    while pos < len(lines) and count_numbers(lines[pos]) == 4:
+                        pos += 1
+                orbs.append(last - total)
+                total = last
+                ns += 1
+        elif "OVERLAP MATRIX" in line:
+            idx, coef = cell_header(line, "OVERLAP MATRIX - CELL N.")
+            if idx <= ncells:
+                cells.append(sum(bravais[i] * coef[i] for i in range(st["ndim"])))
+                S.append(matrix(st["norbitals"]))
+        if "FOCK MATRIX" in line:
+            idx, coef = cell_header(line, "FOCK MATRIX - CELL N.")
+            if idx <= ncells:
+                F.append(matrix(st["norbitals"]))
+    if st["ndim"] == 0:
+        raise ValueError("Finite systems are not yet implemented")
+    model = dict(ndim=st["ndim"], bravais=bravais, motif=motif, norbitals=orbs, cells=np.array(cells),
+                 H=np.array(F, complex), S=np.array(S, complex), filling=st["electrons"] // 2)
+    return model, st

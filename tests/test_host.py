@@ -174,3 +174,42 @@ def test_exciton_band_selection():
     assert list(ex2.bandList) == list(ex.bandList)
     ex3 = X.ExcitonTB(cfg, X.ExcitonConfiguration(str(MODELS / "MoS2_exciton.txt")))
     assert list(ex3.bandList) == list(ex.bandList) and ex3.cutoff == pytest.approx(30 / 2.5)
+
+
+def test_crystal_parser_matches_oracle():
+    path = str(MODELS / "hBN_base_CREN_LDA.outp")
+    m, st = O.parse_crystal(path, 30)
+    c = X.CRYSTALConfiguration(path, 30)
+    assert (c.ndim, c.natoms, c.norbitalsAO, c.nsh, c.totalElectrons, c.coreElectrons) == \
+        (st["ndim"], st["natoms"], st["norbitals"], st["nsh"], st["electrons"], st["core"])
+    assert list(c.orbitalsPerSpecies) == m["norbitals"] and c.filling == m["filling"]
+    assert c.nOverlap == c.nFock == 30 and not c.MAGNETIC_FLAG
+    assert np.array_equal(np.moveaxis(np.asarray(c.hamiltonian), 2, 0), m["H"])
+    assert np.array_equal(np.moveaxis(np.asarray(c.overlap), 2, 0), m["S"])
+    assert np.array_equal(np.asarray(c.bravaisVectors), m["cells"])
+    assert np.array_equal(np.asarray(c.motif), m["motif"])
+    # four shells with four gaussians each
+    assert [len(sh) for sp in sorted(c.gaussianCoefficients) for sh in c.gaussianCoefficients[sp]] == [4, 4, 4, 4]
+
+
+def test_crystal_bands_match_oracle():
+    path = str(MODELS / "hBN_base_CREN_LDA.outp")
+    s = X.SystemTB(X.CRYSTALConfiguration(path, 100))
+    s.setAU(True)
+    m, _ = O.parse_crystal(path, 100)
+    o = O.System(m)
+    o.au = True
+    assert s.isTriangular and o.triangular
+    s.brillouinZoneMesh(9)
+    o.brillouin_zone_mesh(9)
+    for k in s.kpoints[::7]:
+        w, _ = s.solveBands(list(k))
+        wr, _ = o.solve_bands(k)
+        assert np.abs(w - wr[0]).max() < 1e-10
+
+
+def test_crystal_needs_a_lattice(tmp_path):
+    f = tmp_path / "x.outp"
+    f.write_text("nothing here\n")
+    with pytest.raises(Exception, match="Finite systems"):
+        X.CRYSTALConfiguration(str(f), 10)

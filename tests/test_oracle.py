@@ -423,3 +423,45 @@ def test_spin_reference_values():
     # hole and electron spins of +-1/2
     assert np.allclose(spin[:, 0], spin[:, 1] + spin[:, 2], atol=1e-12)
     assert np.all(np.abs(spin[:, 1:]) <= 0.5 + 1e-12)
+
+
+# ------------------------------------------------------------------ CRYSTAL outputs
+
+CRYSTAL_LDA = MODELS / "hBN_base_CREN_LDA.outp"
+
+
+def test_crystal_parser_reads_the_scf_output():
+    # header values printed in the file (lines 63-72, 137-162): 2 atoms, 4 shells, 8 AO,
+    # 8 electrons, B and N with an s and a p shell each (AO 1, 2-4 and 5, 6-8)
+    m, st = O.parse_crystal(CRYSTAL_LDA, 100)
+    assert (st["natoms"], st["nsh"], st["norbitals"], st["electrons"], st["core"], st["ndim"]) == (2, 4, 8, 8, 0, 2)
+    assert m["norbitals"] == [4, 4] and m["filling"] == 4
+    assert np.allclose(m["bravais"], [[2.16506, -1.25, 0], [0, 2.5, 0]])
+    assert np.allclose(m["motif"], [[-0.722, -1.25, 0, 0], [-1.443, 0, 0, 1]])
+    assert len(m["cells"]) == 100 and m["H"].shape == (100, 8, 8) and m["S"].shape == (100, 8, 8)
+    # first cell is the origin; every printed matrix is a lower triangle; S(0) has a unit
+    # diagonal (normalised atomic functions)
+    assert np.allclose(m["cells"][0], 0) and np.all(np.triu(m["S"], 1) == 0) and np.all(np.triu(m["H"], 1) == 0)
+    assert np.allclose(np.diag(m["S"][0]), 1.0)
+    # cell 2 is (-1, -1, 0) in lattice coordinates
+    assert np.allclose(m["cells"][1], -m["bravais"][0] - m["bravais"][1])
+    # fewer cells when asked
+    m10, _ = O.parse_crystal(CRYSTAL_LDA, 10)
+    assert len(m10["cells"]) == 10 and np.array_equal(m10["H"], m["H"][:10])
+
+
+def test_crystal_lda_bands():
+    # Lowdin-orthogonalised bands of the LDA hBN output (Fock in Hartree, 27.2 eV each):
+    # the overlap is positive definite on the mesh and the direct gap sits at K with the
+    # LDA value (about 4.5 eV)
+    m, _ = O.parse_crystal(CRYSTAL_LDA, 100)
+    s = O.System(m)
+    s.au = True
+    assert s.triangular
+    s.brillouin_zone_mesh(30)
+    assert np.linalg.eigvalsh(s._bloch(s.kpoints, s.S)).min() > 0.1
+    w, _ = s.solve_bands(s.kpoints)
+    gap = w[:, 4] - w[:, 3]
+    kmin = s.kpoints[np.argmin(gap)]
+    assert abs(np.linalg.norm(kmin) - np.linalg.norm(s.recip[0]) / np.sqrt(3)) < 1e-5  # |K| = |b1|/sqrt(3)
+    assert 4.4 < gap.min() < 4.7

@@ -18,11 +18,12 @@ def _load():
 
 _xatu = _load()
 SystemConfiguration = _xatu.SystemConfiguration
+CRYSTALConfiguration = _xatu.CRYSTALConfiguration
 ExcitonConfiguration = _xatu.ExcitonConfiguration
 SystemTB = _xatu.SystemTB
 ExcitonTB = _xatu.ExcitonTB
 ResultTB = _xatu.ResultTB
 detectDegeneracies = _xatu.detectDegeneracies
 
-__all__ = ["SystemConfiguration", "ExcitonConfiguration", "SystemTB", "ExcitonTB", "ResultTB",
+__all__ = ["SystemConfiguration", "CRYSTALConfiguration", "ExcitonConfiguration", "SystemTB", "ExcitonTB", "ResultTB",
            "detectDegeneracies", "LIB_PATH"]

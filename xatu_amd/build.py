@@ -49,6 +49,7 @@ HOST_SOURCES = [
     "host/exciton.cpp",
     "host/optics.cpp",
     "host/results.cpp",
+    "host/crystal.cpp",
     "host/pybind.cpp",
 ]
 

@@ -28,6 +28,8 @@ public:
 
 protected:
     explicit ConfigurationBase(const std::string& file);
+    // the file's lines, without their '\n'
+    const std::vector<std::string>& fileLines() const { return lines_; }
 
     void extractArguments();
     void extractRawContent();
@@ -80,6 +82,11 @@ class SystemConfiguration : public ConfigurationBase {
 public:
     SystemInfo systemInfo;
     explicit SystemConfiguration(const std::string& filename);
+
+protected:
+    // Reads the file without parsing it as a '#'-block model (for other formats).
+    struct RawFile {};
+    SystemConfiguration(const std::string& filename, RawFile) : ConfigurationBase(filename) {}
 
 private:
     void parseContent();

@@ -10,6 +10,7 @@
 #include <cstring>
 
 #include "../common/heev.hpp"
+#include "crystal.hpp"
 #include "exciton.hpp"
 
 namespace py = pybind11;
@@ -95,6 +96,23 @@ PYBIND11_MODULE(_xatu, m) {
                                [](const ExcitonConfiguration& c) { return c.excitonInfo.exchangePotential; })
         .def_property_readonly("regularization",
                                [](const ExcitonConfiguration& c) { return c.excitonInfo.regularization; });
+
+    py::class_<CRYSTALConfiguration, SystemConfiguration>(m, "CRYSTALConfiguration")
+        .def(py::init<const std::string&, int>(), py::arg("file"), py::arg("ncells") = 20)
+        .def_readonly("natoms", &CRYSTALConfiguration::natoms)
+        .def_readonly("norbitalsAO", &CRYSTALConfiguration::norbitals)
+        .def_readonly("nsh", &CRYSTALConfiguration::nsh)
+        .def_readonly("nspecies", &CRYSTALConfiguration::nspecies)
+        .def_readonly("totalElectrons", &CRYSTALConfiguration::totalElectrons)
+        .def_readonly("coreElectrons", &CRYSTALConfiguration::coreElectrons)
+        .def_readonly("SOC_FLAG", &CRYSTALConfiguration::SOC_FLAG)
+        .def_readonly("MAGNETIC_FLAG", &CRYSTALConfiguration::MAGNETIC_FLAG)
+        .def_readonly("shellsPerSpecies", &CRYSTALConfiguration::shellsPerSpecies)
+        .def_readonly("orbitalsPerSpecies", &CRYSTALConfiguration::orbitalsPerSpecies)
+        .def_readonly("atomic_number_ordering", &CRYSTALConfiguration::atomic_number_ordering)
+        .def_readonly("gaussianCoefficients", &CRYSTALConfiguration::gaussianCoefficients)
+        .def_readonly("nOverlap", &CRYSTALConfiguration::nOverlap)
+        .def_readonly("nFock", &CRYSTALConfiguration::nFock);
 
     py::class_<SystemTB, std::shared_ptr<SystemTB>>(m, "SystemTB")
         .def(py::init<const SystemConfiguration&>())
