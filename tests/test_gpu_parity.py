@@ -170,6 +170,44 @@ def test_split_product_scaling(name, request):
     ctx.close()
 
 
+@pytest.mark.parametrize("name,nstates", [("hbn20", 4), ("random18", 8)])
+def test_block_lower_fp32_build(name, nstates, request):
+    # without a copy-out the FP32 build keeps only the block-lower part that the split product
+    # reads; a solve on the FP32 matrix cores and a later copy-out complete the rest from it
+    ex, H = get_case(name, request)
+    wr = np.linalg.eigvalsh(H)[:nstates]
+    ctx = gpu_context(ex)
+    ctx.build(1, scissor=ex.scissor)
+    rng = np.random.default_rng(23)
+    x = rng.normal(size=(ex.N, 5)) + 1j * rng.normal(size=(ex.N, 5))
+    assert rel_err(ctx.matvec(x, precision=1), H @ x) < 1e-5
+    w, v, rep = ctx.solve(method=1, nstates=nstates, precision=1, tol=1e-6, fp32_arith=0)
+    assert np.abs(w - wr).max() < 1e-4
+    w, v, rep = ctx.solve(method=1, nstates=nstates, precision=1, tol=1e-6, fp32_arith=1)
+    assert np.abs(w - wr).max() < 1e-4
+    Hg = ctx.get_bse(1).astype(np.complex128)
+    assert np.abs(Hg - H).max() < 2e-6 * max(1.0, np.abs(H).max())
+    assert np.array_equal(Hg, Hg.conj().T)
+    ctx.close()
+
+
+def test_split_product_blocks_and_chunks():
+    # N = 5184: 21 row blocks of 256 (the last one partial) and 3 column chunks of 2048 (the
+    # last one ragged), so every kind of work item of the split product (diagonal block,
+    # diagonal chunk, full and ragged chunks, transposed blocks) against the FP64 product
+    ex = oracle_case(HBN, 72, 1, 0, (1, 1, 10), exchange=True)
+    ctx = gpu_context(ex)
+    ctx.build(0)
+    ctx.build(1)
+    rng = np.random.default_rng(17)
+    x = rng.normal(size=(ex.N, 37)) + 1j * rng.normal(size=(ex.N, 37))
+    y0 = ctx.matvec(x, precision=0)
+    y1 = ctx.matvec(x, precision=1)
+    for j in range(37):
+        assert rel_err(y1[:, j], y0[:, j]) < 1e-5, j
+    ctx.close()
+
+
 MF_FORMS = [1, 2]  # XATU_MATRIX_FREE_CONV (k-space convolution), XATU_MATRIX_FREE_TILES (recomputed tiles)
 
 

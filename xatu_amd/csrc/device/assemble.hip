@@ -204,7 +204,8 @@ __global__ __launch_bounds__(TK* TK) void bse_tile_kernel(LatticeDesc lat, int n
                                                           const double2* __restrict__ ft,
                                                           const double2* __restrict__ e1,
                                                           const double2* __restrict__ e2, int exchange,
-                                                          double scissor, cx_t<R>* __restrict__ H) {
+                                                          double scissor, int lower_rows,
+                                                          cx_t<R>* __restrict__ H) {
     using C = cx_t<R>;
     constexpr int NB = NC + NV;
     constexpr int NCV = NC * NV;
@@ -212,6 +213,13 @@ __global__ __launch_bounds__(TK* TK) void bse_tile_kernel(LatticeDesc lat, int n
     C* s_coef = reinterpret_cast<C*>(smem_raw);
     const int norb = lat.norb;
     const int Kr = blockIdx.x, Kc = blockIdx.y;
+    // block-lower build: skip tiles whose last row lies above the row block of their first
+    // column (no entry of theirs is read)
+    if (lower_rows > 0) {
+        const int64_t last_row = static_cast<int64_t>(Kr + 1) * TK * NCV - 1;
+        const int64_t first_col = static_cast<int64_t>(Kc) * TK * NCV;
+        if (last_row < first_col / lower_rows * lower_rows) return;
+    }
     stage_coefficients<R, NC, NV, TK>(s_coef, norb, nk, Kr, Kc, evk, evkq, TK * TK);
     __syncthreads();
 
@@ -550,17 +558,17 @@ struct Shape {
     static bool matches(const Context& c) { return c.lat.natoms == NA && c.nc == NC && c.nv == NV; }
 
     template <int TK>
-    static void build_tk(const Context& c, cx_t<R>* H, hipStream_t s) {
+    static void build_tk(const Context& c, cx_t<R>* H, int lower_rows, hipStream_t s) {
         const int nT = (c.nk + TK - 1) / TK;
         const size_t lds = coef_bytes<R, TK>(c, NB);
         hipLaunchKernelGGL((bse_tile_kernel<R, NA, NC, NV, TK>), dim3(nT, nT), dim3(TK * TK), lds, s, c.lat, c.nk, c.d_evec_k.p, c.d_evec_kq.p, c.d_eval_k.p, c.d_eval_kq.p, c.d_kcoord.p,
-                           c.d_ft.p, c.d_e1.p, c.d_e2.p, c.inter.exchange, c.scissor, H);
+                           c.d_ft.p, c.d_e1.p, c.d_e2.p, c.inter.exchange, c.scissor, lower_rows, H);
     }
-    static void build(const Context& c, cx_t<R>* H, hipStream_t s) {
+    static void build(const Context& c, cx_t<R>* H, hipStream_t s, int lower_rows = 0) {
         if (coef_bytes<R, 16>(c, NB) <= 64 * 1024) {
-            build_tk<16>(c, H, s);
+            build_tk<16>(c, H, lower_rows, s);
         } else if (coef_bytes<R, 8>(c, NB) <= 160 * 1024) {
-            build_tk<8>(c, H, s);
+            build_tk<8>(c, H, lower_rows, s);
         } else {
             throw Error(XATU_ERR_INVALID_ARGUMENT, "too many orbitals for the tile kernel");
         }
@@ -691,8 +699,9 @@ void launch_build_tiles64(const Context& c, double2* H, hipStream_t s) {
 static const char* kShapeMsg =
     "this operation supports natoms <= 4 and nv == nc <= 3 (the FP64 build covers other shapes)";
 
-void launch_build_tiles32(const Context& c, float2* H, hipStream_t s) {
-    if (!for_shape<float>(c, [&](auto sh) { decltype(sh)::build(c, H, s); })) {
+void launch_build_tiles32(const Context& c, float2* H, bool lower, hipStream_t s) {
+    const int lower_rows = lower ? kSplitRowBlock : 0;
+    if (!for_shape<float>(c, [&](auto sh) { decltype(sh)::build(c, H, s, lower_rows); })) {
         throw Error(XATU_ERR_INVALID_ARGUMENT, kShapeMsg);
     }
 }

@@ -444,7 +444,11 @@ int xatu_build_bse(xatu_ctx* ctx, int precision, double scissor, void* hbs_out) 
             // the last column (multiplied by zero vector padding)
             c.d_H32.alloc(nn + 32);
             XATU_HIP(hipMemsetAsync(c.d_H32.p + nn, 0, 32 * sizeof(float2), c.stream));
-            launch_build_tiles32(c, c.d_H32.p, c.stream);
+            // only the block-lower part when the split product will read it (and nothing is
+            // copied out); completed on demand for the FP32 matrix-core product and copies
+            c.h32_lower = !hbs_out && split_layout_available(c);
+            if (c.h32_lower) clear_upper_heads(c);
+            launch_build_tiles32(c, c.d_H32.p, c.h32_lower, c.stream);
         }
         evk.stop();
         ev.stop();
@@ -495,6 +499,7 @@ int xatu_get_bse(xatu_ctx* ctx, int precision, void* hbs_out) {
             XATU_HIP(hipMemcpy(hbs_out, c.d_H64.p, nn * sizeof(double2), hipMemcpyDeviceToHost));
         } else {
             if (!c.built32) throw Error(XATU_ERR_LOGIC, "FP32 BSE matrix is not resident");
+            complete_matrix32(c);
             XATU_HIP(hipMemcpy(hbs_out, c.d_H32.p, nn * sizeof(float2), hipMemcpyDeviceToHost));
         }
     });
@@ -652,6 +657,7 @@ int xatu_compare_builds(xatu_ctx* ctx, int kblock, double* block_err, double* ma
     return guarded(ctx, [&](Context& c) {
         if (!c.built64 || !c.built32) throw Error(XATU_ERR_LOGIC, "both FP64 and FP32 matrices must be resident");
         if (kblock < 1) throw Error(XATU_ERR_INVALID_ARGUMENT, "kblock must be positive");
+        complete_matrix32(c);
         const int rows = kblock * c.nc * c.nv;
         const int nblk = static_cast<int>((c.N + rows - 1) / rows);
         DevBuf<double> d;

@@ -147,7 +147,13 @@ struct Context {
     // scratch for products and projections
     DevBuf<double2> s_xt64, s_yp64;   // staged vectors / split partial sums, FP64 dense product
     DevBuf<float2> s_xt32, s_yp32;    // same, FP32 dense product (partial sums also of the split product)
+    bool h32_lower = false;           // the FP32 matrix holds only its block-lower part (split product)
     DevBuf<float4> s_xs16;            // vectors staged as split-FP16 fragments
+    DevBuf<float4> s_xt16;            // same, in the row order of the transposed-block products
+    DevBuf<float2> s_pt32;            // partial sums of the transposed-block products (split product)
+    DevBuf<int2> d_herm_items;        // (row block, column chunk) work items of the split product
+    int herm_items = 0;
+    int64_t herm_n = 0;               // matrix dimension the work items were laid out for
     DevBuf<float> s_colmax, s_xscale; // column maxima and power-of-two scales of the staged vectors
     DevBuf<double> s_xinv;            // inverse combined scales applied by the split-product reduction
     float h16_scale = 0.0f;           // power-of-two scale of H for the split product (0: unavailable)
@@ -205,7 +211,16 @@ void launch_fourier_q(const Context& c, hipStream_t s);
 void launch_reciprocal_table(const Context& c, const double* d_gvec, int ng, double pref, hipStream_t s);
 void launch_exchange_factors(const Context& c, hipStream_t s);
 void launch_build_tiles64(const Context& c, double2* H, hipStream_t s);
-void launch_build_tiles32(const Context& c, float2* H, hipStream_t s);
+// lower: only the tiles holding entries (k, i) with k >= kSplitRowBlock floor(i / kSplitRowBlock)
+// (the block-lower part of column-major H read by the split product)
+void launch_build_tiles32(const Context& c, float2* H, bool lower, hipStream_t s);
+// Fills the block-upper part of a FP32 matrix built block-lower (Hermitian mirror).
+void complete_matrix32(Context& c);
+// Before a block-lower build: zeroes the never-built entries that the split product reads
+// past the end of a column.
+void clear_upper_heads(Context& c);
+// The split product's layout is usable for the current operator (even N, finite scale).
+bool split_layout_available(const Context& c);
 void launch_build_basis(const Context& c, int nbasis, const int* d_basis, double2* H, hipStream_t s);
 void launch_diag_extract(const Context& c, const double2* H, double* diag, hipStream_t s);
 void launch_diag_direct(const Context& c, double* diag, hipStream_t s);

@@ -344,6 +344,7 @@ void launch_matvec32(Context& c, const double2* X, double2* Y, int nvec, hipStre
         launch_matvec_split16(c, X, Y, nvec, s);
         return;
     }
+    complete_matrix32(c);
     dense_product<double, float, double>(c, c.d_H32.p, X, Y, nvec, s, c.s_xt32, c.s_yp32);
 }
 

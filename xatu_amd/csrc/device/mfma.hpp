@@ -39,6 +39,10 @@ __device__ inline f32x4 mma_f16(f16x8 a, f16x8 b, f32x4 c) {
 // Vectors per product launch (two 16-column MFMA tiles).
 constexpr int kBlockVectors = 32;
 
+// Rows per workgroup of the split product; with the FP32 matrix built block-lower, column i
+// holds rows k >= kSplitRowBlock floor(i / kSplitRowBlock) only.
+constexpr int kSplitRowBlock = 256;
+
 // Number of XCDs of the MI355X; consecutive workgroup ids are dealt round-robin to them.
 constexpr int kXcds = 8;
 

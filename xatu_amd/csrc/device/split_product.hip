@@ -25,7 +25,9 @@
 // last step of the last split may read up to 31 entries past the end of a column: the
 // staged vectors are zero there and the matrix allocation carries 32 zero entries of
 // padding past its end.
+#include <algorithm>
 #include <cmath>
+#include <vector>
 
 #include "context.hpp"
 #include "cplx.hpp"
@@ -36,7 +38,6 @@ namespace xatu_dev {
 namespace {
 
 constexpr int kVec = kBlockVectors;
-constexpr int kSplitTiles = 8;      // 16-row tiles per wave: 128 rows per wave, 512 per workgroup
 constexpr int kMaxChunks = 64;      // row chunks of the column-maximum pass
 
 struct XFrag16 {
@@ -114,32 +115,48 @@ __global__ void vector_scale_kernel(const float* __restrict__ partial, int chunk
     inv[n] = 1.0 / (static_cast<double>(sc) * static_cast<double>(sH));
 }
 
+__device__ inline void split_entry(const double2* __restrict__ X, int64_t N, int nvec, int n, float sc, int64_t k,
+                                   int e, XFrag16& f) {
+    float c = 0.0f, d = 0.0f;
+    if (n < nvec && k < N) {
+        const double2 v = X[k + n * N];
+        c = static_cast<float>(v.x * sc);
+        d = static_cast<float>(v.y * sc);
+    }
+    const _Float16 ch = static_cast<_Float16>(c), dh = static_cast<_Float16>(d);
+    f.ch[e] = ch;
+    f.cl[e] = static_cast<_Float16>(c - static_cast<float>(ch));
+    f.dh[e] = dh;
+    f.dl[e] = static_cast<_Float16>(d - static_cast<float>(dh));
+}
+
 // Xs[(step 32 + n) 4 + g]: element e = 2 q + r of the fragment holds k = 32 step + 8 q + 2 g + r.
+// Xt[((G 2 + jj) 32 + n) 4 + g]: element e holds row 64 G + 32 jj + 16 (e & 1) + 4 g + (e >> 1).
+// Rows past N are zero.
 __global__ void stage_split_kernel(const double2* __restrict__ X, int64_t N, int nvec, int64_t nsteps,
-                                   const float* __restrict__ s, XFrag16* __restrict__ Xs) {
+                                   int64_t ngroups, const float* __restrict__ s, XFrag16* __restrict__ Xs,
+                                   XFrag16* __restrict__ Xt) {
     const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (idx >= nsteps * kVec * 4) return;
+    const int64_t ns = nsteps * kVec * 4;
+    if (idx >= ns + ngroups * 2 * kVec * 4) return;
     const int g = static_cast<int>(idx & 3);
     const int n = static_cast<int>((idx >> 2) & (kVec - 1));
-    const int64_t step = idx >> 7;
     const float sc = n < nvec ? s[n] : 0.0f;
     XFrag16 f;
+    if (idx < ns) {
+        const int64_t step = idx >> 7;
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-        const int64_t k = 32 * step + 8 * (e >> 1) + 2 * g + (e & 1);
-        float c = 0.0f, d = 0.0f;
-        if (n < nvec && k < N) {
-            const double2 v = X[k + n * N];
-            c = static_cast<float>(v.x * sc);
-            d = static_cast<float>(v.y * sc);
-        }
-        const _Float16 ch = static_cast<_Float16>(c), dh = static_cast<_Float16>(d);
-        f.ch[e] = ch;
-        f.cl[e] = static_cast<_Float16>(c - static_cast<float>(ch));
-        f.dh[e] = dh;
-        f.dl[e] = static_cast<_Float16>(d - static_cast<float>(dh));
+        for (int e = 0; e < 8; ++e) split_entry(X, N, nvec, n, sc, 32 * step + 8 * (e >> 1) + 2 * g + (e & 1), e, f);
+        Xs[idx] = f;
+    } else {
+        const int64_t it = idx - ns;
+        const int jj = static_cast<int>((it >> 7) & 1);
+        const int64_t G = it >> 8;
+#pragma unroll
+        for (int e = 0; e < 8; ++e)
+            split_entry(X, N, nvec, n, sc, 64 * G + 32 * jj + 16 * (e & 1) + 4 * g + (e >> 1), e, f);
+        Xt[it] = f;
     }
-    Xs[idx] = f;
 }
 
 __device__ inline HFrag16 split_h(const float4 (&r)[4], float sH) {
@@ -161,71 +178,271 @@ __device__ inline HFrag16 split_h(const float4 (&r)[4], float sH) {
     return f;
 }
 
-// 1-D grid of row_blocks * splits workgroups, block 256; split = id % splits (splits a
-// multiple of 8). Partial sums (scaled) to Yp[split][row][32].
-template <int TW>
-__global__ __launch_bounds__(256, 1) void split_product_kernel(const float2* __restrict__ H,
-                                                               const XFrag16* __restrict__ Xs, int64_t N,
-                                                               int64_t kchunk, int splits, float sH,
-                                                               float2* __restrict__ Yp) {
-    const int sp = blockIdx.x % splits;
-    const int rb = blockIdx.x / splits;
+// Work items of the product: (row block b, column chunk c) pairs with c >= the chunk
+// holding the diagonal block of b, so that only the lower block triangle of H (and the
+// diagonal blocks) is read. One workgroup of 4 waves owns the kHermRows rows of a row
+// block, 16 kHermTiles rows per wave.
+constexpr int kHermTiles = 4;
+constexpr int kHermRows = 4 * 16 * kHermTiles;
+static_assert(kHermRows == kSplitRowBlock, "row block of the split product");
+constexpr int kHermChunk = 2048;   // columns per work item, a multiple of kHermRows
+constexpr int kTrRow = 72;         // FP16 entries per row of the transposition buffer (64 + 8 padding)
+constexpr int kCbRow = 34;         // complex entries per row of the combine buffer (32 + 2 padding)
+
+struct HermLds {
+    _Float16 tr[4][4][32 * kTrRow];  // per wave, per part (a hi, a lo, b hi, b lo): [rho][t / 2][li][t & 1]
+    float2 cb[2][4][32][kCbRow];     // per slot (step parity), per wave: transposed-block partial sums
+};
+
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+
+// Orders the LDS accesses of one wave (written by some lanes, read by others) without
+// waiting for its global loads.
+__device__ inline void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// One work item (b, c), grid = number of items, block 256.
+//  Direct blocks: y_i += sum_k conj(H(k, i)) x_k for the rows i of b and the columns k of
+//  the item (k >= first row of b), read down the columns of H as in the full product;
+//  partial sums to Pn[c][i][32].
+//  Transposed blocks: for k past the diagonal block, y_k += sum_{i in b} H(k, i) x_i with the
+//  same loaded entries. Each wave's split FP16 parts go through its LDS buffer so that a
+//  lane holds one k and eight i (rho = 8 q + 4 r + g orders the k of a step); the four waves'
+//  sums over their 64 rows are combined in LDS and stored per step to Pt[b][k][32].
+// Xt holds the vectors in the i order of the transposed fragments: element e of lane
+// (n, g) of fragment jj of the wave's 64 rows holds row 32 jj + 16 (e & 1) + 4 g + (e >> 1).
+template <int TW, bool kPipe>
+__global__ __launch_bounds__(256, 1) void herm_product_kernel(const float2* __restrict__ H,
+                                                              const XFrag16* __restrict__ Xs,
+                                                              const XFrag16* __restrict__ Xt,
+                                                              const int2* __restrict__ items, int64_t N, float sH,
+                                                              float2* __restrict__ Pn, float2* __restrict__ Pt) {
+    static_assert(TW == 4, "the transposed fragments assume 64 rows per wave");
+    __shared__ HermLds lds;
+    const int2 bc = items[blockIdx.x];
+    const int b = bc.x, c = bc.y;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int li = lane & 15, g = lane >> 4;
-    const int64_t row0 = static_cast<int64_t>(rb) * (64 * TW) + wave * (16 * TW);
-    const int64_t kbeg = static_cast<int64_t>(sp) * kchunk;
-    const int64_t kend = min(N, kbeg + kchunk);
-    const int64_t nsteps = kend > kbeg ? (kend - kbeg + 31) / 32 : 0;
-    f32x4 acc[TW][2][3];
+    const int64_t rb0 = static_cast<int64_t>(b) * kHermRows;
+    const int64_t row0 = rb0 + wave * (16 * TW);
+    const int64_t cbeg = static_cast<int64_t>(c) * kHermChunk;
+    const int64_t kend = min(N, cbeg + kHermChunk);
+    const bool diag = cbeg <= rb0;
+    const int64_t kbeg = diag ? rb0 : cbeg;
+    const int nd = diag ? static_cast<int>((min(kend, rb0 + kHermRows) - rb0 + 31) / 32) : 0;
+    const int64_t klow = diag ? rb0 + kHermRows : cbeg;
+    const int nl = kend > klow ? static_cast<int>((kend - klow + 31) / 32) : 0;
+    const int nsteps = nd + nl;
+
+    f32x4 acc[TW][2][2];
+    f32x4 acct[2][2][2];
 #pragma unroll
     for (int t = 0; t < TW; ++t)
 #pragma unroll
-        for (int u = 0; u < 2; ++u)
+        for (int u = 0; u < 2; ++u) acc[t][u][0] = acc[t][u][1] = f32x4{0, 0, 0, 0};
 #pragma unroll
-            for (int a = 0; a < 3; ++a) acc[t][u][a] = f32x4{0, 0, 0, 0};
-    // rows past N are clamped to row N-1 (their outputs are never stored)
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int u = 0; u < 2; ++u) acct[h][u][0] = acct[h][u][1] = f32x4{0, 0, 0, 0};
+    // rows past N are clamped to row N-1 (their outputs are never stored, their vector
+    // entries are staged as zero)
     const float4* hcol[TW];
 #pragma unroll
     for (int t = 0; t < TW; ++t) {
         const int64_t i = row0 + 16 * t + li;
-        hcol[t] = reinterpret_cast<const float4*>(H + (i < N ? i : N - 1) * N + kbeg + 2 * g);
+        hcol[t] = reinterpret_cast<const float4*>(H + (i < N ? i : N - 1) * N) + g;
     }
-    const XFrag16* xb = Xs + ((kbeg / 32) * kVec + li) * 4 + g;
-    for (int64_t st = 0; st < nsteps; ++st) {
-        float4 raw[TW][4];
-        XFrag16 xf[2];
+    XFrag16 xi[2][2];
+    {
+        const int64_t G = row0 / 64;
+#pragma unroll
+        for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+            for (int u = 0; u < 2; ++u) xi[jj][u] = Xt[((G * 2 + jj) * kVec + 16 * u + li) * 4 + g];
+    }
+    _Float16* trw = &lds.tr[wave][0][0];
+    // two steps of H in flight: step st + 2 is loaded into the registers of step st as soon
+    // as these are split
+    auto load = [&](float4 (&rb)[TW][4], XFrag16 (&xb)[2], int st) {
+        const int64_t k0 = st < nd ? kbeg + 32 * st : klow + 32 * (st - nd);
 #pragma unroll
         for (int t = 0; t < TW; ++t)
 #pragma unroll
-            for (int q = 0; q < 4; ++q) raw[t][q] = hcol[t][st * 16 + 4 * q];
+            for (int q = 0; q < 4; ++q) rb[t][q] = hcol[t][k0 / 2 + 4 * q];
 #pragma unroll
-        for (int u = 0; u < 2; ++u) xf[u] = xb[(st * kVec + 16 * u) * 4];
+        for (int u = 0; u < 2; ++u) xb[u] = Xs[((k0 / 32) * kVec + 16 * u + li) * 4 + g];
+    };
+    // sums of one transposed step over the four waves, in wave order; wave w stores rows
+    // 8 w .. 8 w + 7 of the step
+    auto combine = [&](int slot, int64_t kc) {
+        float2* out = Pt + static_cast<int64_t>(b) * N * kVec;
+        float2 part[4][4];
 #pragma unroll
-        for (int t = 0; t < TW; ++t) {
-            const HFrag16 h = split_h(raw[t], sH);
+        for (int it = 0; it < 4; ++it) {
+            const int f = lane + 64 * it;
+            const int row = 8 * wave + (f >> 5), n = f & 31;
 #pragma unroll
-            for (int u = 0; u < 2; ++u) {
-                const XFrag16& x = xf[u];
-                f32x4 re = acc[t][u][0], ad = acc[t][u][1], bc = acc[t][u][2];
-                re = mma_f16(h.al, x.ch, re);
-                re = mma_f16(h.ah, x.cl, re);
-                re = mma_f16(h.ah, x.ch, re);
-                re = mma_f16(h.bl, x.dh, re);
-                re = mma_f16(h.bh, x.dl, re);
-                re = mma_f16(h.bh, x.dh, re);
-                ad = mma_f16(h.al, x.dh, ad);
-                ad = mma_f16(h.ah, x.dl, ad);
-                ad = mma_f16(h.ah, x.dh, ad);
-                bc = mma_f16(h.bl, x.ch, bc);
-                bc = mma_f16(h.bh, x.cl, bc);
-                bc = mma_f16(h.bh, x.ch, bc);
-                acc[t][u][0] = re;
-                acc[t][u][1] = ad;
-                acc[t][u][2] = bc;
+            for (int w = 0; w < 4; ++w) part[it][w] = lds.cb[slot][w][row][n];
+        }
+#pragma unroll
+        for (int it = 0; it < 4; ++it) {
+            const int f = lane + 64 * it;
+            const int row = 8 * wave + (f >> 5), n = f & 31;
+            float2 sum = part[it][0];
+#pragma unroll
+            for (int w = 1; w < 4; ++w) {
+                sum.x += part[it][w].x;
+                sum.y += part[it][w].y;
+            }
+            const int64_t k = kc + row;
+            if (k < N) out[k * kVec + n] = sum;
+        }
+    };
+    bool pend = false;
+    int pend_slot = 0;
+    int64_t pend_k0 = 0;
+    // body(st) with st == nsteps (odd step counts) only reloads; loads past the last step
+    // repeat the last step, so that every register set is reloaded unconditionally
+    auto body = [&](float4 (&rb)[TW][4], XFrag16 (&xb)[2], int st) {
+        const bool valid = st < nsteps;
+        const bool low = valid && st >= nd;
+        const int64_t k0 = st < nd ? kbeg + 32 * st : klow + 32 * (st - nd);
+        const int sn = min(st + 2, nsteps - 1);
+        const int64_t kn = sn < nd ? kbeg + 32 * sn : klow + 32 * (sn - nd);
+        if (low) wave_lds_sync();
+        // tiles in pairs: split, reload their registers with step st + 2, store the parts to
+        // the transposition buffer, direct-block products
+#pragma unroll
+        for (int tp = 0; tp < TW / 2; ++tp) {
+            HFrag16 hf[2];
+#pragma unroll
+            for (int j = 0; j < 2; ++j) hf[j] = split_h(rb[2 * tp + j], sH);
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) rb[2 * tp + j][q] = hcol[2 * tp + j][kn / 2 + 4 * q];
+            if (low) {
+                // element e = 2 q + r of a fragment is k = 8 q + 2 g + r, row rho = 8 q + 4 r + g;
+                // a row holds [tile pair][li][tile & 1]
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const int rho = 8 * (e >> 1) + 4 * (e & 1) + g;
+                    _Float16* dst = trw + rho * kTrRow + 32 * tp + 2 * li;
+                    *reinterpret_cast<f16x2*>(dst) = f16x2{hf[0].ah[e], hf[1].ah[e]};
+                    *reinterpret_cast<f16x2*>(dst + 32 * kTrRow) = f16x2{hf[0].al[e], hf[1].al[e]};
+                    *reinterpret_cast<f16x2*>(dst + 64 * kTrRow) = f16x2{hf[0].bh[e], hf[1].bh[e]};
+                    *reinterpret_cast<f16x2*>(dst + 96 * kTrRow) = f16x2{hf[0].bl[e], hf[1].bl[e]};
+                }
+            }
+            // direct blocks: A = conj(H(k, i)) = a - i b, x = c + i d:
+            // Re y += a c + b d, Im y += a d + (-b) c
+#pragma unroll
+            for (int j = 0; j < 2 && valid; ++j) {
+                const int t = 2 * tp + j;
+                const HFrag16& h = hf[j];
+                const f16x8 nbh = -h.bh, nbl = -h.bl;
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    const XFrag16& x = xb[u];
+                    f32x4 re = acc[t][u][0], im = acc[t][u][1];
+                    re = mma_f16(h.al, x.ch, re);
+                    re = mma_f16(h.ah, x.cl, re);
+                    re = mma_f16(h.ah, x.ch, re);
+                    re = mma_f16(h.bl, x.dh, re);
+                    re = mma_f16(h.bh, x.dl, re);
+                    re = mma_f16(h.bh, x.dh, re);
+                    im = mma_f16(h.al, x.dh, im);
+                    im = mma_f16(h.ah, x.dl, im);
+                    im = mma_f16(h.ah, x.dh, im);
+                    im = mma_f16(nbl, x.ch, im);
+                    im = mma_f16(nbh, x.cl, im);
+                    im = mma_f16(nbh, x.ch, im);
+                    acc[t][u][0] = re;
+                    acc[t][u][1] = im;
+                }
             }
         }
+#pragma unroll
+        for (int u = 0; u < 2; ++u) xb[u] = Xs[((kn / 32) * kVec + 16 * u + li) * 4 + g];
+        if (kPipe && pend && low) {
+            __syncthreads();
+            combine(pend_slot, pend_k0);
+            pend = false;
+        }
+        if (low) {
+            wave_lds_sync();
+            // transposed blocks: lane (m, g) holds row k(rho = 16 h + m) and the eight i of
+            // fragment jj (tiles 2 jj, 2 jj + 1, li = 4 g .. 4 g + 3); A = H(k, i) = a + i b:
+            // Re y += a c + (-b) d, Im y += a d + b c
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+#pragma unroll
+                for (int jj = 0; jj < 2; ++jj) {
+                    const _Float16* src = trw + (16 * h + li) * kTrRow + 32 * jj + 8 * g;
+                    const f16x8 ah = *reinterpret_cast<const f16x8*>(src);
+                    const f16x8 al = *reinterpret_cast<const f16x8*>(src + 32 * kTrRow);
+                    const f16x8 bh = *reinterpret_cast<const f16x8*>(src + 64 * kTrRow);
+                    const f16x8 bl = *reinterpret_cast<const f16x8*>(src + 96 * kTrRow);
+                    const f16x8 nbh = -bh, nbl = -bl;
+#pragma unroll
+                    for (int u = 0; u < 2; ++u) {
+                        const XFrag16& x = xi[jj][u];
+                        f32x4 re = acct[h][u][0], im = acct[h][u][1];
+                        re = mma_f16(al, x.ch, re);
+                        re = mma_f16(ah, x.cl, re);
+                        re = mma_f16(ah, x.ch, re);
+                        re = mma_f16(nbl, x.dh, re);
+                        re = mma_f16(nbh, x.dl, re);
+                        re = mma_f16(nbh, x.dh, re);
+                        im = mma_f16(al, x.dh, im);
+                        im = mma_f16(ah, x.dl, im);
+                        im = mma_f16(ah, x.dh, im);
+                        im = mma_f16(bl, x.ch, im);
+                        im = mma_f16(bh, x.cl, im);
+                        im = mma_f16(bh, x.ch, im);
+                        acct[h][u][0] = re;
+                        acct[h][u][1] = im;
+                    }
+                }
+            // output row m = 4 g + v of fragment h is rho = 16 h + m, i.e. k = k0 + kl below
+            const int slot = (st - nd) & 1;
+            float2* cbw = &lds.cb[slot][wave][0][0];
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+#pragma unroll
+                for (int u = 0; u < 2; ++u)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int kl = 16 * h + 8 * (g >> 1) + (g & 1) + 2 * v;
+                        cbw[kl * kCbRow + 16 * u + li] = make_float2(acct[h][u][0][v], acct[h][u][1][v]);
+                        acct[h][u][0][v] = 0.0f;
+                        acct[h][u][1][v] = 0.0f;
+                    }
+            if (!kPipe) {
+                __syncthreads();
+                combine(slot, k0);
+            } else {
+                pend = true;
+                pend_slot = slot;
+                pend_k0 = k0;
+            }
+        }
+    };
+    float4 bufa[TW][4], bufb[TW][4];
+    XFrag16 xa[2], xbb[2];
+    load(bufa, xa, 0);
+    load(bufb, xbb, min(1, nsteps - 1));
+    for (int st = 0; st < nsteps; st += 2) {
+        body(bufa, xa, st);
+        body(bufb, xbb, st + 1);
     }
-    float2* out = Yp + static_cast<int64_t>(sp) * N * kVec;
+    if (pend) {
+        __syncthreads();
+        combine(pend_slot, pend_k0);
+    }
+    float2* out = Pn + static_cast<int64_t>(c) * N * kVec;
 #pragma unroll
     for (int t = 0; t < TW; ++t)
 #pragma unroll
@@ -234,13 +451,109 @@ __global__ __launch_bounds__(256, 1) void split_product_kernel(const float2* __r
             if (r < N) {
 #pragma unroll
                 for (int u = 0; u < 2; ++u)
-                    out[r * kVec + 16 * u + li] =
-                        make_float2(acc[t][u][0][v], acc[t][u][1][v] - acc[t][u][2][v]);
+                    out[r * kVec + 16 * u + li] = make_float2(acc[t][u][0][v], acc[t][u][1][v]);
             }
         }
 }
 
+// Y[r][n] (column-major, FP64) = scale[n] (sum over the items' direct partial sums of r and
+// the transposed partial sums of r from the row blocks above it); 64 rows per workgroup
+// (all in one row block), fixed summation order.
+__global__ __launch_bounds__(256) void herm_reduce_kernel(const float2* __restrict__ Pn,
+                                                          const float2* __restrict__ Pt, int64_t N, int nvec,
+                                                          int nchunks, const double* __restrict__ scale,
+                                                          double2* __restrict__ Y) {
+    constexpr int kRows = 64;
+    __shared__ double2 tile[kRows][kVec + 1];
+    const int64_t r0 = static_cast<int64_t>(blockIdx.x) * kRows;
+    const int b = static_cast<int>(r0 / kHermRows);
+    const int c0 = static_cast<int>(static_cast<int64_t>(b) * kHermRows / kHermChunk);
+#pragma unroll
+    for (int j = 0; j < kRows * kVec / 256; ++j) {
+        const int f = threadIdx.x + 256 * j;
+        const int rr = f / kVec, n = f % kVec;
+        const int64_t r = r0 + rr;
+        double sr = 0.0, si = 0.0;
+        if (r < N && n < nvec) {
+            // terms: the direct partial sums of chunks c0.., then the transposed ones of row
+            // blocks 0..b-1, summed in that order in eight interleaved partial sums
+            const int nc = nchunks - c0, nt = nc + b;
+            const int64_t off = r * kVec + n, stride = N * kVec;
+            const float2* pn = Pn + static_cast<int64_t>(c0) * stride + off;
+            const float2* pt = Pt + off;
+            double ar[8] = {0, 0, 0, 0, 0, 0, 0, 0}, ai[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            int t = 0;
+            for (; t + 8 <= nt; t += 8) {
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const int u = t + j;
+                    const float2 v = u < nc ? pn[u * stride] : pt[(u - nc) * stride];
+                    ar[j] += v.x;
+                    ai[j] += v.y;
+                }
+            }
+            for (int j = 0; t < nt; ++t, ++j) {
+                const float2 v = t < nc ? pn[t * stride] : pt[(t - nc) * stride];
+                ar[j] += v.x;
+                ai[j] += v.y;
+            }
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                sr += ar[j];
+                si += ai[j];
+            }
+            sr *= scale[n];
+            si *= scale[n];
+        }
+        tile[rr][n] = make_double2(sr, si);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kRows * kVec / 256; ++j) {
+        const int f = threadIdx.x + 256 * j;
+        const int rr = f % kRows, n = f / kRows;
+        const int64_t r = r0 + rr;
+        if (r < N && n < nvec) Y[r + static_cast<int64_t>(n) * N] = tile[rr][n];
+    }
+}
+
+// Zeroes rows 0..31 of the columns i >= kSplitRowBlock, which a block-lower build leaves
+// unwritten: the last step of a column reads up to 31 entries into the next column, where
+// they meet zero vector entries and must be finite.
+__global__ void zero_heads_kernel(float2* __restrict__ H, int64_t N) {
+    const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= (N - kSplitRowBlock) * 32) return;
+    const int64_t i = kSplitRowBlock + idx / 32;
+    H[i * N + idx % 32] = make_float2(0.0f, 0.0f);
+}
+
+// Fills H(k, i) = conj(H(i, k)) for the rows k < kSplitRowBlock floor(i / kSplitRowBlock) of
+// column i (the block-upper part left out by a block-lower build), 64 x 64 tiles through
+// LDS; grid (tiles, tiles), block 256, tiles entirely outside that part exit.
+__global__ __launch_bounds__(256) void mirror_upper_kernel(float2* __restrict__ H, int64_t N) {
+    __shared__ float2 tile[64][65];
+    const int64_t k0 = static_cast<int64_t>(blockIdx.x) * 64, i0 = static_cast<int64_t>(blockIdx.y) * 64;
+    const int64_t thr0 = i0 / kSplitRowBlock * kSplitRowBlock;
+    if (k0 >= thr0) return;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    // source: H(i, k) for i in the column tile, k in the row tile, read down column k
+    for (int r = ty; r < 64; r += 4) {
+        const int64_t k = k0 + r, i = i0 + tx;
+        if (k < N && i < N) tile[r][tx] = H[k * N + i];
+    }
+    __syncthreads();
+    for (int r = ty; r < 64; r += 4) {
+        const int64_t i = i0 + r, k = k0 + tx;
+        if (i < N && k < N && k < i / kSplitRowBlock * kSplitRowBlock) {
+            const float2 v = tile[tx][r];
+            H[i * N + k] = make_float2(v.x, -v.y);
+        }
+    }
+}
+
 }  // namespace
+
+static void lay_out_items(Context& c);
 
 void split_product_scale(Context& c) {
     const size_t nft = static_cast<size_t>(c.lat.natoms) * c.lat.natoms * c.lat.ntab;
@@ -271,43 +584,92 @@ void split_product_scale(Context& c) {
     if (bound > 0.0 && std::isfinite(bound)) std::frexp(bound, &e);
     c.h16_scale = (bound > 0.0 && std::isfinite(bound)) ? static_cast<float>(std::ldexp(1.0, 14 - e)) : 0.0f;
     if (!std::isnormal(c.h16_scale)) c.h16_scale = 0.0f;
+    if (c.h16_scale > 0.0f) lay_out_items(c);
+}
+
+// Work items (row block, column chunk) of the split product, the longest first; laid out
+// once per matrix dimension.
+static void lay_out_items(Context& c) {
+    const int64_t N = c.N;
+    if (c.herm_n == N && c.herm_items > 0) return;
+    const int B = static_cast<int>((N + kHermRows - 1) / kHermRows);
+    const int C = static_cast<int>((N + kHermChunk - 1) / kHermChunk);
+    std::vector<int2> it;
+    std::vector<int64_t> len;
+    for (int ch = 0; ch < C; ++ch)
+        for (int b = 0; b < B; ++b) {
+            const int64_t rb0 = static_cast<int64_t>(b) * kHermRows, cb = static_cast<int64_t>(ch) * kHermChunk;
+            if (rb0 >= cb + kHermChunk) continue;
+            it.push_back(make_int2(b, ch));
+            len.push_back(std::min<int64_t>(N, cb + kHermChunk) - std::max(rb0, cb));
+        }
+    std::vector<size_t> order(it.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return len[x] > len[y]; });
+    std::vector<int2> sorted(it.size());
+    for (size_t i = 0; i < order.size(); ++i) sorted[i] = it[order[i]];
+    c.d_herm_items.alloc(sorted.size());
+    XATU_HIP(hipMemcpy(c.d_herm_items.p, sorted.data(), sorted.size() * sizeof(int2), hipMemcpyHostToDevice));
+    c.herm_items = static_cast<int>(sorted.size());
+    c.herm_n = N;
+}
+
+void clear_upper_heads(Context& c) {
+    if (c.N <= kSplitRowBlock) return;
+    const int64_t n = (c.N - kSplitRowBlock) * 32;
+    hipLaunchKernelGGL(zero_heads_kernel, dim3((n + 255) / 256), dim3(256), 0, c.stream, c.d_H32.p, c.N);
+    XATU_HIP(hipGetLastError());
+}
+
+void complete_matrix32(Context& c) {
+    if (!c.built32 || !c.h32_lower) return;
+    const unsigned t = static_cast<unsigned>((c.N + 63) / 64);
+    hipLaunchKernelGGL(mirror_upper_kernel, dim3(t, t), dim3(256), 0, c.stream, c.d_H32.p, c.N);
+    XATU_HIP(hipGetLastError());
+    c.h32_lower = false;
+}
+
+bool split_layout_available(const Context& c) {
+    return c.h16_scale > 0.0f && (c.N % 2) == 0 && c.herm_n == c.N;
 }
 
 bool split_product_available(const Context& c) {
-    return c.built32 && !c.fp32_arith_only && c.h16_scale > 0.0f && (c.N % 2) == 0;
+    return c.built32 && !c.fp32_arith_only && split_layout_available(c);
 }
 
 void launch_matvec_split16(Context& c, const double2* X, double2* Y, int nvec, hipStream_t s) {
-    constexpr int TW = kSplitTiles;
     const int64_t N = c.N;
-    const int rows_wg = 64 * TW;
-    const int row_blocks = static_cast<int>((N + rows_wg - 1) / rows_wg);
-    // about four workgroups per CU (one wave per SIMD each); more splits only add partial sums
-    int splits = std::max(1, std::min(64, (1024 + row_blocks - 1) / row_blocks));
-    splits = (splits + kXcds - 1) / kXcds * kXcds;
-    const int64_t kchunk = ((N + splits - 1) / splits + 31) / 32 * 32;
+    const int B = static_cast<int>((N + kHermRows - 1) / kHermRows);
+    const int C = static_cast<int>((N + kHermChunk - 1) / kHermChunk);
     const int64_t nsteps = (N + 31) / 32;
+    const int64_t ngroups = 4 * static_cast<int64_t>(B);
     const int64_t rows = (N + kMaxChunks - 1) / kMaxChunks;
     const int chunks = static_cast<int>((N + rows - 1) / rows);
-    const size_t xs_bytes = static_cast<size_t>(nsteps) * kVec * 4 * sizeof(XFrag16);
-    c.s_xs16.alloc(std::max(c.s_xs16.n, (xs_bytes + sizeof(float4) - 1) / sizeof(float4)));
-    c.s_yp32.alloc(std::max<size_t>(c.s_yp32.n, static_cast<size_t>(splits) * N * kVec));
+    const size_t xs_n = static_cast<size_t>(nsteps) * kVec * 4 * sizeof(XFrag16) / sizeof(float4);
+    const size_t xt_n = static_cast<size_t>(ngroups) * 2 * kVec * 4 * sizeof(XFrag16) / sizeof(float4);
+    c.s_xs16.alloc(std::max(c.s_xs16.n, xs_n));
+    c.s_xt16.alloc(std::max(c.s_xt16.n, xt_n));
+    c.s_yp32.alloc(std::max<size_t>(c.s_yp32.n, static_cast<size_t>(C) * N * kVec));
+    c.s_pt32.alloc(std::max<size_t>(c.s_pt32.n, static_cast<size_t>(std::max(1, B - 1)) * N * kVec));
     c.s_colmax.alloc(static_cast<size_t>(kVec) * kMaxChunks);
     c.s_xscale.alloc(kVec);
     c.s_xinv.alloc(kVec);
     XFrag16* xs = reinterpret_cast<XFrag16*>(c.s_xs16.p);
+    XFrag16* xt = reinterpret_cast<XFrag16*>(c.s_xt16.p);
     const float sH = c.h16_scale;
+    const int64_t nstage = (nsteps + 2 * ngroups) * kVec * 4;
     for (int v0 = 0; v0 < nvec; v0 += kVec) {
         const int nv = std::min(kVec, nvec - v0);
         const double2* Xb = X + static_cast<int64_t>(v0) * N;
         hipLaunchKernelGGL(column_max_kernel, dim3(chunks, kVec), dim3(256), 0, s, Xb, N, nv, rows, c.s_colmax.p);
         hipLaunchKernelGGL(vector_scale_kernel, dim3(1), dim3(64), 0, s, c.s_colmax.p, chunks, sH, c.s_xscale.p,
                            c.s_xinv.p);
-        hipLaunchKernelGGL(stage_split_kernel, dim3((nsteps * kVec * 4 + 255) / 256), dim3(256), 0, s, Xb, N, nv,
-                           nsteps, c.s_xscale.p, xs);
-        hipLaunchKernelGGL((split_product_kernel<TW>), dim3(row_blocks * splits), dim3(256), 0, s, c.d_H32.p, xs, N,
-                           kchunk, splits, sH, c.s_yp32.p);
-        launch_reduce_partials(c.s_yp32.p, N, nv, splits, Y + static_cast<int64_t>(v0) * N, s, c.s_xinv.p);
+        hipLaunchKernelGGL(stage_split_kernel, dim3((nstage + 255) / 256), dim3(256), 0, s, Xb, N, nv, nsteps, ngroups,
+                           c.s_xscale.p, xs, xt);
+        hipLaunchKernelGGL((herm_product_kernel<kHermTiles, true>), dim3(c.herm_items), dim3(256), 0, s, c.d_H32.p, xs, xt,
+                           c.d_herm_items.p, N, sH, c.s_yp32.p, c.s_pt32.p);
+        hipLaunchKernelGGL(herm_reduce_kernel, dim3((N + 63) / 64), dim3(256), 0, s, c.s_yp32.p, c.s_pt32.p, N, nv, C,
+                           c.s_xinv.p, Y + static_cast<int64_t>(v0) * N);
     }
     XATU_HIP(hipGetLastError());
 }
