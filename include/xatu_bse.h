@@ -159,6 +159,9 @@ int xatu_fourier_sums(xatu_ctx* ctx, const xatu_interaction* inter);
 /* Assemble the BSE matrix in the given arithmetic and keep it resident in HBM
  * (ExcitonTB.cpp:840-921). scissor in eV. If hbs_out is non-null it receives the full
  * N x N matrix, column-major complex (double pairs for FP64, float pairs for FP32).
+ * Without hbs_out an FP32 build at even N assembles only the block-lower half that the
+ * split-FP16 products read (the rest follows on the first call that needs it:
+ * xatu_get_bse, xatu_compare_builds or a solve with XATU_ARITH_FP32).
  * With precision XATU_MATRIX_FREE nothing is stored: the call only prepares the
  * exchange factors and the diagonal so that solves and products can run matrix-free. */
 int xatu_build_bse(xatu_ctx* ctx, int precision, double scissor, void* hbs_out);

@@ -11,20 +11,26 @@
 // by its own power of two the same way, so that every hi part is a normal FP16 number
 // below 2^14 and the lo parts keep 11 more bits; the reduction divides the scales out.
 //
-// Layout: H is Hermitian, so row i of H is the conjugate of column i, which is
-// contiguous in the column-major matrix. A wave owns 16 TW output rows (TW tiles of 16)
-// and all 32 vectors; lane (i = l % 16, g = l / 16) of tile t reads 8 consecutive
-// complex entries of column row0 + 16 t + i per 32-k step as four 16-byte loads, load q
-// covering k = 8 q + 2 g + {0, 1}, so that each load instruction reads 64 contiguous
-// bytes per column. The vectors are staged in the same k order as FP16 fragments
-// Xs[step][n][g] = {c_hi, c_lo, d_hi, d_lo} (x = c + i d, 64 bytes), so that a lane's
-// B operand for one 16-vector tile is one contiguous 64-byte read.
-// With A = conj(H(k, i)) = a - i b: Re y = a c + b d, Im y = a d - b c (three FP32
-// accumulators per tile: a c + b d, a d, b c).
-// Columns are split across workgroups (split-K, XCD-aware as in the FP32 kernel); the
-// last step of the last split may read up to 31 entries past the end of a column: the
-// staged vectors are zero there and the matrix allocation carries 32 zero entries of
-// padding past its end.
+// Layout: H is Hermitian and column-major, so row i of H is the conjugate of column i,
+// which is contiguous. The product reads each distinct entry once: row block b (256 rows,
+// one workgroup of 4 waves, 64 rows per wave) reads columns i of b from row k = 256 b down
+// (its diagonal block and the blocks below it). From one load it forms
+//   direct blocks      y_i += sum_k conj(H(k, i)) x_k   (rows of b, accumulated in registers)
+//   transposed blocks  y_k += sum_i H(k, i) x_i         (k below the diagonal block, per step)
+// so that the block-upper part of H is never read (a block-lower build never writes it).
+// Lane (i = l % 16, g = l / 16) of tile t reads 8 consecutive complex entries of column
+// row0 + 16 t + i per 32-k step as four 16-byte loads, load q covering k = 8 q + 2 g + {0, 1};
+// the vectors are staged in the same k order as FP16 fragments Xs[step][n][g] =
+// {c_hi, c_lo, d_hi, d_lo} (x = c + i d, 64 bytes). For the transposed blocks the split
+// parts pass through a per-wave LDS buffer so that a lane holds one k and eight i, the
+// vectors being staged a second time in that i order (Xt). Complex products use two FP32
+// accumulators per tile, the negated imaginary parts of H supplying the minus signs.
+// The columns are cut into chunks of 2048 (work items (b, chunk) for the chunks at or
+// below b's diagonal block); the direct sums go to one slot per chunk, the transposed
+// sums (the four waves' added in LDS) to one slot per row block, and a reduction adds
+// the slots of each row in a fixed order. The last step of a column may read up to 31
+// entries past its end: the staged vectors are zero there and the entries read are finite
+// (32 zero entries of padding after the matrix, zeroed heads of the unbuilt columns).
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -213,7 +219,7 @@ __device__ inline void wave_lds_sync() {
 //  sums over their 64 rows are combined in LDS and stored per step to Pt[b][k][32].
 // Xt holds the vectors in the i order of the transposed fragments: element e of lane
 // (n, g) of fragment jj of the wave's 64 rows holds row 32 jj + 16 (e & 1) + 4 g + (e >> 1).
-template <int TW, bool kPipe>
+template <int TW>
 __global__ __launch_bounds__(256, 1) void herm_product_kernel(const float2* __restrict__ H,
                                                               const XFrag16* __restrict__ Xs,
                                                               const XFrag16* __restrict__ Xt,
@@ -300,9 +306,6 @@ __global__ __launch_bounds__(256, 1) void herm_product_kernel(const float2* __re
             if (k < N) out[k * kVec + n] = sum;
         }
     };
-    bool pend = false;
-    int pend_slot = 0;
-    int64_t pend_k0 = 0;
     // body(st) with st == nsteps (odd step counts) only reloads; loads past the last step
     // repeat the last step, so that every register set is reloaded unconditionally
     auto body = [&](float4 (&rb)[TW][4], XFrag16 (&xb)[2], int st) {
@@ -366,11 +369,6 @@ __global__ __launch_bounds__(256, 1) void herm_product_kernel(const float2* __re
         }
 #pragma unroll
         for (int u = 0; u < 2; ++u) xb[u] = Xs[((kn / 32) * kVec + 16 * u + li) * 4 + g];
-        if (kPipe && pend && low) {
-            __syncthreads();
-            combine(pend_slot, pend_k0);
-            pend = false;
-        }
         if (low) {
             wave_lds_sync();
             // transposed blocks: lane (m, g) holds row k(rho = 16 h + m) and the eight i of
@@ -420,14 +418,8 @@ __global__ __launch_bounds__(256, 1) void herm_product_kernel(const float2* __re
                         acct[h][u][0][v] = 0.0f;
                         acct[h][u][1][v] = 0.0f;
                     }
-            if (!kPipe) {
-                __syncthreads();
-                combine(slot, k0);
-            } else {
-                pend = true;
-                pend_slot = slot;
-                pend_k0 = k0;
-            }
+            __syncthreads();
+            combine(slot, k0);
         }
     };
     float4 bufa[TW][4], bufb[TW][4];
@@ -438,10 +430,7 @@ __global__ __launch_bounds__(256, 1) void herm_product_kernel(const float2* __re
         body(bufa, xa, st);
         body(bufb, xbb, st + 1);
     }
-    if (pend) {
-        __syncthreads();
-        combine(pend_slot, pend_k0);
-    }
+
     float2* out = Pn + static_cast<int64_t>(c) * N * kVec;
 #pragma unroll
     for (int t = 0; t < TW; ++t)
@@ -666,7 +655,7 @@ void launch_matvec_split16(Context& c, const double2* X, double2* Y, int nvec, h
                            c.s_xinv.p);
         hipLaunchKernelGGL(stage_split_kernel, dim3((nstage + 255) / 256), dim3(256), 0, s, Xb, N, nv, nsteps, ngroups,
                            c.s_xscale.p, xs, xt);
-        hipLaunchKernelGGL((herm_product_kernel<kHermTiles, true>), dim3(c.herm_items), dim3(256), 0, s, c.d_H32.p, xs, xt,
+        hipLaunchKernelGGL((herm_product_kernel<kHermTiles>), dim3(c.herm_items), dim3(256), 0, s, c.d_H32.p, xs, xt,
                            c.d_herm_items.p, N, sH, c.s_yp32.p, c.s_pt32.p);
         hipLaunchKernelGGL(herm_reduce_kernel, dim3((N + 63) / 64), dim3(256), 0, s, c.s_yp32.p, c.s_pt32.p, N, nv, C,
                            c.s_xinv.p, Y + static_cast<int64_t>(v0) * N);
