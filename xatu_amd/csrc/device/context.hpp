@@ -63,6 +63,29 @@ struct DevBuf {
     size_t bytes() const { return n * sizeof(T); }
 };
 
+// Owning pinned host allocation, grown on demand: small transfers to and from it need no
+// staging copy.
+template <typename T>
+struct HostBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    HostBuf() = default;
+    HostBuf(const HostBuf&) = delete;
+    HostBuf& operator=(const HostBuf&) = delete;
+    ~HostBuf() { release(); }
+    void release() {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void reserve(size_t count) {
+        if (count <= n && p) return;
+        release();
+        XATU_HIP(hipHostMalloc(reinterpret_cast<void**>(&p), count * sizeof(T), hipHostMallocDefault));
+        n = count;
+    }
+};
+
 // Geometry and lattice-sum description shared with the kernels (by value).
 struct LatticeDesc {
     int ndim;
@@ -159,6 +182,10 @@ struct Context {
     float h16_scale = 0.0f;           // power-of-two scale of H for the split product (0: unavailable)
     bool fp32_arith_only = false;     // products with the FP32 matrix stay on the FP32 matrix cores
     DevBuf<double2> s_gram;           // split partial sums of the Gram kernel
+    // pinned host staging of the Davidson solver's per-iteration transfers: projections,
+    // Ritz coefficients and values, residual partial sums, block whitening coefficients
+    HostBuf<double2> h_proj, h_coef, h_whiten;
+    HostBuf<double> h_theta, h_part;
 
     // last solve
     int last_nstates = 0;

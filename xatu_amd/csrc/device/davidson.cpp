@@ -82,6 +82,11 @@ public:
         ws_.theta.alloc(b_);
         ws_.partial.alloc(static_cast<size_t>(b_) * ((N_ + 255) / 256));
         T_.assign(static_cast<size_t>(mmax_) * mmax_, cd(0.0, 0.0));
+        c_.h_proj.reserve(static_cast<size_t>(mmax_) * b_);
+        c_.h_coef.reserve(static_cast<size_t>(mmax_) * b_);
+        c_.h_whiten.reserve(static_cast<size_t>(b_) * b_);
+        c_.h_theta.reserve(b_);
+        c_.h_part.reserve(static_cast<size_t>(b_) * ((N_ + 255) / 256));
     }
 
     // Start from the unit vectors on the b smallest diagonal entries.
@@ -152,8 +157,9 @@ private:
         evg.start();
         launch_gram(c_, ws_.V.p, static_cast<int>(N_), Wn, static_cast<int>(N_), mn, nb, ws_.P.p, mn, c_.stream);
         evg.stop();
-        std::vector<cd> P(static_cast<size_t>(mn) * nb);
-        XATU_HIP(hipMemcpyAsync(P.data(), ws_.P.p, P.size() * sizeof(cd), hipMemcpyDeviceToHost, c_.stream));
+        const cd* P = reinterpret_cast<const cd*>(c_.h_proj.p);
+        XATU_HIP(hipMemcpyAsync(c_.h_proj.p, ws_.P.p, static_cast<size_t>(mn) * nb * sizeof(cd), hipMemcpyDeviceToHost,
+                                c_.stream));
         XATU_HIP(hipStreamSynchronize(c_.stream));
         const double pms = evp.ms();
         prod_ms_ += pms;
@@ -183,15 +189,18 @@ private:
     void rayleigh_ritz() {
         const double t0 = now_ms();
         std::vector<double> w(m_);
-        Yb_.assign(static_cast<size_t>(m_) * b_, cd(0.0, 0.0));
-        xatu_linalg::heev_lowest(m_, T_.data(), mmax_, b_, w.data(), Yb_.data());
+        cd* Yb = reinterpret_cast<cd*>(c_.h_coef.p);
+        std::fill(Yb, Yb + static_cast<size_t>(m_) * b_, cd(0.0, 0.0));
+        xatu_linalg::heev_lowest(m_, T_.data(), mmax_, b_, w.data(), Yb);
         theta_.assign(w.begin(), w.begin() + b_);
         rr_ms_ += now_ms() - t0;
     }
 
     void ritz_and_residuals() {
-        XATU_HIP(hipMemcpyAsync(ws_.Y.p, Yb_.data(), Yb_.size() * sizeof(cd), hipMemcpyHostToDevice, c_.stream));
-        XATU_HIP(hipMemcpyAsync(ws_.theta.p, theta_.data(), b_ * sizeof(double), hipMemcpyHostToDevice, c_.stream));
+        std::copy(theta_.begin(), theta_.end(), c_.h_theta.p);
+        XATU_HIP(hipMemcpyAsync(ws_.Y.p, c_.h_coef.p, static_cast<size_t>(m_) * b_ * sizeof(cd), hipMemcpyHostToDevice,
+                                c_.stream));
+        XATU_HIP(hipMemcpyAsync(ws_.theta.p, c_.h_theta.p, b_ * sizeof(double), hipMemcpyHostToDevice, c_.stream));
         const int n = static_cast<int>(N_);
         EventTimer ev(c_.stream);
         ev.start();
@@ -202,9 +211,9 @@ private:
         ev.stop();
         int nblk = 0;
         launch_residual(ws_.X.p, ws_.AX.p, ws_.theta.p, N_, b_, ws_.R.p, ws_.partial.p, &nblk, c_.stream);
-        std::vector<double> part(static_cast<size_t>(nblk) * b_);
-        XATU_HIP(hipMemcpyAsync(part.data(), ws_.partial.p, part.size() * sizeof(double), hipMemcpyDeviceToHost,
-                                c_.stream));
+        const double* part = c_.h_part.p;
+        XATU_HIP(hipMemcpyAsync(c_.h_part.p, ws_.partial.p, static_cast<size_t>(nblk) * b_ * sizeof(double),
+                                hipMemcpyDeviceToHost, c_.stream));
         XATU_HIP(hipStreamSynchronize(c_.stream));
         proj_ms_ += ev.ms();
         res_.assign(b_, 0.0);
@@ -247,8 +256,9 @@ private:
             }
         }
         ev.stop();
-        std::vector<cd> P(static_cast<size_t>(mn) * nb);
-        XATU_HIP(hipMemcpyAsync(P.data(), ws_.P.p, P.size() * sizeof(cd), hipMemcpyDeviceToHost, c_.stream));
+        const cd* P = reinterpret_cast<const cd*>(c_.h_proj.p);
+        XATU_HIP(hipMemcpyAsync(c_.h_proj.p, ws_.P.p, static_cast<size_t>(mn) * nb * sizeof(cd), hipMemcpyDeviceToHost,
+                                c_.stream));
         XATU_HIP(hipStreamSynchronize(c_.stream));
         proj_ms_ += ev.ms();
         // column-normalised Gram matrix of the block
@@ -277,7 +287,8 @@ private:
         }
         if (kept == 0) return 0;
         // V[:, m:m+kept] = T S (through the scratch block: the product may not overlap its input)
-        XATU_HIP(hipMemcpyAsync(ws_.P.p, S.data(), S.size() * sizeof(cd), hipMemcpyHostToDevice, c_.stream));
+        std::copy(S.begin(), S.end(), reinterpret_cast<cd*>(c_.h_whiten.p));
+        XATU_HIP(hipMemcpyAsync(ws_.P.p, c_.h_whiten.p, S.size() * sizeof(cd), hipMemcpyHostToDevice, c_.stream));
         XATU_BLAS(rocblas_zgemm(c_.blas, rocblas_operation_none, rocblas_operation_none, n, kept, nb, &kOne, bz(T), n,
                                 bz(ws_.P.p), nb, &kZero, bz(ws_.Tb.p), n));
         XATU_HIP(hipMemcpyAsync(T, ws_.Tb.p, static_cast<size_t>(N_) * kept * sizeof(double2), hipMemcpyDeviceToDevice,
@@ -305,7 +316,6 @@ private:
     Workspace ws_;
     std::vector<cd> T_;
     std::vector<double> theta_, res_;
-    std::vector<cd> Yb_;
     double prod_ms_ = 0.0, rr_ms_ = 0.0, proj_ms_ = 0.0, last_worst_ = 0.0;
 };
 
