@@ -29,6 +29,7 @@
 // precision and batch size. The exchange part runs in FP64 for both precisions.
 #include <map>
 #include <mutex>
+#include <tuple>
 #include <type_traits>
 #include <utility>
 #include <vector>
@@ -53,6 +54,20 @@ namespace xatu_dev {
 namespace {
 
 constexpr int kScanRows = 256;
+
+// Layout of the correlation buffer G (batch entries b of M mesh points g = o0 + mesh_n o1).
+// Entry-major: G[b M + g]. Row-interleaved (2-D meshes): G[(o1 batch + b) mesh_n + o0],
+// so that the FFTs along o0 (unit stride, rows of mesh_n) and along o1 (stride
+// batch mesh_n, consecutive transforms adjacent) are each one uniform batched 1-D pass
+// with no transposition between them.
+struct GLayout {
+    int64_t bstride;    // distance between consecutive batch entries
+    int rowlen;         // 0: entry-major; mesh_n: row-interleaved
+    int64_t rowstride;  // row-interleaved: batch mesh_n
+    __device__ int64_t at(int64_t b, int g) const {
+        return b * bstride + (rowlen ? static_cast<int64_t>(g / rowlen) * rowstride + g % rowlen : g);
+    }
+};
 
 __device__ inline int atom_of(const LatticeDesc& lat, int orb) {
     int a = 0;
@@ -154,10 +169,10 @@ __global__ __launch_bounds__(256) void conv_u_kernel(int nk, int norb, int nvec,
     }
 }
 
-// G[((al norb + be) nvec + n) M + g(k)] = sum_c C_k[al, c] u[n][c][be][k]
+// G[entry (al norb + be) nvec + n][g(k)] = sum_c C_k[al, c] u[n][c][be][k]
 // thread per k, blockIdx.y = group of kConvOrbGroup orbitals al, blockIdx.z = vector n
 template <typename R, int NC>
-__global__ __launch_bounds__(256) void conv_z_kernel(int nk, int norb, int nvec, int64_t M, const int* __restrict__ gk,
+__global__ __launch_bounds__(256) void conv_z_kernel(int nk, int norb, int nvec, GLayout lay, const int* __restrict__ gk,
                                                      const cx_t<R>* __restrict__ Ct, const cx_t<R>* __restrict__ U,
                                                      cx_t<R>* __restrict__ G) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -170,7 +185,7 @@ __global__ __launch_bounds__(256) void conv_z_kernel(int nk, int norb, int nvec,
 #pragma unroll
         for (int c = 0; c < NC; ++c)
             Ca[a][c] = al0 + a < norb ? Ct[(static_cast<size_t>(c) * norb + al0 + a) * nk + k] : mk(R(0), R(0));
-    const int64_t g = gk[k];
+    const int g = gk[k];
     for (int be = 0; be < norb; ++be) {
         cx_t<R> ub[NC];
 #pragma unroll
@@ -181,7 +196,7 @@ __global__ __launch_bounds__(256) void conv_z_kernel(int nk, int norb, int nvec,
                 cx_t<R> s = mk(R(0), R(0));
 #pragma unroll
                 for (int c = 0; c < NC; ++c) cfma(s, Ca[a][c], ub[c]);
-                G[((static_cast<int64_t>(al0 + a) * norb + be) * nvec + n) * M + g] = s;
+                G[lay.at((static_cast<int64_t>(al0 + a) * norb + be) * nvec + n, g)] = s;
             }
         }
     }
@@ -189,7 +204,7 @@ __global__ __launch_bounds__(256) void conv_z_kernel(int nk, int norb, int nvec,
 
 // G[b][xi] *= P[a(al) + NA b(be)][xi], batch b = (al norb + be) nvec + n
 template <typename R>
-__global__ void conv_multiply_kernel(LatticeDesc lat, int nvec, int64_t M, const cx_t<R>* __restrict__ P,
+__global__ void conv_multiply_kernel(LatticeDesc lat, int nvec, int64_t M, GLayout lay, const cx_t<R>* __restrict__ P,
                                      cx_t<R>* __restrict__ G) {
     const int64_t xi = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     const int b = blockIdx.y;
@@ -197,14 +212,184 @@ __global__ void conv_multiply_kernel(LatticeDesc lat, int nvec, int64_t M, const
     const int ab = b / nvec;
     const int al = ab / lat.norb, be = ab % lat.norb;
     const int pa = atom_of(lat, al) + lat.natoms * atom_of(lat, be);
-    const int64_t i = static_cast<int64_t>(b) * M + xi;
+    const int64_t i = lay.at(b, static_cast<int>(xi));
     G[i] = cmul(G[i], P[static_cast<int64_t>(pa) * M + xi]);
+}
+
+// ---- fused transform along the second mesh axis (row-interleaved layout) ----
+// For 2-D meshes whose side factors into 2, 3, 4 and 5 (at most kColMax), the transform
+// along o1, the product with the spectrum and the inverse transform along o1 run in one
+// pass: a workgroup holds kColTile adjacent columns u = b mesh_n + xi0 (each a length-mesh_n
+// vector of stride batch mesh_n) in LDS and applies mixed-radix Stockham passes (natural
+// order in and out, ping-pong between two buffers), so G is read and written once instead
+// of twice per direction with a transposition in between.
+constexpr int kColMax = 128;
+
+struct Radices {
+    int n;          // mesh side
+    int nrad;       // number of passes
+    int r[8];       // radix of each pass (2, 3, 4 or 5)
+    int64_t count;  // columns: batch mesh_n
+};
+
+template <typename R>
+constexpr int col_tile() {
+    return sizeof(R) == 8 ? 16 : 32;
+}
+
+// In-register DFT of length RD; sg = -1 forward (e^{-2 pi i / RD}), +1 inverse.
+template <typename R, int RD>
+__device__ inline void small_dft(cx_t<R> (&v)[RD], R sg) {
+    using C = cx_t<R>;
+    auto isc = [&](C z, R f) { return mk(-sg * f * z.y, sg * f * z.x); };  // i sg f z
+    if constexpr (RD == 2) {
+        const C a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    } else if constexpr (RD == 3) {
+        const R h3 = R(0.86602540378443864676);  // sin(2 pi / 3)
+        const C a = cadd(v[1], v[2]), d = csub(v[1], v[2]);
+        const C m = mk(v[0].x - R(0.5) * a.x, v[0].y - R(0.5) * a.y);
+        const C t = isc(d, h3);
+        v[0] = cadd(v[0], a);
+        v[1] = cadd(m, t);
+        v[2] = csub(m, t);
+    } else if constexpr (RD == 4) {
+        const C t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]), t2 = cadd(v[1], v[3]), t3 = isc(csub(v[1], v[3]), R(1));
+        v[0] = cadd(t0, t2);
+        v[1] = cadd(t1, t3);
+        v[2] = csub(t0, t2);
+        v[3] = csub(t1, t3);
+    } else {
+        static_assert(RD == 5, "radix");
+        // cos and sin of 2 pi / 5 and 4 pi / 5
+        const R c1 = R(0.30901699437494742410), s1 = R(0.95105651629515357212);
+        const R c2 = R(-0.80901699437494742410), s2 = R(0.58778525229247312917);
+        const C a1 = cadd(v[1], v[4]), d1 = csub(v[1], v[4]), a2 = cadd(v[2], v[3]), d2 = csub(v[2], v[3]);
+        const C p1 = mk(v[0].x + c1 * a1.x + c2 * a2.x, v[0].y + c1 * a1.y + c2 * a2.y);
+        const C p2 = mk(v[0].x + c2 * a1.x + c1 * a2.x, v[0].y + c2 * a1.y + c1 * a2.y);
+        const C q1 = isc(mk(s1 * d1.x + s2 * d2.x, s1 * d1.y + s2 * d2.y), R(1));
+        const C q2 = isc(mk(s2 * d1.x - s1 * d2.x, s2 * d1.y - s1 * d2.y), R(1));
+        v[0] = cadd(v[0], cadd(a1, a2));
+        v[1] = cadd(p1, q1);
+        v[4] = csub(p1, q1);
+        v[2] = cadd(p2, q2);
+        v[3] = csub(p2, q2);
+    }
+}
+
+// One radix-RD Stockham pass over the tile (n x T, rows j, column c contiguous across the
+// tile): butterfly j reads rows j + r n/RD, twiddles them by e^{-/+2 pi i (j mod Ns) r / (Ns RD)}
+// and writes rows (j / Ns) Ns RD + (j mod Ns) + r Ns.
+template <typename R, int T, int RD>
+__device__ inline void radix_pass(const cx_t<R>* __restrict__ in, cx_t<R>* __restrict__ out,
+                                  const cx_t<R>* __restrict__ tw, int n, int Ns, R sg) {
+    using C = cx_t<R>;
+    const int nb = n / RD, step = n / (Ns * RD);
+    for (int bi = threadIdx.x; bi < nb * T; bi += blockDim.x) {
+        const unsigned j = static_cast<unsigned>(bi) / T, c = static_cast<unsigned>(bi) % T;
+        const unsigned jh = j / static_cast<unsigned>(Ns), jm = j - jh * static_cast<unsigned>(Ns);
+        C v[RD];
+        v[0] = in[j * T + c];
+#pragma unroll
+        for (int r = 1; r < RD; ++r) {
+            const C w = tw[jm * r * step];
+            v[r] = cmul(in[(j + r * nb) * T + c], mk(w.x, sg < R(0) ? w.y : -w.y));
+        }
+        small_dft<R, RD>(v, sg);
+        const unsigned d = jh * Ns * RD + jm;
+#pragma unroll
+        for (int r = 0; r < RD; ++r) out[(d + r * Ns) * T + c] = v[r];
+    }
+}
+
+// One direction of the transform of the tile's columns; returns the buffer holding the result.
+template <typename R, int T>
+__device__ inline cx_t<R>* column_passes(const Radices& rad, cx_t<R>* in, cx_t<R>* out, const cx_t<R>* tw, R sg) {
+    int Ns = 1;
+    for (int p = 0; p < rad.nrad; ++p) {
+        const int RD = rad.r[p];
+        switch (RD) {
+            case 2: radix_pass<R, T, 2>(in, out, tw, rad.n, Ns, sg); break;
+            case 3: radix_pass<R, T, 3>(in, out, tw, rad.n, Ns, sg); break;
+            case 4: radix_pass<R, T, 4>(in, out, tw, rad.n, Ns, sg); break;
+            default: radix_pass<R, T, 5>(in, out, tw, rad.n, Ns, sg); break;
+        }
+        __syncthreads();
+        cx_t<R>* t = in;
+        in = out;
+        out = t;
+        Ns *= RD;
+    }
+    return in;
+}
+
+// G columns u0 .. u0 + T - 1: forward transform along o1, times P[a(al) + NA b(be)][xi0 + n xi1],
+// inverse transform along o1. grid ceil(count / T), block 512 (two workgroups per CU, so that
+// one loads or stores while the other transforms).
+template <typename R>
+__global__ __launch_bounds__(512) void conv_columns_kernel(LatticeDesc lat, Radices rad, int nvec, int64_t M,
+                                                           const cx_t<R>* __restrict__ P, cx_t<R>* __restrict__ G) {
+    using C = cx_t<R>;
+    constexpr int T = col_tile<R>();
+    __shared__ C b0[kColMax * T], b1[kColMax * T], tw[kColMax];
+    const int n = rad.n;
+    const int64_t count = rad.count;
+    const int64_t u0 = static_cast<int64_t>(blockIdx.x) * T;
+    for (int idx = threadIdx.x; idx < n * T; idx += blockDim.x) {
+        const int j = idx / T, c = idx % T;
+        const int64_t u = u0 + c;
+        b0[idx] = u < count ? G[j * count + u] : mk(R(0), R(0));
+    }
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        double sn, cs;
+        sincospi(2.0 * k / n, &sn, &cs);
+        tw[k] = mk(static_cast<R>(cs), static_cast<R>(-sn));  // e^{-2 pi i k / n}
+    }
+    __syncthreads();
+    C* x = column_passes<R, T>(rad, b0, b1, tw, R(-1));
+    for (int idx = threadIdx.x; idx < n * T; idx += blockDim.x) {
+        const int j = idx / T, c = idx % T;
+        const int64_t u = u0 + c;
+        if (u < count) {
+            const int64_t b = u / n;
+            const int xi0 = static_cast<int>(u - b * n);
+            const int ab = static_cast<int>(b / nvec);
+            const int al = ab / lat.norb, be = ab % lat.norb;
+            const int pa = atom_of(lat, al) + lat.natoms * atom_of(lat, be);
+            x[idx] = cmul(x[idx], P[static_cast<int64_t>(pa) * M + xi0 + static_cast<int64_t>(n) * j]);
+        }
+    }
+    __syncthreads();
+    C* y = x == b0 ? b1 : b0;
+    x = column_passes<R, T>(rad, x, y, tw, R(1));
+    for (int idx = threadIdx.x; idx < n * T; idx += blockDim.x) {
+        const int j = idx / T, c = idx % T;
+        const int64_t u = u0 + c;
+        if (u < count) G[j * count + u] = x[idx];
+    }
+}
+
+// Radices of the fused column transform for mesh side n (nrad = -1: not applicable).
+Radices column_radices(int n) {
+    Radices rd{};
+    rd.n = n;
+    rd.nrad = -1;
+    if (n < 1 || n > kColMax) return rd;
+    int m = n, k = 0;
+    for (int r : {5, 3, 4, 2})
+        while (m % r == 0 && k < 8) {
+            rd.r[k++] = r;
+            m /= r;
+        }
+    if (m == 1) rd.nrad = k;
+    return rd;
 }
 
 // Y[k NCV + c NV + v, n] = e(kcv) x - sum_be V_k[be, v] sum_al conj(C_k[al, c]) w[(al norb + be) nvec + n][g(k)]
 // thread per k, blockIdx.y = group of vectors (the band coefficients are loaded once per group)
 template <typename R, int NC, int NV>
-__global__ __launch_bounds__(256) void conv_y_kernel(int nk, int norb, int nvec, int64_t M, const int* __restrict__ gk,
+__global__ __launch_bounds__(256) void conv_y_kernel(int nk, int norb, int nvec, GLayout lay, const int* __restrict__ gk,
                                                      const cx_t<R>* __restrict__ Ct, const cx_t<R>* __restrict__ Vt,
                                                      const double* __restrict__ eval_k, const double* __restrict__ eval_kq,
                                                      double scissor, const cx_t<R>* __restrict__ W,
@@ -215,7 +400,7 @@ __global__ __launch_bounds__(256) void conv_y_kernel(int nk, int norb, int nvec,
     if (k >= nk) return;
     constexpr int nb = NV + NC, ncv = NC * NV;
     const int ng = min(VG, nvec - n0);
-    const int64_t g = gk[k];
+    const int g = gk[k];
     cx_t<R> acc[VG][NC][NV];
 #pragma unroll
     for (int q = 0; q < VG; ++q)
@@ -233,11 +418,11 @@ __global__ __launch_bounds__(256) void conv_y_kernel(int nk, int norb, int nvec,
             cx_t<R> ca[NC];
 #pragma unroll
             for (int c = 0; c < NC; ++c) ca[c] = Ct[(static_cast<size_t>(c) * norb + al) * nk + k];
-            const cx_t<R>* w = W + ((static_cast<int64_t>(al) * norb + be) * nvec + n0) * M + g;
+            const cx_t<R>* w = W + lay.at((static_cast<int64_t>(al) * norb + be) * nvec + n0, g);
 #pragma unroll
             for (int q = 0; q < VG; ++q) {
                 if (q < ng) {
-                    const cx_t<R> wq = w[q * M];
+                    const cx_t<R> wq = w[q * lay.bstride];
 #pragma unroll
                     for (int c = 0; c < NC; ++c) cfma_conj(t[q][c], ca[c], wq);
                 }
@@ -444,16 +629,19 @@ struct ConvState {
     DevBuf<double2> G64, U64, Ct64, Vt64;
     DevBuf<float2> G32, U32, Ct32, Vt32;
     DevBuf<double2> part, total;
+    // A transform over the whole mesh: one rocFFT plan per direction (entry-major layout),
+    // or for the row-interleaved layout two batched 1-D plans per direction (along o0,
+    // then along o1)
     struct Plan {
-        rocfft_plan fwd = nullptr, bwd = nullptr;
+        rocfft_plan fwd = nullptr, bwd = nullptr, fwd2 = nullptr, bwd2 = nullptr;
         rocfft_execution_info info = nullptr;
         void* work = nullptr;
     };
-    std::map<std::pair<int, int>, Plan> plans;  // (bytes per real, batch)
+    std::map<std::tuple<int, int, int>, Plan> plans;  // (bytes per real, batch, row-interleaved)
     ~ConvState() {
         for (auto& kv : plans) {
-            if (kv.second.fwd) rocfft_plan_destroy(kv.second.fwd);
-            if (kv.second.bwd) rocfft_plan_destroy(kv.second.bwd);
+            for (rocfft_plan q : {kv.second.fwd, kv.second.bwd, kv.second.fwd2, kv.second.bwd2})
+                if (q) rocfft_plan_destroy(q);
             if (kv.second.info) rocfft_execution_info_destroy(kv.second.info);
             if (kv.second.work) (void)hipFree(kv.second.work);
         }
@@ -471,25 +659,53 @@ void fft_setup_once() {
     if (st != rocfft_status_success) throw Error(XATU_ERR_DEVICE, "rocfft_setup failed");
 }
 
+// Batched 1-D plan of length n with the given stride and distance.
 template <typename R>
-ConvState::Plan& get_plan(Context& c, ConvState& cs, int batch) {
-    const auto key = std::make_pair(static_cast<int>(sizeof(R)), batch);
+rocfft_plan strided_plan(rocfft_transform_type dir, size_t n, size_t count, size_t stride, size_t dist) {
+    rocfft_plan_description d = nullptr;
+    XATU_FFT(rocfft_plan_description_create(&d));
+    size_t st[1] = {stride};
+    rocfft_status r = rocfft_plan_description_set_data_layout(d, rocfft_array_type_complex_interleaved,
+                                                             rocfft_array_type_complex_interleaved, nullptr, nullptr,
+                                                             1, st, dist, 1, st, dist);
+    rocfft_plan q = nullptr;
+    if (r == rocfft_status_success) r = rocfft_plan_create(&q, rocfft_placement_inplace, dir, fft_precision<R>(), 1, &n, count, d);
+    rocfft_plan_description_destroy(d);
+    XATU_FFT(r);
+    return q;
+}
+
+// Plans over `batch` mesh transforms; interleaved (2-D meshes only): the row-interleaved
+// layout of GLayout.
+template <typename R>
+ConvState::Plan& get_plan(Context& c, ConvState& cs, int batch, bool interleaved = false) {
+    const auto key = std::make_tuple(static_cast<int>(sizeof(R)), batch, interleaved ? 1 : 0);
     auto it = cs.plans.find(key);
     if (it != cs.plans.end()) return it->second;
     fft_setup_once();
     ConvState::Plan p;
-    size_t lengths[3] = {static_cast<size_t>(c.lat.mesh_n), static_cast<size_t>(c.lat.mesh_n),
-                         static_cast<size_t>(c.lat.mesh_n)};
-    XATU_FFT(rocfft_plan_create(&p.fwd, rocfft_placement_inplace, rocfft_transform_type_complex_forward,
-                                fft_precision<R>(), c.lat.ndim, lengths, batch, nullptr));
-    XATU_FFT(rocfft_plan_create(&p.bwd, rocfft_placement_inplace, rocfft_transform_type_complex_inverse,
-                                fft_precision<R>(), c.lat.ndim, lengths, batch, nullptr));
-    size_t w1 = 0, w2 = 0;
-    XATU_FFT(rocfft_plan_get_work_buffer_size(p.fwd, &w1));
-    XATU_FFT(rocfft_plan_get_work_buffer_size(p.bwd, &w2));
+    const size_t m = static_cast<size_t>(c.lat.mesh_n);
+    size_t lengths[3] = {m, m, m};
+    if (interleaved) {
+        const size_t count = static_cast<size_t>(batch) * m;
+        p.fwd = strided_plan<R>(rocfft_transform_type_complex_forward, m, count, 1, m);
+        p.fwd2 = strided_plan<R>(rocfft_transform_type_complex_forward, m, count, count, 1);
+        p.bwd = strided_plan<R>(rocfft_transform_type_complex_inverse, m, count, 1, m);
+        p.bwd2 = strided_plan<R>(rocfft_transform_type_complex_inverse, m, count, count, 1);
+    } else {
+        XATU_FFT(rocfft_plan_create(&p.fwd, rocfft_placement_inplace, rocfft_transform_type_complex_forward,
+                                    fft_precision<R>(), c.lat.ndim, lengths, batch, nullptr));
+        XATU_FFT(rocfft_plan_create(&p.bwd, rocfft_placement_inplace, rocfft_transform_type_complex_inverse,
+                                    fft_precision<R>(), c.lat.ndim, lengths, batch, nullptr));
+    }
+    size_t w = 0;
+    for (rocfft_plan q : {p.fwd, p.bwd, p.fwd2, p.bwd2}) {
+        size_t wq = 0;
+        if (q) XATU_FFT(rocfft_plan_get_work_buffer_size(q, &wq));
+        w = std::max(w, wq);
+    }
     XATU_FFT(rocfft_execution_info_create(&p.info));
     XATU_FFT(rocfft_execution_info_set_stream(p.info, c.stream));
-    const size_t w = std::max(w1, w2);
     if (w > 0) {
         XATU_HIP(hipMalloc(&p.work, w));
         XATU_FFT(rocfft_execution_info_set_work_buffer(p.info, p.work, w));
@@ -564,6 +780,10 @@ void conv_product(Context& c, const double2* X, double2* Y, int nvec, hipStream_
         const double2* Xb = X + static_cast<int64_t>(v0) * N;
         double2* Yb = Y + static_cast<int64_t>(v0) * N;
         const int batch = norb * norb * nv;
+        // 2-D meshes: row-interleaved layout, two 1-D passes without transpositions
+        const bool inter = c.lat.ndim == 2;
+        const GLayout lay = inter ? GLayout{c.lat.mesh_n, c.lat.mesh_n, static_cast<int64_t>(batch) * c.lat.mesh_n}
+                                  : GLayout{M, 0, 0};
         G.alloc(std::max<size_t>(G.n, static_cast<size_t>(batch) * M));
         U.alloc(std::max<size_t>(U.n, static_cast<size_t>(nv) * c.nc * norb * c.nk));
         if (!full) XATU_HIP(hipMemsetAsync(G.p, 0, static_cast<size_t>(batch) * M * sizeof(cx_t<R>), s));
@@ -574,19 +794,29 @@ void conv_product(Context& c, const double2* X, double2* Y, int nvec, hipStream_
             hipLaunchKernelGGL((conv_u_kernel<R, NC, NV>), dim3(kb.x, vg), dim3(256), 0, s, c.nk, norb, nv, Vt, Xb, N,
                                U.p);
             hipLaunchKernelGGL((conv_z_kernel<R, NC>), dim3(kb.x, (norb + kConvOrbGroup - 1) / kConvOrbGroup, nv),
-                               dim3(256), 0, s, c.nk, norb, nv, M, cs.d_gk.p, Ct, U.p, G.p);
+                               dim3(256), 0, s, c.nk, norb, nv, lay, cs.d_gk.p, Ct, U.p, G.p);
         });
         XATU_HIP(hipGetLastError());
-        ConvState::Plan& p = get_plan<R>(c, cs, batch);
+        ConvState::Plan& p = get_plan<R>(c, cs, batch, inter);
         void* buf[1] = {G.p};
         XATU_FFT(rocfft_execute(p.fwd, buf, nullptr, p.info));
-        hipLaunchKernelGGL((conv_multiply_kernel<R>), dim3((M + 255) / 256, batch), dim3(256), 0, s, c.lat, nv, M, P,
-                           G.p);
+        Radices rd = column_radices(c.lat.mesh_n);
+        if (inter && rd.nrad >= 0) {
+            rd.count = static_cast<int64_t>(batch) * c.lat.mesh_n;
+            constexpr int T = col_tile<R>();
+            hipLaunchKernelGGL((conv_columns_kernel<R>), dim3(static_cast<unsigned>((rd.count + T - 1) / T)), dim3(512),
+                               0, s, c.lat, rd, nv, M, P, G.p);
+        } else {
+            if (p.fwd2) XATU_FFT(rocfft_execute(p.fwd2, buf, nullptr, p.info));
+            hipLaunchKernelGGL((conv_multiply_kernel<R>), dim3((M + 255) / 256, batch), dim3(256), 0, s, c.lat, nv, M,
+                               lay, P, G.p);
+            if (p.bwd2) XATU_FFT(rocfft_execute(p.bwd2, buf, nullptr, p.info));
+        }
         XATU_HIP(hipGetLastError());
         XATU_FFT(rocfft_execute(p.bwd, buf, nullptr, p.info));
         with_bands<R>(c, [&](auto ncc, auto nvc) {
             constexpr int NC = decltype(ncc)::value, NV = decltype(nvc)::value;
-            hipLaunchKernelGGL((conv_y_kernel<R, NC, NV>), dim3(kb.x, vg), dim3(256), 0, s, c.nk, norb, nv, M,
+            hipLaunchKernelGGL((conv_y_kernel<R, NC, NV>), dim3(kb.x, vg), dim3(256), 0, s, c.nk, norb, nv, lay,
                                cs.d_gk.p, Ct, Vt, c.d_eval_k.p, c.d_eval_kq.p, c.scissor, G.p, Xb, N, Yb);
         });
         XATU_HIP(hipGetLastError());
@@ -636,7 +866,8 @@ void conv_pair_density(Context& c, const double2* psi, int ns, double2* S) {
         hipLaunchKernelGGL((conv_u_kernel<double, NC, NV>), dim3(kb.x, (ns + VG - 1) / VG), dim3(256), 0, s, c.nk, norb,
                            ns, cs.Vt64.p, psi, N, cs.U64.p);
         hipLaunchKernelGGL((conv_z_kernel<double, NC>), dim3(kb.x, (norb + kConvOrbGroup - 1) / kConvOrbGroup, ns),
-                           dim3(256), 0, s, c.nk, norb, ns, M, cs.d_gk.p, cs.Ct64.p, cs.U64.p, cs.G64.p);
+                           dim3(256), 0, s, c.nk, norb, ns, GLayout{M, 0, 0}, cs.d_gk.p, cs.Ct64.p, cs.U64.p,
+                           cs.G64.p);
     });
     XATU_HIP(hipGetLastError());
     ConvState::Plan& pf = get_plan<double>(c, cs, batch);

@@ -91,9 +91,12 @@ public:
 
     // Start from the unit vectors on the b smallest diagonal entries.
     void init_unit() {
+        // the b smallest diagonal entries, ties by index (the order a stable sort gives)
         std::vector<int> idx(N_);
         std::iota(idx.begin(), idx.end(), 0);
-        std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return c_.h_diag[a] < c_.h_diag[b]; });
+        std::partial_sort(idx.begin(), idx.begin() + b_, idx.end(), [&](int a, int b) {
+            return c_.h_diag[a] < c_.h_diag[b] || (c_.h_diag[a] == c_.h_diag[b] && a < b);
+        });
         idx.resize(b_);
         DevBuf<int> d_idx;
         d_idx.alloc(b_);
