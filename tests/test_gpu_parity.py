@@ -133,7 +133,8 @@ def test_fp32_build_and_mixed_solve(name, nstates, arith, request):
     ctx.close()
 
 
-@pytest.mark.parametrize("name", ["hbn20", "random18"])
+# N = 400, 576, 144 (a single row block of the split product) and 196
+@pytest.mark.parametrize("name", ["hbn20", "random18", "mos2_reduced", "mos2_odd"])
 def test_matvec(name, request):
     ex, H = get_case(name, request)
     ctx = gpu_context(ex)
