@@ -228,7 +228,7 @@ constexpr int kColMax = 128;
 struct Radices {
     int n;          // mesh side
     int nrad;       // number of passes
-    int r[8];       // radix of each pass (2, 3, 4 or 5)
+    int a[8], b[8]; // radix of each pass: a (2, 3, 4 or 5) times b (1, or 2 .. 5 for a composite radix)
     int64_t count;  // columns: batch mesh_n
 };
 
@@ -278,13 +278,49 @@ __device__ inline void small_dft(cx_t<R> (&v)[RD], R sg) {
     }
 }
 
+// Twiddle e^{sg 2 pi i k / n} from the table of e^{-2 pi i k / n}.
+template <typename R>
+__device__ inline cx_t<R> twiddle(const cx_t<R>* tw, int k, R sg) {
+    const cx_t<R> w = tw[k];
+    return mk(w.x, sg < R(0) ? w.y : -w.y);
+}
+
+// DFT of length A B in registers as two Stockham stages (radix A, then radix B with the
+// twiddles e^{sg 2 pi i j r / (A B)} read from the n-point table).
+template <typename R, int A, int B>
+__device__ inline void composite_dft(cx_t<R> (&v)[A * B], R sg, const cx_t<R>* tw, int n) {
+    using C = cx_t<R>;
+    constexpr int RD = A * B;
+    C t[RD];
+#pragma unroll
+    for (int j = 0; j < B; ++j) {
+        C x[A];
+#pragma unroll
+        for (int r = 0; r < A; ++r) x[r] = v[j + r * B];
+        small_dft<R, A>(x, sg);
+#pragma unroll
+        for (int r = 0; r < A; ++r) t[j * A + r] = x[r];
+    }
+    const int unit = n / RD;
+#pragma unroll
+    for (int j = 0; j < A; ++j) {
+        C x[B];
+#pragma unroll
+        for (int r = 0; r < B; ++r) x[r] = (j * r) ? cmul(t[j + r * A], twiddle(tw, j * r * unit, sg)) : t[j + r * A];
+        small_dft<R, B>(x, sg);
+#pragma unroll
+        for (int r = 0; r < B; ++r) v[j + r * A] = x[r];
+    }
+}
+
 // One radix-RD Stockham pass over the tile (n x T, rows j, column c contiguous across the
-// tile): butterfly j reads rows j + r n/RD, twiddles them by e^{-/+2 pi i (j mod Ns) r / (Ns RD)}
-// and writes rows (j / Ns) Ns RD + (j mod Ns) + r Ns.
-template <typename R, int T, int RD>
+// tile), RD = A B: butterfly j reads rows j + r n/RD, twiddles them by
+// e^{-/+2 pi i (j mod Ns) r / (Ns RD)} and writes rows (j / Ns) Ns RD + (j mod Ns) + r Ns.
+template <typename R, int T, int A, int B>
 __device__ inline void radix_pass(const cx_t<R>* __restrict__ in, cx_t<R>* __restrict__ out,
                                   const cx_t<R>* __restrict__ tw, int n, int Ns, R sg) {
     using C = cx_t<R>;
+    constexpr int RD = A * B;
     const int nb = n / RD, step = n / (Ns * RD);
     for (int bi = threadIdx.x; bi < nb * T; bi += blockDim.x) {
         const unsigned j = static_cast<unsigned>(bi) / T, c = static_cast<unsigned>(bi) % T;
@@ -296,7 +332,10 @@ __device__ inline void radix_pass(const cx_t<R>* __restrict__ in, cx_t<R>* __res
             const C w = tw[jm * r * step];
             v[r] = cmul(in[(j + r * nb) * T + c], mk(w.x, sg < R(0) ? w.y : -w.y));
         }
-        small_dft<R, RD>(v, sg);
+        if constexpr (B == 1)
+            small_dft<R, RD>(v, sg);
+        else
+            composite_dft<R, A, B>(v, sg, tw, n);
         const unsigned d = jh * Ns * RD + jm;
 #pragma unroll
         for (int r = 0; r < RD; ++r) out[(d + r * Ns) * T + c] = v[r];
@@ -308,18 +347,27 @@ template <typename R, int T>
 __device__ inline cx_t<R>* column_passes(const Radices& rad, cx_t<R>* in, cx_t<R>* out, const cx_t<R>* tw, R sg) {
     int Ns = 1;
     for (int p = 0; p < rad.nrad; ++p) {
-        const int RD = rad.r[p];
-        switch (RD) {
-            case 2: radix_pass<R, T, 2>(in, out, tw, rad.n, Ns, sg); break;
-            case 3: radix_pass<R, T, 3>(in, out, tw, rad.n, Ns, sg); break;
-            case 4: radix_pass<R, T, 4>(in, out, tw, rad.n, Ns, sg); break;
-            default: radix_pass<R, T, 5>(in, out, tw, rad.n, Ns, sg); break;
+        const int key = rad.a[p] * 8 + rad.b[p];
+        const int n = rad.n;
+        switch (key) {
+            case 2 * 8 + 1: radix_pass<R, T, 2, 1>(in, out, tw, n, Ns, sg); break;
+            case 3 * 8 + 1: radix_pass<R, T, 3, 1>(in, out, tw, n, Ns, sg); break;
+            case 4 * 8 + 1: radix_pass<R, T, 4, 1>(in, out, tw, n, Ns, sg); break;
+            case 5 * 8 + 1: radix_pass<R, T, 5, 1>(in, out, tw, n, Ns, sg); break;
+            case 3 * 8 + 2: radix_pass<R, T, 3, 2>(in, out, tw, n, Ns, sg); break;
+            case 4 * 8 + 2: radix_pass<R, T, 4, 2>(in, out, tw, n, Ns, sg); break;
+            case 5 * 8 + 2: radix_pass<R, T, 5, 2>(in, out, tw, n, Ns, sg); break;
+            case 3 * 8 + 3: radix_pass<R, T, 3, 3>(in, out, tw, n, Ns, sg); break;
+            case 5 * 8 + 3: radix_pass<R, T, 5, 3>(in, out, tw, n, Ns, sg); break;
+            case 3 * 8 + 4: radix_pass<R, T, 3, 4>(in, out, tw, n, Ns, sg); break;
+            case 4 * 8 + 4: radix_pass<R, T, 4, 4>(in, out, tw, n, Ns, sg); break;
+            default: radix_pass<R, T, 5, 4>(in, out, tw, n, Ns, sg); break;  // 5 * 8 + 4
         }
         __syncthreads();
         cx_t<R>* t = in;
         in = out;
         out = t;
-        Ns *= RD;
+        Ns *= rad.a[p] * rad.b[p];
     }
     return in;
 }
@@ -336,11 +384,12 @@ __global__ __launch_bounds__(512) void conv_columns_kernel(LatticeDesc lat, Radi
     const int n = rad.n;
     const int64_t count = rad.count;
     const int64_t u0 = static_cast<int64_t>(blockIdx.x) * T;
-    for (int idx = threadIdx.x; idx < n * T; idx += blockDim.x) {
-        const int j = idx / T, c = idx % T;
-        const int64_t u = u0 + c;
-        b0[idx] = u < count ? G[j * count + u] : mk(R(0), R(0));
-    }
+    // every thread keeps one column c of the tile (blockDim is a multiple of T) and rows
+    // j = threadIdx.x / T + multiples of blockDim / T
+    const int c = threadIdx.x % T, j0 = threadIdx.x / T, js = blockDim.x / T;
+    const bool live = u0 + c < count;
+    cx_t<R>* gcol = G + u0 + c;
+    for (int j = j0; j < n; j += js) b0[j * T + c] = live ? gcol[j * count] : mk(R(0), R(0));
     for (int k = threadIdx.x; k < n; k += blockDim.x) {
         double sn, cs;
         sincospi(2.0 * k / n, &sn, &cs);
@@ -348,41 +397,56 @@ __global__ __launch_bounds__(512) void conv_columns_kernel(LatticeDesc lat, Radi
     }
     __syncthreads();
     C* x = column_passes<R, T>(rad, b0, b1, tw, R(-1));
-    for (int idx = threadIdx.x; idx < n * T; idx += blockDim.x) {
-        const int j = idx / T, c = idx % T;
+    {
         const int64_t u = u0 + c;
-        if (u < count) {
+        if (live) {
             const int64_t b = u / n;
             const int xi0 = static_cast<int>(u - b * n);
             const int ab = static_cast<int>(b / nvec);
             const int al = ab / lat.norb, be = ab % lat.norb;
             const int pa = atom_of(lat, al) + lat.natoms * atom_of(lat, be);
-            x[idx] = cmul(x[idx], P[static_cast<int64_t>(pa) * M + xi0 + static_cast<int64_t>(n) * j]);
+            const cx_t<R>* pc = P + static_cast<int64_t>(pa) * M + xi0;
+            for (int j = j0; j < n; j += js) x[j * T + c] = cmul(x[j * T + c], pc[static_cast<int64_t>(n) * j]);
         }
     }
     __syncthreads();
     C* y = x == b0 ? b1 : b0;
     x = column_passes<R, T>(rad, x, y, tw, R(1));
-    for (int idx = threadIdx.x; idx < n * T; idx += blockDim.x) {
-        const int j = idx / T, c = idx % T;
-        const int64_t u = u0 + c;
-        if (u < count) G[j * count + u] = x[idx];
-    }
+    if (live)
+        for (int j = j0; j < n; j += js) gcol[j * count] = x[j * T + c];
 }
 
-// Radices of the fused column transform for mesh side n (nrad = -1: not applicable).
-Radices column_radices(int n) {
+// Radices of the fused column transform for mesh side n (nrad = -1: not applicable): the
+// factors 5, 3, 4, 2 (in that order) paired into composite radices of at most maxrad
+// (fewer passes over LDS), the pairs that composite_dft is instantiated for.
+Radices column_radices(int n, int maxrad) {
     Radices rd{};
     rd.n = n;
     rd.nrad = -1;
     if (n < 1 || n > kColMax) return rd;
-    int m = n, k = 0;
+    int f[16], nf = 0, m = n;
     for (int r : {5, 3, 4, 2})
-        while (m % r == 0 && k < 8) {
-            rd.r[k++] = r;
+        while (m % r == 0 && nf < 16) {
+            f[nf++] = r;
             m /= r;
         }
-    if (m == 1) rd.nrad = k;
+    if (m != 1) return rd;
+    auto composite = [](int a, int b) {
+        return (b == 2 && a >= 3) || (b == 3 && (a == 3 || a == 5)) || (b == 4 && (a == 3 || a == 4 || a == 5));
+    };
+    int k = 0;
+    for (int i = 0; i < nf && k < 8; ++k) {
+        if (i + 1 < nf && f[i] * f[i + 1] <= maxrad && composite(f[i], f[i + 1])) {
+            rd.a[k] = f[i];
+            rd.b[k] = f[i + 1];
+            i += 2;
+        } else {
+            rd.a[k] = f[i];
+            rd.b[k] = 1;
+            i += 1;
+        }
+    }
+    rd.nrad = k;
     return rd;
 }
 
@@ -800,7 +864,7 @@ void conv_product(Context& c, const double2* X, double2* Y, int nvec, hipStream_
         ConvState::Plan& p = get_plan<R>(c, cs, batch, inter);
         void* buf[1] = {G.p};
         XATU_FFT(rocfft_execute(p.fwd, buf, nullptr, p.info));
-        Radices rd = column_radices(c.lat.mesh_n);
+        Radices rd = column_radices(c.lat.mesh_n, sizeof(R) == 8 ? 16 : 20);
         if (inter && rd.nrad >= 0) {
             rd.count = static_cast<int64_t>(batch) * c.lat.mesh_n;
             constexpr int T = col_tile<R>();
