@@ -127,6 +127,10 @@ constexpr int kConvOrbGroup = 4;  // orbitals per thread in the z contraction
 // vectors per thread in the u and y contractions (register budget: FP64 halves it)
 template <typename R>
 constexpr int conv_vec_group() { return sizeof(R) == 8 ? 2 : 4; }
+// vectors per thread of the y contraction: one, for enough threads to cover the latency of
+// its norb^2 loads per vector (the band coefficients it re-reads hit the caches)
+template <typename R>
+constexpr int conv_y_group() { return 1; }
 
 // u[((n NC + c) norb + be) nk + k] = sum_v conj(V_k[be, v]) x[k NCV + c NV + v, n]
 // thread per k, blockIdx.y = group of vectors
@@ -458,7 +462,7 @@ __global__ __launch_bounds__(256) void conv_y_kernel(int nk, int norb, int nvec,
                                                      const double* __restrict__ eval_k, const double* __restrict__ eval_kq,
                                                      double scissor, const cx_t<R>* __restrict__ W,
                                                      const double2* __restrict__ X, int64_t N, double2* __restrict__ Y) {
-    constexpr int VG = conv_vec_group<R>();
+    constexpr int VG = conv_y_group<R>();
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int n0 = blockIdx.y * VG;
     if (k >= nk) return;
@@ -880,7 +884,8 @@ void conv_product(Context& c, const double2* X, double2* Y, int nvec, hipStream_
         XATU_FFT(rocfft_execute(p.bwd, buf, nullptr, p.info));
         with_bands<R>(c, [&](auto ncc, auto nvc) {
             constexpr int NC = decltype(ncc)::value, NV = decltype(nvc)::value;
-            hipLaunchKernelGGL((conv_y_kernel<R, NC, NV>), dim3(kb.x, vg), dim3(256), 0, s, c.nk, norb, nv, lay,
+            const int yg = (nv + conv_y_group<R>() - 1) / conv_y_group<R>();
+            hipLaunchKernelGGL((conv_y_kernel<R, NC, NV>), dim3(kb.x, yg), dim3(256), 0, s, c.nk, norb, nv, lay,
                                cs.d_gk.p, Ct, Vt, c.d_eval_k.p, c.d_eval_kq.p, c.scissor, G.p, Xb, N, Yb);
         });
         XATU_HIP(hipGetLastError());
