@@ -281,11 +281,8 @@ void launch_matfree32(Context& c, const double2* X, double2* Y, int nvec, hipStr
 // product's precision, and the fixed-order sum Y[row + n N] = sum_s Yp[s][row][n].
 void launch_stage_vectors(const double2* X, int64_t N, int nvec, double2* Xt, hipStream_t s);
 void launch_stage_vectors(const double2* X, int64_t N, int nvec, float2* Xt, hipStream_t s);
-// scale (nullable, nvec entries): Y[row + n N] = scale[n] sum_s Yp[s][row][n].
-void launch_reduce_partials(const double2* Yp, int64_t N, int nvec, int splits, double2* Y, hipStream_t s,
-                            const double* scale = nullptr);
-void launch_reduce_partials(const float2* Yp, int64_t N, int nvec, int splits, double2* Y, hipStream_t s,
-                            const double* scale = nullptr);
+void launch_reduce_partials(const double2* Yp, int64_t N, int nvec, int splits, double2* Y, hipStream_t s);
+void launch_reduce_partials(const float2* Yp, int64_t N, int nvec, int splits, double2* Y, hipStream_t s);
 
 // Solver entry points (davidson.cpp).
 void solve_dense_diag(Context& c, int nstates, double* eigval_out, double* eigvec_out);

@@ -173,14 +173,13 @@ __global__ __launch_bounds__(256, 3) void dense_product_kernel(const cx_t<R>* __
         }
 }
 
-// Y[r + n N] = scale[n] sum_s Yp[s][r][n] for n < nvec (scale null: 1), summed in the
-// output precision in a fixed order. A workgroup owns 64 rows: the partial rows are read
+// Y[r + n N] = sum_s Yp[s][r][n] for n < nvec, summed in the output precision in a fixed
+// order. A workgroup owns 64 rows: the partial rows are read
 // with consecutive threads on consecutive (row, vector) pairs, the sums are transposed
 // through LDS and written column by column.
 template <typename R, typename Rout>
 __global__ __launch_bounds__(256) void reduce_product_kernel(const cx_t<R>* __restrict__ Yp, int64_t N, int nvec,
-                                                             int splits, const double* __restrict__ scale,
-                                                             cx_t<Rout>* __restrict__ Y) {
+                                                             int splits, cx_t<Rout>* __restrict__ Y) {
     constexpr int kRows = 64;
     __shared__ cx_t<Rout> tile[kRows][kVec + 1];
     const int64_t r0 = static_cast<int64_t>(blockIdx.x) * kRows;
@@ -195,11 +194,6 @@ __global__ __launch_bounds__(256) void reduce_product_kernel(const cx_t<R>* __re
                 const cx_t<R> v = Yp[(static_cast<int64_t>(s) * N + r) * kVec + n];
                 sr += static_cast<Rout>(v.x);
                 si += static_cast<Rout>(v.y);
-            }
-            if (scale) {
-                const Rout sc = static_cast<Rout>(scale[n]);
-                sr *= sc;
-                si *= sc;
             }
         }
         tile[rr][n] = mk(sr, si);
@@ -325,7 +319,7 @@ void dense_product(const Context& c, const cx_t<R>* H, const cx_t<Rin>* X, cx_t<
         hipLaunchKernelGGL((dense_product_kernel<R>), dim3(row_blocks * splits), dim3(256), 0, s, H, xt.p, N, kchunk,
                            splits, yp.p);
         hipLaunchKernelGGL((reduce_product_kernel<R, Rout>), dim3((N + 63) / 64), dim3(256), 0, s, yp.p, N, nv,
-                           splits, nullptr, Y + v0 * N);
+                           splits, Y + v0 * N);
     }
     XATU_HIP(hipGetLastError());
 }
@@ -356,15 +350,13 @@ void launch_stage_vectors(const double2* X, int64_t N, int nvec, float2* Xt, hip
     hipLaunchKernelGGL((stage_vectors_kernel<double, float>), dim3((N * kVec + 255) / 256), dim3(256), 0, s, X, N,
                        nvec, Xt);
 }
-void launch_reduce_partials(const double2* Yp, int64_t N, int nvec, int splits, double2* Y, hipStream_t s,
-                            const double* scale) {
+void launch_reduce_partials(const double2* Yp, int64_t N, int nvec, int splits, double2* Y, hipStream_t s) {
     hipLaunchKernelGGL((reduce_product_kernel<double, double>), dim3((N + 63) / 64), dim3(256), 0, s, Yp, N, nvec,
-                       splits, scale, Y);
+                       splits, Y);
 }
-void launch_reduce_partials(const float2* Yp, int64_t N, int nvec, int splits, double2* Y, hipStream_t s,
-                            const double* scale) {
+void launch_reduce_partials(const float2* Yp, int64_t N, int nvec, int splits, double2* Y, hipStream_t s) {
     hipLaunchKernelGGL((reduce_product_kernel<float, double>), dim3((N + 63) / 64), dim3(256), 0, s, Yp, N, nvec,
-                       splits, scale, Y);
+                       splits, Y);
 }
 
 void launch_gram(Context& c, const double2* A, int lda, const double2* B, int ldb, int m, int n, double2* C, int ldc,
