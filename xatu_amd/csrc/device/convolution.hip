@@ -221,25 +221,13 @@ __global__ void conv_multiply_kernel(LatticeDesc lat, int nvec, int64_t M, GLayo
 }
 
 // ---- fused transform along the second mesh axis (row-interleaved layout) ----
-// For 2-D meshes whose side factors into 2, 3, 4 and 5 (at most kColMax), the transform
-// along o1, the product with the spectrum and the inverse transform along o1 run in one
-// pass: a workgroup holds kColTile adjacent columns u = b mesh_n + xi0 (each a length-mesh_n
-// vector of stride batch mesh_n) in LDS and applies mixed-radix Stockham passes (natural
-// order in and out, ping-pong between two buffers), so G is read and written once instead
-// of twice per direction with a transposition in between.
+// For 2-D meshes whose side n = A B factors into two of the lengths 1, 2, 3, 4, 5, 6, 8, 9,
+// 10, 12, 15, 16 (n <= kColMax), the transform along o1, the product with the spectrum and
+// the inverse transform along o1 run in one kernel: a workgroup holds 32 adjacent columns
+// u = b mesh_n + xi0 (each a length-n vector of stride batch mesh_n) and does a four-step
+// transform in registers with two LDS exchanges, so G is read and written once instead of
+// twice per direction with a transposition in between.
 constexpr int kColMax = 128;
-
-struct Radices {
-    int n;          // mesh side
-    int nrad;       // number of passes
-    int a[8], b[8]; // radix of each pass: a (2, 3, 4 or 5) times b (1, or 2 .. 5 for a composite radix)
-    int64_t count;  // columns: batch mesh_n
-};
-
-template <typename R>
-constexpr int col_tile() {
-    return sizeof(R) == 8 ? 16 : 32;
-}
 
 // In-register DFT of length RD; sg = -1 forward (e^{-2 pi i / RD}), +1 inverse.
 template <typename R, int RD>
@@ -317,141 +305,160 @@ __device__ inline void composite_dft(cx_t<R> (&v)[A * B], R sg, const cx_t<R>* t
     }
 }
 
-// One radix-RD Stockham pass over the tile (n x T, rows j, column c contiguous across the
-// tile), RD = A B: butterfly j reads rows j + r n/RD, twiddles them by
-// e^{-/+2 pi i (j mod Ns) r / (Ns RD)} and writes rows (j / Ns) Ns RD + (j mod Ns) + r Ns.
-template <typename R, int T, int A, int B>
-__device__ inline void radix_pass(const cx_t<R>* __restrict__ in, cx_t<R>* __restrict__ out,
-                                  const cx_t<R>* __restrict__ tw, int n, int Ns, R sg) {
-    using C = cx_t<R>;
-    constexpr int RD = A * B;
-    const int nb = n / RD, step = n / (Ns * RD);
-    for (int bi = threadIdx.x; bi < nb * T; bi += blockDim.x) {
-        const unsigned j = static_cast<unsigned>(bi) / T, c = static_cast<unsigned>(bi) % T;
-        const unsigned jh = j / static_cast<unsigned>(Ns), jm = j - jh * static_cast<unsigned>(Ns);
-        C v[RD];
-        v[0] = in[j * T + c];
-#pragma unroll
-        for (int r = 1; r < RD; ++r) {
-            const C w = tw[jm * r * step];
-            v[r] = cmul(in[(j + r * nb) * T + c], mk(w.x, sg < R(0) ? w.y : -w.y));
-        }
-        if constexpr (B == 1)
-            small_dft<R, RD>(v, sg);
-        else
-            composite_dft<R, A, B>(v, sg, tw, n);
-        const unsigned d = jh * Ns * RD + jm;
-#pragma unroll
-        for (int r = 0; r < RD; ++r) out[(d + r * Ns) * T + c] = v[r];
+// DFT of one of the lengths 1, 2, 3, 4, 5, 6, 8, 9, 10, 12, 15, 16.
+template <typename R, int RD>
+__device__ inline void dft_len(cx_t<R> (&a)[RD], R sg, const cx_t<R>* tw, int n) {
+    if constexpr (RD == 1) {
+    } else if constexpr (RD <= 5) {
+        small_dft<R, RD>(a, sg);
+    } else if constexpr (RD == 6) {
+        composite_dft<R, 3, 2>(a, sg, tw, n);
+    } else if constexpr (RD == 8) {
+        composite_dft<R, 4, 2>(a, sg, tw, n);
+    } else if constexpr (RD == 9) {
+        composite_dft<R, 3, 3>(a, sg, tw, n);
+    } else if constexpr (RD == 10) {
+        composite_dft<R, 5, 2>(a, sg, tw, n);
+    } else if constexpr (RD == 12) {
+        composite_dft<R, 3, 4>(a, sg, tw, n);
+    } else if constexpr (RD == 15) {
+        composite_dft<R, 5, 3>(a, sg, tw, n);
+    } else {
+        static_assert(RD == 16, "length");
+        composite_dft<R, 4, 4>(a, sg, tw, n);
     }
 }
 
-// One direction of the transform of the tile's columns; returns the buffer holding the result.
-template <typename R, int T>
-__device__ inline cx_t<R>* column_passes(const Radices& rad, cx_t<R>* in, cx_t<R>* out, const cx_t<R>* tw, R sg) {
-    int Ns = 1;
-    for (int p = 0; p < rad.nrad; ++p) {
-        const int key = rad.a[p] * 8 + rad.b[p];
-        const int n = rad.n;
-        switch (key) {
-            case 2 * 8 + 1: radix_pass<R, T, 2, 1>(in, out, tw, n, Ns, sg); break;
-            case 3 * 8 + 1: radix_pass<R, T, 3, 1>(in, out, tw, n, Ns, sg); break;
-            case 4 * 8 + 1: radix_pass<R, T, 4, 1>(in, out, tw, n, Ns, sg); break;
-            case 5 * 8 + 1: radix_pass<R, T, 5, 1>(in, out, tw, n, Ns, sg); break;
-            case 3 * 8 + 2: radix_pass<R, T, 3, 2>(in, out, tw, n, Ns, sg); break;
-            case 4 * 8 + 2: radix_pass<R, T, 4, 2>(in, out, tw, n, Ns, sg); break;
-            case 5 * 8 + 2: radix_pass<R, T, 5, 2>(in, out, tw, n, Ns, sg); break;
-            case 3 * 8 + 3: radix_pass<R, T, 3, 3>(in, out, tw, n, Ns, sg); break;
-            case 5 * 8 + 3: radix_pass<R, T, 5, 3>(in, out, tw, n, Ns, sg); break;
-            case 3 * 8 + 4: radix_pass<R, T, 3, 4>(in, out, tw, n, Ns, sg); break;
-            case 4 * 8 + 4: radix_pass<R, T, 4, 4>(in, out, tw, n, Ns, sg); break;
-            default: radix_pass<R, T, 5, 4>(in, out, tw, n, Ns, sg); break;  // 5 * 8 + 4
-        }
-        __syncthreads();
-        cx_t<R>* t = in;
-        in = out;
-        out = t;
-        Ns *= rad.a[p] * rad.b[p];
-    }
-    return in;
+constexpr int kFourStepCols = 32; // columns per workgroup (512 threads: 16 per column)
+
+// Stage 1 (thread s < B): rows B j2 + s, A-point DFT, twiddle, Y[s][k1] to LDS.
+template <typename R, int A, int T>
+__device__ inline void four_step_in(const cx_t<R>* gcol, bool live, int64_t count, int B, int s, int c, int n,
+                                    const cx_t<R>* tw, cx_t<R>* ex) {
+    cx_t<R> v[A];
+#pragma unroll
+    for (int j2 = 0; j2 < A; ++j2) v[j2] = live ? gcol[static_cast<int64_t>(B * j2 + s) * count] : mk(R(0), R(0));
+    dft_len<R, A>(v, R(-1), tw, n);
+#pragma unroll
+    for (int k1 = 0; k1 < A; ++k1) ex[(s * A + k1) * T + c] = k1 ? cmul(v[k1], tw[s * k1]) : v[k1];
 }
 
-// G columns u0 .. u0 + T - 1: forward transform along o1, times P[a(al) + NA b(be)][xi0 + n xi1],
-// inverse transform along o1. grid ceil(count / T), block 512 (two workgroups per CU, so that
-// one loads or stores while the other transforms).
+// Stage 2 (threads with act, s < A): Y[j1][s], B-point DFT (xi1 = s + A k2), times the
+// spectrum column pc, inverse B-point DFT, twiddle back, Z[s][j1] to LDS. Every thread of
+// the workgroup calls it (B is uniform) and passes its barrier between the reads and the
+// writes of the shared buffer.
+template <typename R, int B, int T>
+__device__ inline void four_step_mid(bool act, const cx_t<R>* pc, int A, int s, int c, int n, const cx_t<R>* tw,
+                                     cx_t<R>* ex) {
+    cx_t<R> v[B];
+    if (act) {
+#pragma unroll
+        for (int j1 = 0; j1 < B; ++j1) v[j1] = ex[(j1 * A + s) * T + c];
+    }
+    __syncthreads();
+    if (!act) return;
+    dft_len<R, B>(v, R(-1), tw, n);
+    if (pc) {
+#pragma unroll
+        for (int k2 = 0; k2 < B; ++k2) v[k2] = cmul(v[k2], pc[static_cast<int64_t>(n) * (s + A * k2)]);
+    }
+    dft_len<R, B>(v, R(1), tw, n);
+#pragma unroll
+    for (int j1 = 0; j1 < B; ++j1) {
+        const cx_t<R> w = tw[j1 * s];
+        ex[(s * B + j1) * T + c] = (j1 * s) ? cmul(v[j1], mk(w.x, -w.y)) : v[j1];
+    }
+}
+
+// Stage 3 (thread s < B): Z[k1][s], inverse A-point DFT, rows B j2 + s to G.
+template <typename R, int A, int T>
+__device__ inline void four_step_out(cx_t<R>* gcol, bool live, int64_t count, int B, int s, int c, int n,
+                                     const cx_t<R>* tw, const cx_t<R>* ex) {
+    cx_t<R> v[A];
+#pragma unroll
+    for (int k1 = 0; k1 < A; ++k1) v[k1] = ex[(k1 * B + s) * T + c];
+    dft_len<R, A>(v, R(1), tw, n);
+    if (live) {
+#pragma unroll
+        for (int j2 = 0; j2 < A; ++j2) gcol[static_cast<int64_t>(B * j2 + s) * count] = v[j2];
+    }
+}
+
+#define XATU_FOUR_STEP_LENGTHS(X) X(1) X(2) X(3) X(4) X(5) X(6) X(8) X(9) X(10) X(12) X(15) X(16)
+
+// The same fused column transform for n = A B (A >= B, both lengths of dft_len) as a
+// four-step transform in registers: thread (s, c) of column c loads rows B j2 + s (j2 < A)
+// and does the A-point DFTs, twiddles by e^{-2 pi i s k1 / n} and stores Y[s][k1] to LDS;
+// thread (s, c) then takes Y[j1][s] (j1 < B), does the B-point DFT (outputs xi1 = s + A k2),
+// multiplies by the spectrum, inverts the B-point DFT, twiddles back and stores Z[s][j1];
+// thread (s, c) finally takes Z[k1][s] (k1 < A), inverts the A-point DFT and stores rows
+// B j2 + s. Two LDS exchanges instead of a pass per radix. grid ceil(count / 32), block 512.
 template <typename R>
-__global__ __launch_bounds__(512) void conv_columns_kernel(LatticeDesc lat, Radices rad, int nvec, int64_t M,
-                                                           const cx_t<R>* __restrict__ P, cx_t<R>* __restrict__ G) {
+__global__ __launch_bounds__(512) void conv_columns4_kernel(LatticeDesc lat, int n, int A, int B, int64_t count,
+                                                            int nvec, int64_t M, const cx_t<R>* __restrict__ P,
+                                                            cx_t<R>* __restrict__ G) {
     using C = cx_t<R>;
-    constexpr int T = col_tile<R>();
-    __shared__ C b0[kColMax * T], b1[kColMax * T], tw[kColMax];
-    const int n = rad.n;
-    const int64_t count = rad.count;
-    const int64_t u0 = static_cast<int64_t>(blockIdx.x) * T;
-    // every thread keeps one column c of the tile (blockDim is a multiple of T) and rows
-    // j = threadIdx.x / T + multiples of blockDim / T
-    const int c = threadIdx.x % T, j0 = threadIdx.x / T, js = blockDim.x / T;
-    const bool live = u0 + c < count;
-    cx_t<R>* gcol = G + u0 + c;
-    for (int j = j0; j < n; j += js) b0[j * T + c] = live ? gcol[j * count] : mk(R(0), R(0));
+    constexpr int T = kFourStepCols;
+    __shared__ C ex[kColMax * T];
+    __shared__ C tw[kColMax];
+    const int c = threadIdx.x % T, s = threadIdx.x / T;
+    const int64_t u = static_cast<int64_t>(blockIdx.x) * T + c;
+    const bool live = u < count;
     for (int k = threadIdx.x; k < n; k += blockDim.x) {
         double sn, cs;
         sincospi(2.0 * k / n, &sn, &cs);
         tw[k] = mk(static_cast<R>(cs), static_cast<R>(-sn));  // e^{-2 pi i k / n}
     }
     __syncthreads();
-    C* x = column_passes<R, T>(rad, b0, b1, tw, R(-1));
-    {
-        const int64_t u = u0 + c;
-        if (live) {
-            const int64_t b = u / n;
-            const int xi0 = static_cast<int>(u - b * n);
-            const int ab = static_cast<int>(b / nvec);
-            const int al = ab / lat.norb, be = ab % lat.norb;
-            const int pa = atom_of(lat, al) + lat.natoms * atom_of(lat, be);
-            const cx_t<R>* pc = P + static_cast<int64_t>(pa) * M + xi0;
-            for (int j = j0; j < n; j += js) x[j * T + c] = cmul(x[j * T + c], pc[static_cast<int64_t>(n) * j]);
+    C* gcol = G + u;
+    if (s < B) {
+        switch (A) {
+#define XATU_IN(L) \
+    case L: four_step_in<R, L, T>(gcol, live, count, B, s, c, n, tw, ex); break;
+            XATU_FOUR_STEP_LENGTHS(XATU_IN)
+#undef XATU_IN
         }
     }
     __syncthreads();
-    C* y = x == b0 ? b1 : b0;
-    x = column_passes<R, T>(rad, x, y, tw, R(1));
-    if (live)
-        for (int j = j0; j < n; j += js) gcol[j * count] = x[j * T + c];
-}
-
-// Radices of the fused column transform for mesh side n (nrad = -1: not applicable): the
-// factors 5, 3, 4, 2 (in that order) paired into composite radices of at most maxrad
-// (fewer passes over LDS), the pairs that composite_dft is instantiated for.
-Radices column_radices(int n, int maxrad) {
-    Radices rd{};
-    rd.n = n;
-    rd.nrad = -1;
-    if (n < 1 || n > kColMax) return rd;
-    int f[16], nf = 0, m = n;
-    for (int r : {5, 3, 4, 2})
-        while (m % r == 0 && nf < 16) {
-            f[nf++] = r;
-            m /= r;
-        }
-    if (m != 1) return rd;
-    auto composite = [](int a, int b) {
-        return (b == 2 && a >= 3) || (b == 3 && (a == 3 || a == 5)) || (b == 4 && (a == 3 || a == 4 || a == 5));
-    };
-    int k = 0;
-    for (int i = 0; i < nf && k < 8; ++k) {
-        if (i + 1 < nf && f[i] * f[i + 1] <= maxrad && composite(f[i], f[i + 1])) {
-            rd.a[k] = f[i];
-            rd.b[k] = f[i + 1];
-            i += 2;
-        } else {
-            rd.a[k] = f[i];
-            rd.b[k] = 1;
-            i += 1;
+    const C* pc = nullptr;
+    if (live) {
+        const int64_t b = u / n;
+        const int xi0 = static_cast<int>(u - b * n);
+        const int ab = static_cast<int>(b / nvec);
+        const int al = ab / lat.norb, be = ab % lat.norb;
+        pc = P + static_cast<int64_t>(atom_of(lat, al) + lat.natoms * atom_of(lat, be)) * M + xi0;
+    }
+    // B is the same for the whole workgroup: every thread takes the same case and passes the
+    // barrier inside four_step_mid once
+    switch (B) {
+#define XATU_MID(L) \
+    case L: four_step_mid<R, L, T>(s < A, pc, A, s, c, n, tw, ex); break;
+        XATU_FOUR_STEP_LENGTHS(XATU_MID)
+#undef XATU_MID
+    }
+    __syncthreads();
+    if (s < B) {
+        switch (A) {
+#define XATU_OUT(L) \
+    case L: four_step_out<R, L, T>(gcol, live, count, B, s, c, n, tw, ex); break;
+            XATU_FOUR_STEP_LENGTHS(XATU_OUT)
+#undef XATU_OUT
         }
     }
-    rd.nrad = k;
-    return rd;
+}
+
+// Factors (A, B) of the four-step column transform for mesh side n <= kColMax, A >= B, both
+// lengths of dft_len, B as large as possible; A = 0 if none.
+void four_step_factors(int n, int& A, int& B) {
+    static const int len[] = {1, 2, 3, 4, 5, 6, 8, 9, 10, 12, 15, 16};
+    A = B = 0;
+    if (n < 1 || n > kColMax) return;  // the LDS tile and twiddle table hold kColMax rows
+    for (int b : len)
+        for (int a : len)
+            if (a >= b && a * b == n && b > B) {
+                A = a;
+                B = b;
+            }
 }
 
 // Y[k NCV + c NV + v, n] = e(kcv) x - sum_be V_k[be, v] sum_al conj(C_k[al, c]) w[(al norb + be) nvec + n][g(k)]
@@ -868,12 +875,13 @@ void conv_product(Context& c, const double2* X, double2* Y, int nvec, hipStream_
         ConvState::Plan& p = get_plan<R>(c, cs, batch, inter);
         void* buf[1] = {G.p};
         XATU_FFT(rocfft_execute(p.fwd, buf, nullptr, p.info));
-        Radices rd = column_radices(c.lat.mesh_n, sizeof(R) == 8 ? 16 : 20);
-        if (inter && rd.nrad >= 0) {
-            rd.count = static_cast<int64_t>(batch) * c.lat.mesh_n;
-            constexpr int T = col_tile<R>();
-            hipLaunchKernelGGL((conv_columns_kernel<R>), dim3(static_cast<unsigned>((rd.count + T - 1) / T)), dim3(512),
-                               0, s, c.lat, rd, nv, M, P, G.p);
+        int fa = 0, fb = 0;
+        four_step_factors(c.lat.mesh_n, fa, fb);
+        if (inter && fa > 0) {
+            const int64_t count = static_cast<int64_t>(batch) * c.lat.mesh_n;
+            hipLaunchKernelGGL((conv_columns4_kernel<R>),
+                               dim3(static_cast<unsigned>((count + kFourStepCols - 1) / kFourStepCols)), dim3(512), 0, s,
+                               c.lat, c.lat.mesh_n, fa, fb, count, nv, M, P, G.p);
         } else {
             if (p.fwd2) XATU_FFT(rocfft_execute(p.fwd2, buf, nullptr, p.info));
             hipLaunchKernelGGL((conv_multiply_kernel<R>), dim3((M + 255) / 256, batch), dim3(256), 0, s, c.lat, nv, M,
