@@ -209,6 +209,23 @@ def test_split_product_blocks_and_chunks():
     ctx.close()
 
 
+@pytest.mark.parametrize("mesh", [24, 45, 72, 96, 120, 128])
+def test_convolution_column_lengths(mesh):
+    # meshes whose side splits into the larger in-register lengths of the fused column
+    # transform (24 = 6 x 4, 45 = 9 x 5, 72 = 9 x 8, 96 = 12 x 8, 120 = 12 x 10,
+    # 128 = 16 x 8): the convolution product against the resident FP64 matrix
+    print(f"[convolution] hBN mesh {mesh}", flush=True)
+    ex = oracle_case(HBN, mesh, 1, 0, (1, 1, 10), exchange=True)
+    ctx = gpu_context(ex)
+    ctx.build(0)
+    rng = np.random.default_rng(mesh)
+    x = rng.normal(size=(ex.N, 5)) + 1j * rng.normal(size=(ex.N, 5))
+    y = ctx.matvec(x, precision=0)
+    assert rel_err(ctx.matvec(x, precision=0, matrix_free=1), y) < 1e-12
+    assert rel_err(ctx.matvec(x, precision=1, matrix_free=1), y) < 1e-5
+    ctx.close()
+
+
 MF_FORMS = [1, 2]  # XATU_MATRIX_FREE_CONV (k-space convolution), XATU_MATRIX_FREE_TILES (recomputed tiles)
 
 
