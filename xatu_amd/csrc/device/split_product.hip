@@ -454,7 +454,8 @@ __global__ __launch_bounds__(256) void herm_reduce_kernel(const float2* __restri
                                                           double2* __restrict__ Y) {
     constexpr int kRows = 64;
     __shared__ double2 tile[kRows][kVec + 1];
-    const int64_t r0 = static_cast<int64_t>(blockIdx.x) * kRows;
+    // the last rows (the most transposed-block slots) first
+    const int64_t r0 = static_cast<int64_t>(gridDim.x - 1 - blockIdx.x) * kRows;
     const int b = static_cast<int>(r0 / kHermRows);
     const int c0 = static_cast<int>(static_cast<int64_t>(b) * kHermRows / kHermChunk);
 #pragma unroll
