@@ -38,6 +38,7 @@ void launch_residual(const double2* X, const double2* AX, const double* theta, i
 void launch_precondition(const double2* R, const double* theta, const double* diag, int64_t N, int nvec, double2* T,
                          hipStream_t s);
 void launch_unit_vectors(const int* idx, int64_t N, int nvec, double2* V, hipStream_t s);
+void launch_copy_vectors(const double2* src, int64_t n, double2* dst, hipStream_t s);
 
 namespace {
 
@@ -228,9 +229,8 @@ private:
     }
 
     void restart() {
-        const size_t bytes = static_cast<size_t>(N_) * b_ * sizeof(double2);
-        XATU_HIP(hipMemcpyAsync(ws_.V.p, ws_.X.p, bytes, hipMemcpyDeviceToDevice, c_.stream));
-        XATU_HIP(hipMemcpyAsync(ws_.W.p, ws_.AX.p, bytes, hipMemcpyDeviceToDevice, c_.stream));
+        launch_copy_vectors(ws_.X.p, static_cast<int64_t>(N_) * b_, ws_.V.p, c_.stream);
+        launch_copy_vectors(ws_.AX.p, static_cast<int64_t>(N_) * b_, ws_.W.p, c_.stream);
         std::fill(T_.begin(), T_.end(), cd(0.0, 0.0));
         for (int i = 0; i < b_; ++i) T_[i + static_cast<size_t>(i) * mmax_] = cd(theta_[i], 0.0);
         m_ = b_;
@@ -294,8 +294,7 @@ private:
         XATU_HIP(hipMemcpyAsync(ws_.P.p, c_.h_whiten.p, S.size() * sizeof(cd), hipMemcpyHostToDevice, c_.stream));
         XATU_BLAS(rocblas_zgemm(c_.blas, rocblas_operation_none, rocblas_operation_none, n, kept, nb, &kOne, bz(T), n,
                                 bz(ws_.P.p), nb, &kZero, bz(ws_.Tb.p), n));
-        XATU_HIP(hipMemcpyAsync(T, ws_.Tb.p, static_cast<size_t>(N_) * kept * sizeof(double2), hipMemcpyDeviceToDevice,
-                                c_.stream));
+        launch_copy_vectors(ws_.Tb.p, static_cast<int64_t>(N_) * kept, T, c_.stream);
         if (round == 0 && lmin_kept < 1e-8 * lmax) return orthonormalise_slot(kept, 1);
         return kept;
     }

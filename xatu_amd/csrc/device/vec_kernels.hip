@@ -45,6 +45,13 @@ __global__ void precondition_kernel(const double2* __restrict__ R, const double*
     T[i + j * N] = make_double2(r.x / den, r.y / den);
 }
 
+// dst[i] = src[i] for n complex entries (non-overlapping), 2 per thread
+__global__ void copy_vectors_kernel(const double2* __restrict__ src, int64_t n, double2* __restrict__ dst) {
+    const int64_t i = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) * 2;
+    if (i < n) dst[i] = src[i];
+    if (i + 1 < n) dst[i + 1] = src[i + 1];
+}
+
 // V[:, j] = e_{idx[j]}
 __global__ void unit_vectors_kernel(const int* __restrict__ idx, int64_t N, int nvec, double2* __restrict__ V) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -86,6 +93,14 @@ void launch_residual(const double2* X, const double2* AX, const double* theta, i
 void launch_precondition(const double2* R, const double* theta, const double* diag, int64_t N, int nvec, double2* T,
                          hipStream_t s) {
     hipLaunchKernelGGL(precondition_kernel, dim3((N + 255) / 256, nvec), dim3(256), 0, s, R, theta, diag, N, T);
+    XATU_HIP(hipGetLastError());
+}
+
+// Device-to-device copy of vector blocks as a kernel on the stream (no copy-engine
+// hand-over between the solver's kernels).
+void launch_copy_vectors(const double2* src, int64_t n, double2* dst, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(copy_vectors_kernel, dim3(static_cast<unsigned>((n + 511) / 512)), dim3(256), 0, s, src, n, dst);
     XATU_HIP(hipGetLastError());
 }
 
