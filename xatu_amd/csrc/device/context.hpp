@@ -281,6 +281,10 @@ void launch_matfree32(Context& c, const double2* X, double2* Y, int nvec, hipStr
 // product's precision, and the fixed-order sum Y[row + n N] = sum_s Yp[s][row][n].
 void launch_stage_vectors(const double2* X, int64_t N, int nvec, double2* Xt, hipStream_t s);
 void launch_stage_vectors(const double2* X, int64_t N, int nvec, float2* Xt, hipStream_t s);
+// C (N x nb) = alpha A B + beta C, A: N x m (lda), B: m x nb (ldb), FP64 complex, in
+// launches of 32 columns; beta = 0 does not read C. C must not overlap A or B.
+void launch_block_gemm(const double2* A, int lda, const double2* B, int ldb, int64_t N, int m, int nb, double alpha,
+                       double beta, double2* C, int ldc, hipStream_t s);
 void launch_reduce_partials(const double2* Yp, int64_t N, int nvec, int splits, double2* Y, hipStream_t s);
 void launch_reduce_partials(const float2* Yp, int64_t N, int nvec, int splits, double2* Y, hipStream_t s);
 

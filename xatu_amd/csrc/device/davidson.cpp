@@ -6,11 +6,12 @@
 // Block Davidson ("davidson", reference src/davidson.cpp:5-61, redesigned):
 // the basis V, its image W = H V, Ritz vectors and residuals live in HBM in FP64;
 // products H V use the FP64 or the FP32 matrix (or the matrix-free product);
-// the projected matrix V^H W is formed incrementally (rocBLAS zgemm, FP64) and
+// the projected matrix V^H W is formed incrementally (Gram kernel, FP64) and
 // diagonalised on the host (Rayleigh-Ritz); corrections use the diagonal
 // preconditioner r / (theta - diag H), are orthogonalised twice against V
 // (classical Gram-Schmidt, FP64) and orthonormalised by an eigen-decomposition
-// of their Gram matrix, dropping dependent directions. The basis restarts from
+// of their Gram matrix, dropping dependent directions; the block products with the
+// small coefficient matrices run on a tall-by-small kernel. The basis restarts from
 // the current Ritz block when it would exceed max_basis.
 // Unlike the reference, the guess is the unit vectors on the smallest diagonal
 // entries (the reference uses the first 2*nstates unit vectors, :19), no dense
@@ -44,9 +45,6 @@ namespace {
 
 using cd = std::complex<double>;
 
-const rocblas_double_complex kOne{1.0, 0.0};
-const rocblas_double_complex kZero{0.0, 0.0};
-const rocblas_double_complex kMinusOne{-1.0, 0.0};
 
 inline rocblas_double_complex* bz(double2* p) { return reinterpret_cast<rocblas_double_complex*>(p); }
 inline const rocblas_double_complex* bz(const double2* p) {
@@ -208,10 +206,8 @@ private:
         const int n = static_cast<int>(N_);
         EventTimer ev(c_.stream);
         ev.start();
-        XATU_BLAS(rocblas_zgemm(c_.blas, rocblas_operation_none, rocblas_operation_none, n, b_, m_, &kOne,
-                                bz(ws_.V.p), n, bz(ws_.Y.p), m_, &kZero, bz(ws_.X.p), n));
-        XATU_BLAS(rocblas_zgemm(c_.blas, rocblas_operation_none, rocblas_operation_none, n, b_, m_, &kOne,
-                                bz(ws_.W.p), n, bz(ws_.Y.p), m_, &kZero, bz(ws_.AX.p), n));
+        launch_block_gemm(ws_.V.p, n, ws_.Y.p, m_, N_, m_, b_, 1.0, 0.0, ws_.X.p, n, c_.stream);
+        launch_block_gemm(ws_.W.p, n, ws_.Y.p, m_, N_, m_, b_, 1.0, 0.0, ws_.AX.p, n, c_.stream);
         ev.stop();
         int nblk = 0;
         launch_residual(ws_.X.p, ws_.AX.p, ws_.theta.p, N_, b_, ws_.R.p, ws_.partial.p, &nblk, c_.stream);
@@ -254,8 +250,7 @@ private:
         for (int pass = 0; pass < 2; ++pass) {
             launch_gram(c_, ws_.V.p, n, T, n, mn, nb, ws_.P.p, mn, c_.stream);
             if (m_ > 0) {
-                XATU_BLAS(rocblas_zgemm(c_.blas, rocblas_operation_none, rocblas_operation_none, n, nb, m_,
-                                        &kMinusOne, bz(ws_.V.p), n, bz(ws_.P.p), mn, &kOne, bz(T), n));
+                launch_block_gemm(ws_.V.p, n, ws_.P.p, mn, N_, m_, nb, -1.0, 1.0, T, n, c_.stream);
             }
         }
         ev.stop();
@@ -292,8 +287,7 @@ private:
         // V[:, m:m+kept] = T S (through the scratch block: the product may not overlap its input)
         std::copy(S.begin(), S.end(), reinterpret_cast<cd*>(c_.h_whiten.p));
         XATU_HIP(hipMemcpyAsync(ws_.P.p, c_.h_whiten.p, S.size() * sizeof(cd), hipMemcpyHostToDevice, c_.stream));
-        XATU_BLAS(rocblas_zgemm(c_.blas, rocblas_operation_none, rocblas_operation_none, n, kept, nb, &kOne, bz(T), n,
-                                bz(ws_.P.p), nb, &kZero, bz(ws_.Tb.p), n));
+        launch_block_gemm(T, n, ws_.P.p, nb, N_, nb, kept, 1.0, 0.0, ws_.Tb.p, n, c_.stream);
         launch_copy_vectors(ws_.Tb.p, static_cast<int64_t>(N_) * kept, T, c_.stream);
         if (round == 0 && lmin_kept < 1e-8 * lmax) return orthonormalise_slot(kept, 1);
         return kept;

@@ -208,6 +208,54 @@ __global__ __launch_bounds__(256) void reduce_product_kernel(const cx_t<R>* __re
     }
 }
 
+// C (N x nb) = alpha A B + beta C (beta = 0: C is not read) for a tall A (N x m,
+// column-major, leading dimension lda) and a small B (m x nb, ldb), nb <= 32, FP64
+// complex: 64 rows per workgroup, one wave per 8 columns, B staged in LDS 32 rows of k at
+// a time (its entries are read by the whole wave at once).
+__global__ __launch_bounds__(256) void block_gemm_kernel(const double2* __restrict__ A, int lda,
+                                                         const double2* __restrict__ B, int ldb, int64_t N, int m,
+                                                         int nb, double alpha, double beta, double2* __restrict__ C,
+                                                         int ldc) {
+    __shared__ double2 Bs[32][kVec];
+    const int r = threadIdx.x & 63, cg = threadIdx.x >> 6;
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * 64 + r;
+    double2 acc[8];
+#pragma unroll
+    for (int jj = 0; jj < 8; ++jj) acc[jj] = make_double2(0.0, 0.0);
+    for (int k0 = 0; k0 < m; k0 += 32) {
+        const int kc = min(32, m - k0);
+        __syncthreads();
+        for (int t = threadIdx.x; t < 32 * kVec; t += 256) {
+            const int kk = t % 32, j = t / 32;
+            Bs[kk][j] = (kk < kc && j < nb) ? B[(k0 + kk) + static_cast<int64_t>(j) * ldb] : make_double2(0.0, 0.0);
+        }
+        __syncthreads();
+        if (i < N) {
+            for (int kk = 0; kk < kc; ++kk) {
+                const double2 a = A[i + static_cast<int64_t>(k0 + kk) * lda];
+#pragma unroll
+                for (int jj = 0; jj < 8; ++jj) cfma(acc[jj], a, Bs[kk][8 * cg + jj]);
+            }
+        }
+    }
+    if (i < N) {
+#pragma unroll
+        for (int jj = 0; jj < 8; ++jj) {
+            const int j = 8 * cg + jj;
+            if (j < nb) {
+                double2* cp = C + i + static_cast<int64_t>(j) * ldc;
+                double2 v = make_double2(alpha * acc[jj].x, alpha * acc[jj].y);
+                if (beta != 0.0) {
+                    const double2 o = *cp;
+                    v.x += beta * o.x;
+                    v.y += beta * o.y;
+                }
+                *cp = v;
+            }
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Gram kernel: C (m x n) = A^H B over N rows, FP64 complex. Tile 64 x 32 per
 // workgroup, each thread a 4 x 2 micro-tile; rows split into chunks, partial
@@ -357,6 +405,17 @@ void launch_reduce_partials(const double2* Yp, int64_t N, int nvec, int splits, 
 void launch_reduce_partials(const float2* Yp, int64_t N, int nvec, int splits, double2* Y, hipStream_t s) {
     hipLaunchKernelGGL((reduce_product_kernel<float, double>), dim3((N + 63) / 64), dim3(256), 0, s, Yp, N, nvec,
                        splits, Y);
+}
+
+void launch_block_gemm(const double2* A, int lda, const double2* B, int ldb, int64_t N, int m, int nb, double alpha,
+                       double beta, double2* C, int ldc, hipStream_t s) {
+    for (int j0 = 0; j0 < nb; j0 += kVec) {
+        const int nj = std::min(kVec, nb - j0);
+        hipLaunchKernelGGL(block_gemm_kernel, dim3(static_cast<unsigned>((N + 63) / 64)), dim3(256), 0, s, A, lda,
+                           B + static_cast<int64_t>(j0) * ldb, ldb, N, m, nj, alpha, beta,
+                           C + static_cast<int64_t>(j0) * ldc, ldc);
+    }
+    XATU_HIP(hipGetLastError());
 }
 
 void launch_gram(Context& c, const double2* A, int lda, const double2* B, int ldb, int m, int n, double2* C, int ldc,
