@@ -231,6 +231,8 @@ __global__ __launch_bounds__(256) void block_gemm_kernel(const double2* __restri
         }
         __syncthreads();
         if (i < N) {
+            // unrolled so that several column loads of A are in flight together
+#pragma unroll 8
             for (int kk = 0; kk < kc; ++kk) {
                 const double2 a = A[i + static_cast<int64_t>(k0 + kk) * lda];
 #pragma unroll
@@ -276,18 +278,37 @@ __global__ __launch_bounds__(256) void gram_partial_kernel(const double2* __rest
     for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) acc[i][j] = mk(0.0, 0.0);
-    for (int64_t r0 = rbeg; r0 < rend; r0 += kGR) {
-        for (int e = threadIdx.x; e < kGR * kGM; e += 256) {
-            const int rr = e % kGR, cc = e / kGR;
+    // the next 16-row slab is loaded into registers while the current one is used
+    static_assert(kGR * kGM == 4 * 256 && kGR * kGN == 2 * 256, "one slab = 4 A and 2 B entries per thread");
+    double2 ra[4], rb[2];
+    auto fetch = [&](int64_t r0) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int e = threadIdx.x + 256 * q, rr = e % kGR, cc = e / kGR;
             const int64_t r = r0 + rr;
-            sA[rr][cc] = (r < rend && m0 + cc < m) ? A[r + static_cast<int64_t>(m0 + cc) * lda] : mk(0.0, 0.0);
+            ra[q] = (r < rend && m0 + cc < m) ? A[r + static_cast<int64_t>(m0 + cc) * lda] : mk(0.0, 0.0);
         }
-        for (int e = threadIdx.x; e < kGR * kGN; e += 256) {
-            const int rr = e % kGR, cc = e / kGR;
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int e = threadIdx.x + 256 * q, rr = e % kGR, cc = e / kGR;
             const int64_t r = r0 + rr;
-            sB[rr][cc] = (r < rend && n0 + cc < n) ? B[r + static_cast<int64_t>(n0 + cc) * ldb] : mk(0.0, 0.0);
+            rb[q] = (r < rend && n0 + cc < n) ? B[r + static_cast<int64_t>(n0 + cc) * ldb] : mk(0.0, 0.0);
+        }
+    };
+    if (rbeg < rend) fetch(rbeg);
+    for (int64_t r0 = rbeg; r0 < rend; r0 += kGR) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int e = threadIdx.x + 256 * q;
+            sA[e % kGR][e / kGR] = ra[q];
+        }
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int e = threadIdx.x + 256 * q;
+            sB[e % kGR][e / kGR] = rb[q];
         }
         __syncthreads();
+        if (r0 + kGR < rend) fetch(r0 + kGR);
 #pragma unroll 4
         for (int rr = 0; rr < kGR; ++rr) {
             double2 a[4], b[2];
