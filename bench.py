@@ -158,7 +158,7 @@ def main():
     os.dup2(2, 1)
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path runs on a single "
-              "MI355X; see DESIGN.md for how it would shard)", file=sys.stderr)
+              "MI355X; the design notes describe how it would shard)", file=sys.stderr)
         sys.exit(2)
 
     import xatu_amd as X
