@@ -210,33 +210,33 @@ __global__ __launch_bounds__(256) void reduce_product_kernel(const cx_t<R>* __re
 
 // C (N x nb) = alpha A B + beta C (beta = 0: C is not read) for a tall A (N x m,
 // column-major, leading dimension lda) and a small B (m x nb, ldb), nb <= 32, FP64
-// complex: 64 rows per workgroup, one wave per 8 columns, B staged in LDS 32 rows of k at
-// a time (its entries are read by the whole wave at once).
+// complex: 64 rows per workgroup, one wave per 8 columns. The B entries a wave needs are
+// the same for all its lanes, so they are read with uniform addresses (scalar loads) and
+// feed the FMAs as scalar operands: no LDS traffic and no barriers.
 __global__ __launch_bounds__(256) void block_gemm_kernel(const double2* __restrict__ A, int lda,
                                                          const double2* __restrict__ B, int ldb, int64_t N, int m,
                                                          int nb, double alpha, double beta, double2* __restrict__ C,
                                                          int ldc) {
-    __shared__ double2 Bs[32][kVec];
-    const int r = threadIdx.x & 63, cg = threadIdx.x >> 6;
+    const int r = threadIdx.x & 63;
+    const int cg = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
     const int64_t i = static_cast<int64_t>(blockIdx.x) * 64 + r;
+    const int64_t ia = i < N ? i : N - 1;  // rows past N load row N-1 and are not stored
+    const int jn = nb - 8 * cg;            // columns of this wave present (uniform)
     double2 acc[8];
 #pragma unroll
     for (int jj = 0; jj < 8; ++jj) acc[jj] = make_double2(0.0, 0.0);
-    for (int k0 = 0; k0 < m; k0 += 32) {
-        const int kc = min(32, m - k0);
-        __syncthreads();
-        for (int t = threadIdx.x; t < 32 * kVec; t += 256) {
-            const int kk = t % 32, j = t / 32;
-            Bs[kk][j] = (kk < kc && j < nb) ? B[(k0 + kk) + static_cast<int64_t>(j) * ldb] : make_double2(0.0, 0.0);
-        }
-        __syncthreads();
-        if (i < N) {
-            // unrolled so that several column loads of A are in flight together
-#pragma unroll 8
-            for (int kk = 0; kk < kc; ++kk) {
-                const double2 a = A[i + static_cast<int64_t>(k0 + kk) * lda];
+    if (jn > 0) {
+        const double2* Bc = B + static_cast<int64_t>(8 * cg) * ldb;
+        const double2* Ar = A + ia;
+        // unrolled so that several column loads of A are in flight together
+#pragma unroll 4
+        for (int k = 0; k < m; ++k) {
+            const double2 a = Ar[static_cast<int64_t>(k) * lda];
 #pragma unroll
-                for (int jj = 0; jj < 8; ++jj) cfma(acc[jj], a, Bs[kk][8 * cg + jj]);
+            for (int jj = 0; jj < 8; ++jj) {
+                // a column past nb repeats the last one present; its sum is not stored
+                const double2 b = Bc[k + static_cast<int64_t>(min(jj, jn - 1)) * ldb];
+                cfma(acc[jj], a, b);
             }
         }
     }
