@@ -210,10 +210,14 @@ __global__ __launch_bounds__(256) void reduce_product_kernel(const cx_t<R>* __re
 
 // C (N x nb) = alpha A B + beta C (beta = 0: C is not read) for a tall A (N x m,
 // column-major, leading dimension lda) and a small B (m x nb, ldb), nb <= 32, FP64
-// complex: 64 rows per workgroup, one wave per 8 columns. The B entries a wave needs are
+// complex: 64 rows per workgroup, one wave per kBgCols columns. The B entries a wave needs are
 // the same for all its lanes, so they are read with uniform addresses (scalar loads) and
-// feed the FMAs as scalar operands: no LDS traffic and no barriers.
-__global__ __launch_bounds__(256) void block_gemm_kernel(const double2* __restrict__ A, int lda,
+// feed the FMAs as scalar operands: no LDS traffic and no barriers. Four columns per wave
+// (eight waves per workgroup) measured faster than two, eight or sixteen: the kernel is
+// bound by loads in flight, and more waves keep more of them in flight.
+constexpr int kBgCols = 4, kBgWaves = kVec / kBgCols;
+static_assert(kVec % kBgCols == 0, "whole waves per column block");
+__global__ __launch_bounds__(64 * kBgWaves) void block_gemm_kernel(const double2* __restrict__ A, int lda,
                                                          const double2* __restrict__ B, int ldb, int64_t N, int m,
                                                          int nb, double alpha, double beta, double2* __restrict__ C,
                                                          int ldc) {
@@ -221,19 +225,19 @@ __global__ __launch_bounds__(256) void block_gemm_kernel(const double2* __restri
     const int cg = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
     const int64_t i = static_cast<int64_t>(blockIdx.x) * 64 + r;
     const int64_t ia = i < N ? i : N - 1;  // rows past N load row N-1 and are not stored
-    const int jn = nb - 8 * cg;            // columns of this wave present (uniform)
-    double2 acc[8];
+    const int jn = nb - kBgCols * cg;      // columns of this wave present (uniform)
+    double2 acc[kBgCols];
 #pragma unroll
-    for (int jj = 0; jj < 8; ++jj) acc[jj] = make_double2(0.0, 0.0);
+    for (int jj = 0; jj < kBgCols; ++jj) acc[jj] = make_double2(0.0, 0.0);
     if (jn > 0) {
-        const double2* Bc = B + static_cast<int64_t>(8 * cg) * ldb;
+        const double2* Bc = B + static_cast<int64_t>(kBgCols * cg) * ldb;
         const double2* Ar = A + ia;
         // unrolled so that several column loads of A are in flight together
 #pragma unroll 4
         for (int k = 0; k < m; ++k) {
             const double2 a = Ar[static_cast<int64_t>(k) * lda];
 #pragma unroll
-            for (int jj = 0; jj < 8; ++jj) {
+            for (int jj = 0; jj < kBgCols; ++jj) {
                 // a column past nb repeats the last one present; its sum is not stored
                 const double2 b = Bc[k + static_cast<int64_t>(min(jj, jn - 1)) * ldb];
                 cfma(acc[jj], a, b);
@@ -242,8 +246,8 @@ __global__ __launch_bounds__(256) void block_gemm_kernel(const double2* __restri
     }
     if (i < N) {
 #pragma unroll
-        for (int jj = 0; jj < 8; ++jj) {
-            const int j = 8 * cg + jj;
+        for (int jj = 0; jj < kBgCols; ++jj) {
+            const int j = kBgCols * cg + jj;
             if (j < nb) {
                 double2* cp = C + i + static_cast<int64_t>(j) * ldc;
                 double2 v = make_double2(alpha * acc[jj].x, alpha * acc[jj].y);
@@ -432,7 +436,7 @@ void launch_block_gemm(const double2* A, int lda, const double2* B, int ldb, int
                        double beta, double2* C, int ldc, hipStream_t s) {
     for (int j0 = 0; j0 < nb; j0 += kVec) {
         const int nj = std::min(kVec, nb - j0);
-        hipLaunchKernelGGL(block_gemm_kernel, dim3(static_cast<unsigned>((N + 63) / 64)), dim3(256), 0, s, A, lda,
+        hipLaunchKernelGGL(block_gemm_kernel, dim3(static_cast<unsigned>((N + 63) / 64)), dim3(64 * kBgWaves), 0, s, A, lda,
                            B + static_cast<int64_t>(j0) * ldb, ldb, N, m, nj, alpha, beta,
                            C + static_cast<int64_t>(j0) * ldc, ldc);
     }
