@@ -135,7 +135,7 @@ const char* xatu_last_error(const xatu_ctx* ctx);
 
 /* Lattice and k-mesh geometry (Lattice.cpp:32-47,140-162).
  * bravais: ndim x 3 row-major (row = lattice vector), Angstrom.
- * motif: natoms x 3 row-major atomic positions; norb_per_atom: orbitals of each atom
+ * motif: natoms x 3 row-major atomic positions (1 <= natoms <= 16); norb_per_atom: orbitals of each atom
  * (the basis orders orbitals atom by atom, SystemTB/ExcitonTB.cpp:535-543).
  * mesh_n: points per axis of the full Monkhorst-Pack mesh the lattice sums are taken on
  * (ncell*factor, Lattice.cpp:172-197). */

@@ -88,6 +88,59 @@ double coulomb(double r, const Pot& p) {
     return (r != 0) ? EC_ / (4E-10 * PI_ * EPS0_ * r) : EC_ * 1E10 / (4 * PI_ * EPS0_ * p.reg);
 }
 
+// g(x) = d(x f)/dx = f + x f' for f = STVH0(x) - y0(x), the function the reference's
+// Keldysh potential evaluates (no reference counterpart: the r0 derivative of that
+// potential, for the gradient checks).
+//  x <= 5:      f + x (2/pi - H1 + Y1), H1 by its power series, libm y1;
+//  5 < x <= 20: f and x f' cancel to O(x^-3), so g comes from the integral
+//               f(x) = (2/pi) int_0^inf e^{-xt} (1 + t^2)^(-1/2) dt, which gives
+//               g(x) = (2/pi) int_0^inf e^{-u} (u^2/x^3) (1 + u^2/x^2)^(-3/2) du,
+//               summed with the exp-sinh rule u = exp(pi/2 sinh t), step 1/32, t in [-4, 4];
+//  x > 20:      the exact derivative of STVH0's asymptotic branch (src/ExcitonTB.cpp:321-336)
+//               minus y0: with H0 = 2/(pi x) S + BY0, the S/x terms cancel analytically and
+//               g = (2/pi) S'(x) + (BY0 - y0) + x (BY0' + y1), over the same series terms.
+double keldysh_dxf(double x) {
+    if (x <= 5.0) return stvh0(x) - y0(x) + x * (2.0 / PI_ - h1_series(x) + y1(x));
+    if (x <= 20.0) {
+        const double h = 1.0 / 32;
+        double s = 0.0;
+        for (int i = -128; i <= 128; ++i) {
+            const double t = i * h;
+            const double u = std::exp(0.5 * PI_ * std::sinh(t));
+            const double q = 1.0 + (u / x) * (u / x);
+            s += std::exp(-u) * u * u / (q * std::sqrt(q)) * u * 0.5 * PI_ * std::cosh(t);
+        }
+        return 2.0 / PI_ * h * s / (x * x * x);
+    }
+    int km = int(0.5 * (x + 1.0));
+    if (x >= 50.0) km = 25;
+    double S = 1.0, R = 1.0, dS = 0.0;
+    for (int k = 1; k <= km; ++k) {
+        R = -R * std::pow((2.0 * k - 1.0) / x, 2);  // R_k = (-1)^k ((2k-1)!!)^2 x^-2k
+        S += R;
+        dS += -2.0 * k * R / x;
+        if (std::fabs(R) < std::fabs(S) * 1.0e-12) break;
+    }
+    // BY0 = 2 x^-1/2 (P0(t^2) sin th + t Qp(t^2) cos th), t = 4/x, th = x - pi/4
+    const double t = 4.0 / x, t2 = t * t, dt = -t * t / 4.0;
+    const double P = ((((-.37043e-5 * t2 + .173565e-4) * t2 - .487613e-4) * t2 + .17343e-3) * t2 - 0.1753062e-2) * t2 +
+                     .3989422793;
+    const double dP = (((5 * -.37043e-5 * t2 + 4 * .173565e-4) * t2 - 3 * .487613e-4) * t2 + 2 * .17343e-3) * t2 -
+                      0.1753062e-2;  // dP/d(t^2)
+    const double Qp = ((((.32312e-5 * t2 - 0.142078e-4) * t2 + 0.342468e-4) * t2 - 0.869791e-4) * t2 + 0.4564324e-3) * t2 -
+                      0.0124669441;
+    const double dQp = (((5 * .32312e-5 * t2 - 4 * 0.142078e-4) * t2 + 3 * 0.342468e-4) * t2 - 2 * 0.869791e-4) * t2 +
+                       0.4564324e-3;  // dQp/d(t^2)
+    const double Q = t * Qp;
+    const double dPx = dP * 2.0 * t * dt;                  // dP0/dx
+    const double dQx = (Qp + t * dQp * 2.0 * t) * dt;      // dQ0/dx
+    const double th = x - 0.25 * PI_, sn = std::sin(th), cs = std::cos(th);
+    const double rx = 1.0 / std::sqrt(x);
+    const double by0 = 2.0 * rx * (P * sn + Q * cs);
+    const double dby0 = -rx / x * (P * sn + Q * cs) + 2.0 * rx * (dPx * sn + P * cs + dQx * cs - Q * sn);
+    return 2.0 / PI_ * dS + (by0 - y0(x)) + x * (dby0 + y1(x));
+}
+
 double keldysh_dr0(double r, const Pot& p) {
     const double eps_bar = (p.eps_m + p.eps_s) / 2;
     double x;
@@ -98,15 +151,7 @@ double keldysh_dr0(double r, const Pot& p) {
         x = std::fabs(r) / p.r0;
     }
     const double C = EC_ / (8E-10 * EPS0_ * eps_bar);
-    const double f = stvh0(x) - y0(x);
-    double fp;
-    if (x <= 20.0) {
-        fp = 2.0 / PI_ - h1_series(x) + y1(x);
-    } else {
-        const double i2 = 1.0 / (x * x);
-        fp = 2.0 / PI_ - (2.0 / PI_) * (1.0 + i2 * (1.0 + i2 * (-3.0 + i2 * 45.0)));
-    }
-    return -C / (p.r0 * p.r0) * (f + x * fp);
+    return -C / (p.r0 * p.r0) * keldysh_dxf(x);
 }
 
 double potential(double r, const Pot& p) {
@@ -155,6 +200,7 @@ extern "C" {
 double oracle_stvh0(double x) { return stvh0(x); }
 double oracle_y0(double x) { return y0(x); }
 double oracle_h1(double x) { return h1_series(x); }
+double oracle_keldysh_dxf(double x) { return keldysh_dxf(x); }
 
 double oracle_potential(int kind, double r, double eps_m, double eps_s, double r0, double reg, double cutoff) {
     Pot p{kind, eps_m, eps_s, r0, reg, cutoff};

@@ -56,6 +56,8 @@ def lib():
         L.oracle_y0.argtypes = [ctypes.c_double]
         L.oracle_h1.restype = ctypes.c_double
         L.oracle_h1.argtypes = [ctypes.c_double]
+        L.oracle_keldysh_dxf.restype = ctypes.c_double
+        L.oracle_keldysh_dxf.argtypes = [ctypes.c_double]
         L.oracle_potential.restype = ctypes.c_double
         L.oracle_potential.argtypes = [ctypes.c_int] + [ctypes.c_double] * 6
         L.oracle_truncate_supercell.restype = ctypes.c_int

@@ -151,3 +151,41 @@ def test_autograd_function_wrapper(request):
     fd = np.array(fd)
     assert np.abs(p.grad.numpy() - fd).max() <= 1e-4 * np.abs(fd).max()
     base.close()
+
+
+@pytest.mark.parametrize("r0", [1.0, 0.3])
+def test_small_screening_length_branches(r0):
+    # hBN 30 x 30, cutoff 20 cells (|R| up to 50 A) with r0 = 1 A and 0.3 A: x = r / r0 reaches
+    # 50 and 167, so the lattice sums and dV/dr0 take the asymptotic branches (x > 20, x >= 50);
+    # lattice sums to 1e-12, the FP64 matrix to 1e-10, dE/dr0 against the oracle's explicit dH
+    # and against central differences of the oracle spectrum
+    print(f"[small r0] hBN 30x30 cutoff 20 r0 {r0}", flush=True)
+    kw = dict(cutoff=20, exchange=True, Q=(0.01, 0.0, 0.0))
+    ex = oracle_case(MODELS / "hBN.model", 30, 1, 0, (1.0, 1.0, r0), **kw)
+    assert max(np.linalg.norm(ex.cells, axis=1)) / r0 > (40 if r0 == 1.0 else 150)
+    H = ex.bse()
+    w, V = np.linalg.eigh(H)
+    ns = 6
+    psi = V[:, :ns]
+    ctx = gpu_context(ex)
+    ft, ftq = ctx.get_fourier()
+    assert np.abs(ft - ex.ft).max() <= 1e-12 * np.abs(ex.ft).max()
+    assert np.abs(ftq - ex.ftq).max() <= 1e-12 * np.abs(ex.ftq).max()
+    Hg = ctx.build(0, copy_out=True)
+    assert np.abs(Hg - H).max() <= 1e-10 * np.abs(H).max()
+    g = ctx.hf_gradients(ns, [R0, EPS_S], eigvec=psi)
+    ctx.close()
+    ref_r0 = explicit(ex, psi, O.POT_KELDYSH_DR0)
+    print(f"[small r0 {r0}] dE/dr0 {g[:, 0]}")
+    assert np.abs(g[:, 0] - ref_r0).max() <= 1e-9 * np.abs(ref_r0).max()
+    # central differences of the oracle spectrum with a step of 1e-3 r0: near x = 20 the
+    # reference's power series for H0 carries cancellation noise (~5e-9 relative) that a
+    # step of 1e-5 r0 would amplify to ~1e-4 of the gradient
+    h = 1e-3 * r0
+
+    def energies(r):
+        e = oracle_case(MODELS / "hBN.model", 30, 1, 0, (1.0, 1.0, r), **kw)
+        return np.linalg.eigvalsh(e.bse())[:ns]
+
+    fd = (energies(r0 + h) - energies(r0 - h)) / (2 * h)
+    assert np.all(np.abs(fd - g[:, 0]) <= 1e-4 * np.abs(fd)), (fd, g[:, 0])

@@ -160,6 +160,34 @@ def test_keldysh_closed_form_and_special_cases():
         1.6021766e-19 * 1e10 / (4 * np.pi * 8.8541878e-12 * reg), rel=1e-15)
 
 
+def _dxf_exact(x):
+    # g(x) = H0 - Y0 + x (2/pi - H1 + Y1) at 40 significant digits
+    import mpmath as mp
+    with mp.workdps(40):
+        x = mp.mpf(x)
+        return float(mp.struveh(0, x) - mp.bessely(0, x) + x * (2 / mp.pi - mp.struveh(1, x) + mp.bessely(1, x)))
+
+
+@pytest.mark.parametrize("x", [1e-3, 0.1, 0.7, 2.0, 4.99, 5.01, 7.0, 12.0, 16.5, 19.99])
+def test_dvdr0_kernel_matches_mpmath(x):
+    # d(x f)/dx with f = H0 - Y0, the factor of dV/dr0, up to x = 20 where the reference's H0
+    # is its power series: the series branch below 5 and the integral form above, against
+    # 40-digit values; the tolerance is the series branch's
+    L = O.lib()
+    ref = _dxf_exact(x)
+    assert abs(L.oracle_keldysh_dxf(x) - ref) <= 5e-13 * abs(ref), (x, L.oracle_keldysh_dxf(x), ref)
+
+
+@pytest.mark.parametrize("x", [20.01, 27.0, 45.0, 60.0, 150.0])
+def test_dvdr0_kernel_asymptotic_branch_bounded(x):
+    # beyond 20 the factor differentiates the reference's asymptotic H0 (published Zhang-Jin
+    # branch with its rational Y0 approximation), which departs from the exact derivative by
+    # up to ~1e-3 relative at x <= 60 and 3e-2 at 150
+    ref = _dxf_exact(x)
+    tol = 2e-3 if x <= 60 else 5e-2
+    assert abs(O.lib().oracle_keldysh_dxf(x) - ref) <= tol * abs(ref)
+
+
 @pytest.mark.parametrize("r", [0.0, 1.0, 4.3, 25.0])
 def test_dvdr0_matches_finite_difference(r):
     L = O.lib()
@@ -168,6 +196,19 @@ def test_dvdr0_matches_finite_difference(r):
     fd = (L.oracle_potential(0, r, em, es, r0 + h, reg, cut) - L.oracle_potential(0, r, em, es, r0 - h, reg, cut)) / (2 * h)
     an = L.oracle_potential(2, r, em, es, r0, reg, cut)
     assert abs(fd - an) <= 1e-7 * abs(an)
+
+
+@pytest.mark.parametrize("r", [20.5, 26.0, 37.5, 44.0, 55.0, 58.0, 130.0])
+def test_dvdr0_asymptotic_range_matches_finite_difference(r):
+    # r0 = 1: x = r / r0 beyond 20, where the reference's STVH0 takes its asymptotic branch:
+    # central differences of the reference's V against the analytic dV/dr0 (away from the odd
+    # x where that branch changes its number of terms and V jumps)
+    L = O.lib()
+    em, es, r0, reg, cut = 1.0, 1.0, 1.0, 2.5, 200.0
+    h = 1e-5 * r0
+    fd = (L.oracle_potential(0, r, em, es, r0 + h, reg, cut) - L.oracle_potential(0, r, em, es, r0 - h, reg, cut)) / (2 * h)
+    an = L.oracle_potential(2, r, em, es, r0, reg, cut)
+    assert abs(fd - an) <= 1e-6 * abs(an), (fd, an)
 
 
 def test_deps_kind():

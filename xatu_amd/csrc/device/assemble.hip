@@ -452,6 +452,193 @@ __global__ __launch_bounds__(matfree_threads(TK)) void bse_matfree_kernel(
     }
 }
 
+// ---- generic shapes: any natoms <= kMaxAtoms, any nv and nc ----
+// One element per thread, coefficients read straight from the stacks. The element
+// (row, col) is evaluated for the canonical pair i = min(row, col) <= j = max(row, col)
+// with an instruction sequence that depends only on (i, j), so H(j, i) = conj H(i, j)
+// holds bit for bit. R is the arithmetic of the assembly: coefficients, lattice sums
+// and exchange factors are rounded to R first, as in the specialised kernels.
+template <typename R>
+__device__ inline cx_t<R> generic_dx(const LatticeDesc& lat, int nv, int nc, int64_t i, int64_t j,
+                                     const double2* __restrict__ evk, const double2* __restrict__ evkq,
+                                     const int* __restrict__ kcoord, const double2* __restrict__ ft,
+                                     const double2* __restrict__ e1, const double2* __restrict__ e2, int exchange) {
+    using C = cx_t<R>;
+    const int ncv = nc * nv, nb = nc + nv, norb = lat.norb, na = lat.natoms;
+    const int ka = static_cast<int>(i / ncv), ia = static_cast<int>(i % ncv);
+    const int kb = static_cast<int>(j / ncv), ib = static_cast<int>(j % ncv);
+    const int cA = ia / nv, vA = ia % nv, cB = ib / nv, vB = ib % nv;
+    const double2* Ca = evkq + (static_cast<size_t>(ka) * nb + nv + cA) * norb;
+    const double2* Va = evk + (static_cast<size_t>(ka) * nb + vA) * norb;
+    const double2* Cb = evkq + (static_cast<size_t>(kb) * nb + nv + cB) * norb;
+    const double2* Vb = evk + (static_cast<size_t>(kb) * nb + vB) * norb;
+    const double2* M = ft + static_cast<size_t>(table_index(lat, kcoord, ka, kb)) * na * na;
+    // rho^v_b = sum_{beta in b} conj(V_kb[beta, vB]) V_ka[beta, vA]
+    C rv[kMaxAtoms];
+    for (int b = 0; b < na; ++b) {
+        C sacc = mk(R(0), R(0));
+        for (int be = lat.orb_begin[b]; be < lat.orb_begin[b + 1]; ++be) cfma_conj(sacc, to_cx<R>(Vb[be]), to_cx<R>(Va[be]));
+        rv[b] = sacc;
+    }
+    C D = mk(R(0), R(0)), X = mk(R(0), R(0));
+    for (int a = 0; a < na; ++a) {
+        // rho^c_a = sum_{alpha in a} conj(C_ka[alpha, cA]) C_kb[alpha, cB]
+        C rc = mk(R(0), R(0));
+        for (int al = lat.orb_begin[a]; al < lat.orb_begin[a + 1]; ++al) cfma_conj(rc, to_cx<R>(Ca[al]), to_cx<R>(Cb[al]));
+        C y = mk(R(0), R(0));
+        for (int b = 0; b < na; ++b) cfma(y, to_cx<R>(M[a + na * b]), rv[b]);
+        cfma(D, rc, y);
+        if (exchange) {
+            cfma(X, to_cx<R>(e1[(static_cast<size_t>(ka) * na + a) * ncv + ia]),
+                 to_cx<R>(e2[(static_cast<size_t>(kb) * na + a) * ncv + ib]));
+        }
+    }
+    return csub(D, X);
+}
+
+// Element (row, col) of H from the canonical (D - X): diagonal rule on the diagonal,
+// -(D - X) above it and its conjugate below.
+template <typename R>
+__device__ inline cx_t<R> generic_element(const LatticeDesc& lat, int nv, int nc, int64_t row, int64_t col,
+                                          const double2* __restrict__ evk, const double2* __restrict__ evkq,
+                                          const double* __restrict__ eval_k, const double* __restrict__ eval_kq,
+                                          const int* __restrict__ kcoord, const double2* __restrict__ ft,
+                                          const double2* __restrict__ e1, const double2* __restrict__ e2,
+                                          int exchange, double scissor) {
+    const int64_t i = row < col ? row : col, j = row < col ? col : row;
+    const cx_t<R> DX = generic_dx<R>(lat, nv, nc, i, j, evk, evkq, kcoord, ft, e1, e2, exchange);
+    if (i == j) {
+        const int ncv = nc * nv;
+        const int k = static_cast<int>(i / ncv), ia = static_cast<int>(i % ncv);
+        return mk(R(diag_value(eval_k, eval_kq, nc + nv, nv, k, ia / nv, ia % nv, scissor, static_cast<double>(DX.x))),
+                  R(0));
+    }
+    return row < col ? mk(-DX.x, -DX.y) : mk(-DX.x, DX.y);
+}
+
+// Whole matrix (or, with lower_rows > 0, the block-lower part read by the split product:
+// entries (row, col) with row >= lower_rows floor(col / lower_rows)), column-major.
+template <typename R>
+__global__ __launch_bounds__(256) void bse_generic_kernel(LatticeDesc lat, int nk, int nv, int nc,
+                                                          const double2* __restrict__ evk,
+                                                          const double2* __restrict__ evkq,
+                                                          const double* __restrict__ eval_k,
+                                                          const double* __restrict__ eval_kq,
+                                                          const int* __restrict__ kcoord,
+                                                          const double2* __restrict__ ft,
+                                                          const double2* __restrict__ e1,
+                                                          const double2* __restrict__ e2, int exchange,
+                                                          double scissor, int lower_rows, cx_t<R>* __restrict__ H) {
+    const int64_t N = static_cast<int64_t>(nk) * nc * nv;
+    const int64_t row = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (row >= N) return;
+    for (int64_t col = blockIdx.y; col < N; col += gridDim.y) {
+        if (lower_rows > 0 && row < col / lower_rows * lower_rows) continue;
+        H[row + col * N] = generic_element<R>(lat, nv, nc, row, col, evk, evkq, eval_k, eval_kq, kcoord, ft, e1, e2,
+                                              exchange, scissor);
+    }
+}
+
+// Diagonal of H (FP64), one row per thread.
+__global__ __launch_bounds__(256) void bse_generic_diag_kernel(LatticeDesc lat, int nk, int nv, int nc,
+                                                               const double2* __restrict__ evk,
+                                                               const double2* __restrict__ evkq,
+                                                               const double* __restrict__ eval_k,
+                                                               const double* __restrict__ eval_kq,
+                                                               const int* __restrict__ kcoord,
+                                                               const double2* __restrict__ ft,
+                                                               const double2* __restrict__ e1,
+                                                               const double2* __restrict__ e2, int exchange,
+                                                               double scissor, double* __restrict__ diag) {
+    const int64_t N = static_cast<int64_t>(nk) * nc * nv;
+    const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r >= N) return;
+    diag[r] = generic_element<double>(lat, nv, nc, r, r, evk, evkq, eval_k, eval_kq, kcoord, ft, e1, e2, exchange,
+                                      scissor)
+                  .x;
+}
+
+// Matrix-free product with recomputed tiles for generic shapes: a workgroup owns 64 rows
+// and, for each 64-column tile of its split, evaluates the tile element by element into
+// LDS (column-major, padded) and multiplies it with the staged vectors Xt[col][32] on the
+// 16x16x4 matrix cores; partial sums per split go to Yp[split][row][32].
+constexpr int kGenRows = 64;
+template <typename R>
+__global__ __launch_bounds__(256) void bse_matfree_generic_kernel(
+    LatticeDesc lat, int nk, int nv, int nc, const double2* __restrict__ evk, const double2* __restrict__ evkq,
+    const double* __restrict__ eval_k, const double* __restrict__ eval_kq, const int* __restrict__ kcoord,
+    const double2* __restrict__ ft, const double2* __restrict__ e1, const double2* __restrict__ e2, int exchange,
+    double scissor, const cx_t<R>* __restrict__ Xt, int nvec, int splits, int tiles_per_split,
+    cx_t<R>* __restrict__ Yp) {
+    using C = cx_t<R>;
+    using M = Mfma<R>;
+    using acc_t = typename M::acc_t;
+    constexpr int LDH = kGenRows + 1;
+    constexpr int RT = kGenRows / 16;
+    __shared__ C s_h[kGenRows * LDH];
+    const int64_t N = static_cast<int64_t>(nk) * nc * nv;
+    const int split = blockIdx.x % splits;
+    const int64_t rbase = static_cast<int64_t>(blockIdx.x / splits) * kGenRows;
+    const int nT = static_cast<int>((N + kGenRows - 1) / kGenRows);
+    const int t0 = split * tiles_per_split;
+    const int t1 = min(nT, t0 + tiles_per_split);
+    const int t = threadIdx.x;
+    const int lane = t & 63, wave = t >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const int ntiles = RT * ((nvec + 15) / 16);  // at most 8: two per wave
+    acc_t accR[2], accI[2];
+    accR[0] = accI[0] = accR[1] = accI[1] = acc_t{0, 0, 0, 0};
+    for (int T = t0; T < t1; ++T) {
+        const int64_t cbase = static_cast<int64_t>(T) * kGenRows;
+        __syncthreads();  // the previous tile has been consumed
+        for (int e = t; e < kGenRows * kGenRows; e += 256) {
+            const int rl = e % kGenRows, cl = e / kGenRows;
+            const int64_t row = rbase + rl, col = cbase + cl;
+            C h = mk(R(0), R(0));
+            if (row < N && col < N) {
+                h = generic_element<R>(lat, nv, nc, row, col, evk, evkq, eval_k, eval_kq, kcoord, ft, e1, e2, exchange,
+                                       scissor);
+            }
+            s_h[cl * LDH + rl] = h;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const int tile = wave + 4 * i;
+            if (tile < ntiles) {
+                const int rt = tile % RT, vt = tile / RT;
+                acc_t aR = accR[i], aI = accI[i];
+#pragma unroll 4
+                for (int kk = 0; kk < kGenRows / 4; ++kk) {
+                    const int cl = 4 * kk + lk;
+                    const C a = s_h[cl * LDH + 16 * rt + li];
+                    const int64_t col = cbase + cl;
+                    const C b = (col < N) ? Xt[col * kBlockVectors + 16 * vt + li] : mk(R(0), R(0));
+                    aR = M::mma(a.x, b.x, aR);
+                    aR = M::mma(a.y, -b.y, aR);
+                    aI = M::mma(a.x, b.y, aI);
+                    aI = M::mma(a.y, b.x, aI);
+                }
+                accR[i] = aR;
+                accI[i] = aI;
+            }
+        }
+    }
+    C* out = Yp + static_cast<int64_t>(split) * N * kBlockVectors;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const int tile = wave + 4 * i;
+        if (tile < ntiles) {
+            const int rt = tile % RT, vt = tile / RT;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int64_t row = rbase + 16 * rt + M::out_row(lk, v);
+                if (row < N) out[row * kBlockVectors + 16 * vt + li] = mk(accR[i][v], accI[i][v]);
+            }
+        }
+    }
+}
+
 // Generic per-element kernel for an explicit basis list {vcol, ccol, k} (FP64).
 __global__ void bse_basis_kernel(LatticeDesc lat, int nb, int nbasis, const int* __restrict__ basis,
                                  const double2* __restrict__ evk, const double2* __restrict__ evkq,
@@ -661,67 +848,80 @@ void launch_exchange_factors(const Context& c, hipStream_t s) {
     XATU_HIP(hipGetLastError());
 }
 
-// Full product basis in the structured ordering, expressed as an explicit list.
-static void full_basis(const Context& c, std::vector<int>& basis) {
-    basis.resize(static_cast<size_t>(c.N) * 3);
-    size_t r = 0;
-    for (int k = 0; k < c.nk; ++k)
-        for (int ic = 0; ic < c.nc; ++ic)
-            for (int iv = 0; iv < c.nv; ++iv) {
-                basis[3 * r + 0] = iv;
-                basis[3 * r + 1] = c.nv + ic;
-                basis[3 * r + 2] = k;
-                ++r;
-            }
-}
-
-void launch_build_basis(const Context& c, int nbasis, const int* d_basis, double2* H, hipStream_t s) {
+void launch_build_basis(const Context& c, int nbasis, const int* d_basis, double scissor, double2* H, hipStream_t s) {
     dim3 grid((nbasis + 255) / 256, nbasis);
     hipLaunchKernelGGL(bse_basis_kernel, grid, dim3(256), 0, s, c.lat, c.nb, nbasis, d_basis, c.d_evec_k.p,
                        c.d_evec_kq.p, c.d_eval_k.p, c.d_eval_kq.p, c.d_kcoord.p, c.d_ft.p, c.d_ftq.p,
-                       c.inter.exchange, c.scissor, H);
+                       c.inter.exchange, scissor, H);
     XATU_HIP(hipGetLastError());
+}
+
+namespace {
+
+template <typename R>
+void build_generic(const Context& c, cx_t<R>* H, int lower_rows, hipStream_t s) {
+    const unsigned gx = static_cast<unsigned>((c.N + 255) / 256);
+    const unsigned gy = static_cast<unsigned>(std::min<int64_t>(c.N, 65535));
+    hipLaunchKernelGGL((bse_generic_kernel<R>), dim3(gx, gy), dim3(256), 0, s, c.lat, c.nk, c.nv, c.nc, c.d_evec_k.p,
+                       c.d_evec_kq.p, c.d_eval_k.p, c.d_eval_kq.p, c.d_kcoord.p, c.d_ft.p, c.d_e1.p, c.d_e2.p,
+                       c.inter.exchange, c.scissor, lower_rows, H);
+    XATU_HIP(hipGetLastError());
+}
+
+template <typename R>
+void matfree_generic(Context& c, const double2* X, double2* Y, int nvec, hipStream_t s, DevBuf<cx_t<R>>& xt,
+                     DevBuf<cx_t<R>>& yp) {
+    const int nT = static_cast<int>((c.N + kGenRows - 1) / kGenRows);
+    // about 1024 workgroups or more; splits a multiple of the XCD count
+    int splits = std::max(1, (1024 + nT - 1) / nT);
+    splits = std::min((splits + kXcds - 1) / kXcds * kXcds, std::max(kXcds, nT));
+    const int tiles_per_split = (nT + splits - 1) / splits;
+    xt.alloc(std::max<size_t>(xt.n, static_cast<size_t>(c.N) * kBlockVectors));
+    yp.alloc(std::max<size_t>(yp.n, static_cast<size_t>(splits) * c.N * kBlockVectors));
+    for (int v0 = 0; v0 < nvec; v0 += kBlockVectors) {
+        const int nv = std::min(kBlockVectors, nvec - v0);
+        launch_stage_vectors(X + static_cast<int64_t>(v0) * c.N, c.N, nv, xt.p, s);
+        hipLaunchKernelGGL((bse_matfree_generic_kernel<R>), dim3(nT * splits), dim3(256), 0, s, c.lat, c.nk, c.nv,
+                           c.nc, c.d_evec_k.p, c.d_evec_kq.p, c.d_eval_k.p, c.d_eval_kq.p, c.d_kcoord.p, c.d_ft.p,
+                           c.d_e1.p, c.d_e2.p, c.inter.exchange, c.scissor, xt.p, nv, splits, tiles_per_split, yp.p);
+        XATU_HIP(hipGetLastError());
+        launch_reduce_partials(yp.p, c.N, nv, splits, Y + static_cast<int64_t>(v0) * c.N, s);
+    }
+}
+
+}  // namespace
+
+bool shape_specialised(const Context& c) {
+    return for_shape<double>(c, [](auto) {});
 }
 
 void launch_build_tiles64(const Context& c, double2* H, hipStream_t s) {
     if (for_shape<double>(c, [&](auto sh) { decltype(sh)::build(c, H, s); })) return;
-    // Shapes without a specialised tile kernel use the generic element kernel.
-    std::vector<int> basis;
-    full_basis(c, basis);
-    DevBuf<int> d_basis;
-    d_basis.alloc(basis.size());
-    XATU_HIP(hipMemcpyAsync(d_basis.p, basis.data(), basis.size() * sizeof(int), hipMemcpyHostToDevice, s));
-    if (c.N > (1 << 30)) throw Error(XATU_ERR_INVALID_ARGUMENT, "basis too large for the generic kernel");
-    launch_build_basis(c, static_cast<int>(c.N), d_basis.p, H, s);
-    XATU_HIP(hipStreamSynchronize(s));
+    build_generic<double>(c, H, 0, s);
 }
-
-static const char* kShapeMsg =
-    "this operation supports natoms <= 4 and nv == nc <= 3 (the FP64 build covers other shapes)";
 
 void launch_build_tiles32(const Context& c, float2* H, bool lower, hipStream_t s) {
     const int lower_rows = lower ? kSplitRowBlock : 0;
-    if (!for_shape<float>(c, [&](auto sh) { decltype(sh)::build(c, H, s, lower_rows); })) {
-        throw Error(XATU_ERR_INVALID_ARGUMENT, kShapeMsg);
-    }
+    if (for_shape<float>(c, [&](auto sh) { decltype(sh)::build(c, H, s, lower_rows); })) return;
+    build_generic<float>(c, H, lower_rows, s);
 }
 
 void launch_diag_direct(const Context& c, double* diag, hipStream_t s) {
-    if (!for_shape<double>(c, [&](auto sh) { decltype(sh)::diag(c, diag, s); })) {
-        throw Error(XATU_ERR_INVALID_ARGUMENT, kShapeMsg);
-    }
+    if (for_shape<double>(c, [&](auto sh) { decltype(sh)::diag(c, diag, s); })) return;
+    hipLaunchKernelGGL(bse_generic_diag_kernel, dim3(static_cast<unsigned>((c.N + 255) / 256)), dim3(256), 0, s, c.lat,
+                       c.nk, c.nv, c.nc, c.d_evec_k.p, c.d_evec_kq.p, c.d_eval_k.p, c.d_eval_kq.p, c.d_kcoord.p,
+                       c.d_ft.p, c.d_e1.p, c.d_e2.p, c.inter.exchange, c.scissor, diag);
+    XATU_HIP(hipGetLastError());
 }
 
 void launch_matfree64(Context& c, const double2* X, double2* Y, int nvec, hipStream_t s) {
-    if (!for_shape<double>(c, [&](auto sh) { decltype(sh)::matfree(c, X, Y, nvec, s, c.s_xt64, c.s_yp64); })) {
-        throw Error(XATU_ERR_INVALID_ARGUMENT, kShapeMsg);
-    }
+    if (for_shape<double>(c, [&](auto sh) { decltype(sh)::matfree(c, X, Y, nvec, s, c.s_xt64, c.s_yp64); })) return;
+    matfree_generic<double>(c, X, Y, nvec, s, c.s_xt64, c.s_yp64);
 }
 
 void launch_matfree32(Context& c, const double2* X, double2* Y, int nvec, hipStream_t s) {
-    if (!for_shape<float>(c, [&](auto sh) { decltype(sh)::matfree(c, X, Y, nvec, s, c.s_xt32, c.s_yp32); })) {
-        throw Error(XATU_ERR_INVALID_ARGUMENT, kShapeMsg);
-    }
+    if (for_shape<float>(c, [&](auto sh) { decltype(sh)::matfree(c, X, Y, nvec, s, c.s_xt32, c.s_yp32); })) return;
+    matfree_generic<float>(c, X, Y, nvec, s, c.s_xt32, c.s_yp32);
 }
 
 void launch_diag_extract(const Context& c, const double2* H, double* diag, hipStream_t s) {

@@ -286,7 +286,7 @@ int xatu_set_lattice(xatu_ctx* ctx, int ndim, const double* bravais, int natoms,
                      const int* norb_per_atom, int mesh_n) {
     return guarded(ctx, [&](Context& c) {
         if (ndim < 1 || ndim > 3) throw Error(XATU_ERR_INVALID_ARGUMENT, "ndim must be 1, 2 or 3");
-        if (natoms < 1 || natoms > kMaxAtoms) throw Error(XATU_ERR_INVALID_ARGUMENT, "natoms must be in [1, 8]");
+        if (natoms < 1 || natoms > kMaxAtoms) throw Error(XATU_ERR_INVALID_ARGUMENT, "natoms must be in [1, 16]");
         if (!bravais || !motif || !norb_per_atom) throw Error(XATU_ERR_INVALID_ARGUMENT, "null pointer");
         if (mesh_n < 1) throw Error(XATU_ERR_INVALID_ARGUMENT, "mesh_n must be positive");
         LatticeDesc L{};
@@ -313,6 +313,7 @@ int xatu_set_lattice(xatu_ctx* ctx, int ndim, const double* bravais, int natoms,
         c.a1norm = std::sqrt(bravais[0] * bravais[0] + bravais[1] * bravais[1] + bravais[2] * bravais[2]);
         c.have_lattice = true;
         c.have_bands = c.have_ft = c.built64 = c.built32 = c.prepared = false;
+        c.drop_solution();
     });
 }
 
@@ -356,6 +357,7 @@ int xatu_set_bands(xatu_ctx* ctx, int nk, int nv, int nc, const double* eigval_k
         XATU_HIP(hipStreamSynchronize(c.stream));
         c.have_bands = true;
         c.built64 = c.built32 = c.prepared = false;
+        c.drop_solution();
     });
 }
 
@@ -369,6 +371,9 @@ int xatu_fourier_sums(xatu_ctx* ctx, const xatu_interaction* inter) {
         if (inter->mode != XATU_MODE_REALSPACE && inter->mode != XATU_MODE_RECIPROCAL) {
             throw Error(XATU_ERR_INVALID_ARGUMENT, "setMode(): mode must be either realspace or reciprocalspace");
         }
+        c.drop_solution();
+        c.have_ft = false;
+        c.built64 = c.built32 = c.prepared = false;
         c.inter = *inter;
         c.pot_d = make_pot(c, inter->potential);
         c.pot_x = inter->exchange ? make_pot(c, inter->exchange_potential) : c.pot_d;
@@ -429,6 +434,7 @@ int xatu_build_bse(xatu_ctx* ctx, int precision, double scissor, void* hbs_out) 
             c.built64 = c.built32 = false;
         }
         c.scissor = scissor;
+        c.drop_solution();
         const size_t nn = static_cast<size_t>(c.N) * static_cast<size_t>(c.N);
         EventTimer ev(c.stream), evk(c.stream);
         ev.start();
@@ -460,7 +466,8 @@ int xatu_build_bse(xatu_ctx* ctx, int precision, double scissor, void* hbs_out) 
             const size_t bytes = nn * (precision == XATU_FP64 ? sizeof(double2) : sizeof(float2));
             const void* src = precision == XATU_FP64 ? static_cast<const void*>(c.d_H64.p)
                                                      : static_cast<const void*>(c.d_H32.p);
-            XATU_HIP(hipMemcpy(hbs_out, src, bytes, hipMemcpyDeviceToHost));
+            XATU_HIP(hipMemcpyAsync(hbs_out, src, bytes, hipMemcpyDeviceToHost, c.stream));
+            XATU_HIP(hipStreamSynchronize(c.stream));
         }
     });
 }
@@ -477,14 +484,13 @@ int xatu_build_bse_basis(xatu_ctx* ctx, int nbasis, const int* basis, double sci
                 throw Error(XATU_ERR_INVALID_ARGUMENT, "basis entry out of range");
             }
         }
-        c.scissor = scissor;
         DevBuf<int> d_basis;
         d_basis.alloc(static_cast<size_t>(nbasis) * 3);
         XATU_HIP(hipMemcpyAsync(d_basis.p, basis, d_basis.bytes(), hipMemcpyHostToDevice, c.stream));
         DevBuf<double2> H;
         H.alloc(static_cast<size_t>(nbasis) * nbasis);
         // exchange factors are computed on the fly by the element kernel
-        launch_build_basis(c, nbasis, d_basis.p, H.p, c.stream);
+        launch_build_basis(c, nbasis, d_basis.p, scissor, H.p, c.stream);
         XATU_HIP(hipMemcpyAsync(hbs_out, H.p, H.bytes(), hipMemcpyDeviceToHost, c.stream));
         XATU_HIP(hipStreamSynchronize(c.stream));
     });
@@ -496,11 +502,13 @@ int xatu_get_bse(xatu_ctx* ctx, int precision, void* hbs_out) {
         const size_t nn = static_cast<size_t>(c.N) * static_cast<size_t>(c.N);
         if (precision == XATU_FP64) {
             if (!c.built64) throw Error(XATU_ERR_LOGIC, "FP64 BSE matrix is not resident");
-            XATU_HIP(hipMemcpy(hbs_out, c.d_H64.p, nn * sizeof(double2), hipMemcpyDeviceToHost));
+            XATU_HIP(hipMemcpyAsync(hbs_out, c.d_H64.p, nn * sizeof(double2), hipMemcpyDeviceToHost, c.stream));
+            XATU_HIP(hipStreamSynchronize(c.stream));
         } else {
             if (!c.built32) throw Error(XATU_ERR_LOGIC, "FP32 BSE matrix is not resident");
-            complete_matrix32(c);
-            XATU_HIP(hipMemcpy(hbs_out, c.d_H32.p, nn * sizeof(float2), hipMemcpyDeviceToHost));
+            complete_matrix32(c);  // launched on c.stream: the copy must follow it there
+            XATU_HIP(hipMemcpyAsync(hbs_out, c.d_H32.p, nn * sizeof(float2), hipMemcpyDeviceToHost, c.stream));
+            XATU_HIP(hipStreamSynchronize(c.stream));
         }
     });
 }
@@ -511,9 +519,11 @@ int xatu_free_bse(xatu_ctx* ctx, int precision) {
         if (precision == XATU_FP64) {
             c.d_H64.release();
             c.built64 = false;
-        } else {
+        } else if (precision == XATU_FP32) {
             c.d_H32.release();
             c.built32 = false;
+        } else {
+            throw Error(XATU_ERR_INVALID_ARGUMENT, "bad precision");
         }
     });
 }
@@ -586,8 +596,7 @@ int xatu_hf_gradients(xatu_ctx* ctx, int nstates, const double* eigvec, int npar
             XATU_HIP(hipMemcpyAsync(V.p, eigvec, V.bytes(), hipMemcpyHostToDevice, c.stream));
             dv = V.p;
         } else {
-            if (c.last_nstates < nstates) throw Error(XATU_ERR_LOGIC, "no eigenvectors from a previous solve");
-            dv = c.d_last_vec.p;
+            dv = c.last_vectors(nstates);
         }
         hf_gradients(c, nstates, dv, nparams, params, grad_out);
     });
@@ -604,8 +613,7 @@ int xatu_band_gradients(xatu_ctx* ctx, int nstates, const double* eigvec, double
             XATU_HIP(hipMemcpyAsync(V.p, eigvec, V.bytes(), hipMemcpyHostToDevice, c.stream));
             dv = V.p;
         } else {
-            if (c.last_nstates < nstates) throw Error(XATU_ERR_LOGIC, "no eigenvectors from a previous solve");
-            dv = c.d_last_vec.p;
+            dv = c.last_vectors(nstates);
         }
         band_gradients(c, nstates, dv, w_out, g_out);
     });
@@ -623,8 +631,7 @@ int xatu_oscillator_strengths(xatu_ctx* ctx, int nstates, const double* eigvec, 
             XATU_HIP(hipMemcpyAsync(V.p, eigvec, V.bytes(), hipMemcpyHostToDevice, c.stream));
             dv = V.p;
         } else {
-            if (c.last_nstates < nstates) throw Error(XATU_ERR_LOGIC, "no eigenvectors from a previous solve");
-            dv = c.d_last_vec.p;
+            dv = c.last_vectors(nstates);
         }
         vel.alloc(static_cast<size_t>(c.N) * 3);
         o.alloc(static_cast<size_t>(3) * nstates);
@@ -640,8 +647,12 @@ int xatu_get_fourier(xatu_ctx* ctx, double* ft_out, double* ftq_out) {
         c.require_ft();
         if (c.lat.recip) throw Error(XATU_ERR_LOGIC, "the reciprocal-space mode has no lattice Fourier sums");
         const size_t na2 = static_cast<size_t>(c.lat.natoms) * c.lat.natoms;
-        if (ft_out) XATU_HIP(hipMemcpy(ft_out, c.d_ft.p, na2 * c.lat.nmesh * sizeof(double2), hipMemcpyDeviceToHost));
-        if (ftq_out) XATU_HIP(hipMemcpy(ftq_out, c.d_ftq.p, na2 * sizeof(double2), hipMemcpyDeviceToHost));
+        if (ft_out) {
+            XATU_HIP(hipMemcpyAsync(ft_out, c.d_ft.p, na2 * c.lat.nmesh * sizeof(double2), hipMemcpyDeviceToHost,
+                                    c.stream));
+        }
+        if (ftq_out) XATU_HIP(hipMemcpyAsync(ftq_out, c.d_ftq.p, na2 * sizeof(double2), hipMemcpyDeviceToHost, c.stream));
+        XATU_HIP(hipStreamSynchronize(c.stream));
     });
 }
 

@@ -94,8 +94,8 @@ struct LatticeDesc {
     int mesh_n;   // points per axis of the full mesh
     int nmesh;    // mesh_n^ndim
     double bravais[9];
-    double tau[8 * 3];  // atomic positions (natoms <= 8)
-    int orb_begin[9];   // orbital offset of each atom, orb_begin[natoms] = norb
+    double tau[16 * 3];  // atomic positions (natoms <= kMaxAtoms)
+    int orb_begin[17];   // orbital offset of each atom, orb_begin[natoms] = norb
     // interaction table: real-space mode indexes it by the mesh point of k_b - k_a
     // (tab_n = mesh_n); reciprocal mode by the raw offset difference k_a - k_b + (mesh_n - 1)
     // per axis (tab_n = 2 mesh_n - 1), because its sums over G are not periodic in q
@@ -105,7 +105,9 @@ struct LatticeDesc {
     double recip_vec[9];  // reciprocal lattice vectors (rows), 1/Angstrom
 };
 
-constexpr int kMaxAtoms = 8;
+constexpr int kMaxAtoms = 16;
+static_assert(sizeof(LatticeDesc::tau) == 3 * kMaxAtoms * sizeof(double), "motif storage");
+static_assert(sizeof(LatticeDesc::orb_begin) == (kMaxAtoms + 1) * sizeof(int), "orbital offsets");
 
 // Potential parameters for the device evaluation of V(r).
 struct PotDesc {
@@ -189,8 +191,23 @@ struct Context {
 
     // last solve
     int last_nstates = 0;
+    int64_t last_n = 0;          // dimension of the operator the last solve ran on
     DevBuf<double2> d_last_vec;  // N x last_nstates
     std::vector<double> last_val;
+    // forget the last solve's eigenpairs: they belong to an operator that has changed
+    void drop_solution() {
+        last_nstates = 0;
+        last_n = 0;
+        d_last_vec.release();
+        last_val.clear();
+    }
+    // eigenvectors of the last solve for nstates states of the current operator
+    const double2* last_vectors(int nstates) const {
+        if (last_nstates < nstates || last_n != N || !d_last_vec.p) {
+            throw Error(XATU_ERR_LOGIC, "no eigenvectors of the current BSE Hamiltonian from a previous solve");
+        }
+        return d_last_vec.p;
+    }
 
     // timings
     xatu_timings tim{};
@@ -248,9 +265,12 @@ void complete_matrix32(Context& c);
 void clear_upper_heads(Context& c);
 // The split product's layout is usable for the current operator (even N, finite scale).
 bool split_layout_available(const Context& c);
-void launch_build_basis(const Context& c, int nbasis, const int* d_basis, double2* H, hipStream_t s);
+void launch_build_basis(const Context& c, int nbasis, const int* d_basis, double scissor, double2* H, hipStream_t s);
 void launch_diag_extract(const Context& c, const double2* H, double* diag, hipStream_t s);
 void launch_diag_direct(const Context& c, double* diag, hipStream_t s);
+// The (natoms, nc, nv) shape has specialised tile kernels (natoms <= 4, nc == nv <= 3);
+// every other shape runs on the element-wise kernels.
+bool shape_specialised(const Context& c);
 void launch_compare(const Context& c, int kblock, double* d_blockerr, hipStream_t s);
 
 // Dense products Y = H X (N x nvec, complex FP64 vectors) with the resident FP64 or

@@ -340,6 +340,7 @@ void solve_dense_diag(Context& c, int nstates, double* eigval_out, double* eigve
         XATU_HIP(hipMemcpy(eigvec_out, A.p, static_cast<size_t>(n) * ns * sizeof(double2), hipMemcpyDeviceToHost));
     }
     c.last_nstates = ns;
+    c.last_n = c.N;
     c.d_last_vec.alloc(static_cast<size_t>(n) * ns);
     XATU_HIP(hipMemcpyAsync(c.d_last_vec.p, A.p, static_cast<size_t>(n) * ns * sizeof(double2),
                             hipMemcpyDeviceToDevice, c.stream));
@@ -405,6 +406,7 @@ void solve_davidson(Context& c, const xatu_solve_options& opt, double* eigval_ou
     const std::vector<double>& th = final_d.theta();
     std::copy(th.begin(), th.begin() + nev, eigval_out);
     c.last_nstates = nev;
+    c.last_n = N;
     c.last_val.assign(th.begin(), th.begin() + nev);
     c.d_last_vec.alloc(static_cast<size_t>(N) * nev);
     XATU_HIP(hipMemcpyAsync(c.d_last_vec.p, final_d.ritz_vectors(), static_cast<size_t>(N) * nev * sizeof(double2),

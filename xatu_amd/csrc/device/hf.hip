@@ -37,7 +37,7 @@ constexpr int kHfStates = 4;  // states per pass of the pair-density adjoint
 
 // dV/dtheta for pair p and cell R. which: 0 r0, 1 eps (m or s: identical).
 __global__ void dpotential_table_kernel(LatticeDesc lat, PotDesc pot, const int* __restrict__ cells, int nR, int npairs,
-                                        int which, double* __restrict__ out) {
+                                        int which, const double* __restrict__ gl, double* __restrict__ out) {
     const int R = blockIdx.x * blockDim.x + threadIdx.x;
     const int p = blockIdx.y;
     if (R >= nR || p >= npairs) return;
@@ -60,7 +60,7 @@ __global__ void dpotential_table_kernel(LatticeDesc lat, PotDesc pot, const int*
     const double r = sqrt(dx * dx + dy * dy + dz * dz);
     double d = 0.0;
     if (pot.kind == XATU_POTENTIAL_KELDYSH) {
-        d = (which == 0) ? keldysh_dr0(r, pot) : -potential(r, pot) / (2.0 * pot.eps_bar);
+        d = (which == 0) ? keldysh_dr0(r, pot, gl) : -potential(r, pot) / (2.0 * pot.eps_bar);
     }
     out[static_cast<size_t>(p) * nR + R] = d;
 }
@@ -322,16 +322,25 @@ void hf_gradients(Context& c, int nstates, const double2* d_vec, int nparams, co
     DevBuf<double> dv;
     dv.alloc(static_cast<size_t>(npairs) * nR);
     std::vector<double> h_dv(dv.n), h_dvx(dv.n);
+    // quadrature rule of the dV/dr0 evaluation
+    static const std::vector<double> h_gl = [] {
+        std::vector<double> r(2 * kGlNodes);
+        gauss_legendre_rule(r.data());
+        return r;
+    }();
+    DevBuf<double> gl;
+    gl.alloc(h_gl.size());
+    XATU_HIP(hipMemcpyAsync(gl.p, h_gl.data(), gl.bytes(), hipMemcpyHostToDevice, s));
     for (int ip = 0; ip < nparams; ++ip) {
         const int which = (params[ip] == XATU_PARAM_R0) ? 0 : 1;
         hipLaunchKernelGGL(dpotential_table_kernel, dim3((nR + 255) / 256, npairs), dim3(256), 0, s, c.lat, c.pot_d,
-                           c.d_cells.p, nR, npairs, which, dv.p);
+                           c.d_cells.p, nR, npairs, which, gl.p, dv.p);
         XATU_HIP(hipGetLastError());
         XATU_HIP(hipMemcpyAsync(h_dv.data(), dv.p, dv.bytes(), hipMemcpyDeviceToHost, s));
         if (c.inter.exchange) {
             XATU_HIP(hipStreamSynchronize(s));
             hipLaunchKernelGGL(dpotential_table_kernel, dim3((nR + 255) / 256, npairs), dim3(256), 0, s, c.lat,
-                               c.pot_x, c.d_cells.p, nR, npairs, which, dv.p);
+                               c.pot_x, c.d_cells.p, nR, npairs, which, gl.p, dv.p);
             XATU_HIP(hipGetLastError());
             XATU_HIP(hipMemcpyAsync(h_dvx.data(), dv.p, dv.bytes(), hipMemcpyDeviceToHost, s));
         }

@@ -369,32 +369,49 @@ __global__ __launch_bounds__(kRedOut* kRedSlices) void gram_reduce_kernel(const 
     }
 }
 
-template <typename Rin, typename R, typename Rout>
-void dense_product(const Context& c, const cx_t<R>* H, const cx_t<Rin>* X, cx_t<Rout>* Y, int nvec, hipStream_t s,
-                   DevBuf<cx_t<R>>& xt, DevBuf<cx_t<R>>& yp) {
+// Column splits of the dense product: enough workgroups to fill 256 CUs a few times over,
+// a multiple of the XCD count, at most max_splits (>= 1).
+template <typename R>
+int product_splits(int64_t N, int max_splits) {
     using T = ProductTiling<R>;
-    const int64_t N = c.N;
     const int rows_wg = 64 * T::TW;
     const int row_blocks = static_cast<int>((N + rows_wg - 1) / rows_wg);
-    // enough workgroups to fill 256 CUs a few times over; splits a multiple of the XCD count
     int splits = std::max(1, std::min(64, (2048 + row_blocks - 1) / row_blocks));
     splits = (splits + kXcds - 1) / kXcds * kXcds;
+    return std::max(1, std::min(splits, max_splits));
+}
+
+// Y = H X through the staging buffer xt (N x 32) and the partial sums yp (splits x N x 32).
+template <typename Rin, typename R, typename Rout>
+void dense_product_into(int64_t N, const cx_t<R>* H, const cx_t<Rin>* X, cx_t<Rout>* Y, int nvec, hipStream_t s,
+                        cx_t<R>* xt, cx_t<R>* yp, int splits) {
+    using T = ProductTiling<R>;
+    const int rows_wg = 64 * T::TW;
+    const int row_blocks = static_cast<int>((N + rows_wg - 1) / rows_wg);
     const int step = 4 * T::U;
-    int64_t kchunk = ((N + splits - 1) / splits + step - 1) / step * step;
+    const int64_t kchunk = ((N + splits - 1) / splits + step - 1) / step * step;
     // every split owns a (possibly empty) column range and writes its partial rows
-    xt.alloc(std::max<size_t>(xt.n, static_cast<size_t>(N) * kVec));
-    yp.alloc(std::max<size_t>(yp.n, static_cast<size_t>(splits) * N * kVec));
     for (int v0 = 0; v0 < nvec; v0 += kVec) {
         const int nv = std::min(kVec, nvec - v0);
         const int64_t tot = N * kVec;
         hipLaunchKernelGGL((stage_vectors_kernel<Rin, R>), dim3((tot + 255) / 256), dim3(256), 0, s, X + v0 * N, N, nv,
-                           xt.p);
-        hipLaunchKernelGGL((dense_product_kernel<R>), dim3(row_blocks * splits), dim3(256), 0, s, H, xt.p, N, kchunk,
-                           splits, yp.p);
-        hipLaunchKernelGGL((reduce_product_kernel<R, Rout>), dim3((N + 63) / 64), dim3(256), 0, s, yp.p, N, nv,
-                           splits, Y + v0 * N);
+                           xt);
+        hipLaunchKernelGGL((dense_product_kernel<R>), dim3(row_blocks * splits), dim3(256), 0, s, H, xt, N, kchunk,
+                           splits, yp);
+        hipLaunchKernelGGL((reduce_product_kernel<R, Rout>), dim3((N + 63) / 64), dim3(256), 0, s, yp, N, nv, splits,
+                           Y + v0 * N);
     }
     XATU_HIP(hipGetLastError());
+}
+
+template <typename Rin, typename R, typename Rout>
+void dense_product(const Context& c, const cx_t<R>* H, const cx_t<Rin>* X, cx_t<Rout>* Y, int nvec, hipStream_t s,
+                   DevBuf<cx_t<R>>& xt, DevBuf<cx_t<R>>& yp) {
+    const int64_t N = c.N;
+    const int splits = product_splits<R>(N, 64);
+    xt.alloc(std::max<size_t>(xt.n, static_cast<size_t>(N) * kVec));
+    yp.alloc(std::max<size_t>(yp.n, static_cast<size_t>(splits) * N * kVec));
+    dense_product_into<Rin, R, Rout>(N, H, X, Y, nvec, s, xt.p, yp.p, splits);
 }
 
 }  // namespace

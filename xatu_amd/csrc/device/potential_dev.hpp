@@ -78,9 +78,9 @@ __device__ inline double potential(double r, const PotDesc& p) {
 }
 
 // Struve H1(x) by its power series, H1 = (2 x^2 / (3 pi)) sum_k u_k with
-// u_k = -u_{k-1} x^2 / ((2k+1)(2k+3)); valid (to ~1e-13) for x <= 20.
-// For x > 20 the asymptotic expansion of H1 - Y1 is used by the caller.
-__device__ inline double struve_h1_series(double x) {
+// u_k = -u_{k-1} x^2 / ((2k+1)(2k+3)); the terms grow to ~e^x before they cancel, so it
+// is used for x <= 5 only (relative error below 1e-14 there).
+__host__ __device__ inline double struve_h1_series(double x) {
     double s = 1.0, u = 1.0;
     for (int k = 1; k < 80; ++k) {
         u = -u * x * x / ((2.0 * k + 1.0) * (2.0 * k + 3.0));
@@ -90,10 +90,98 @@ __device__ inline double struve_h1_series(double x) {
     return 2.0 * x * x / (3.0 * kPi) * s;
 }
 
+// Gauss-Legendre rule of kGlNodes points on [0, kGlLength]: nodes then weights.
+constexpr int kGlNodes = 48;
+constexpr double kGlLength = 40.0;
+
+// Nodes and weights of the Gauss-Legendre rule above, by Newton's iteration on the
+// Legendre polynomial (three-term recurrence) from the usual cosine estimates of its roots.
+inline void gauss_legendre_rule(double* out /* 2 * kGlNodes */) {
+    const int n = kGlNodes;
+    for (int i = 0; i < (n + 1) / 2; ++i) {
+        double z = cos(kPi * (i + 0.75) / (n + 0.5)), pp = 0.0;
+        for (int it = 0; it < 100; ++it) {
+            double p1 = 1.0, p2 = 0.0;
+            for (int j = 1; j <= n; ++j) {
+                const double p3 = p2;
+                p2 = p1;
+                p1 = ((2.0 * j - 1.0) * z * p2 - (j - 1.0) * p3) / j;
+            }
+            pp = n * (z * p1 - p2) / (z * z - 1.0);
+            const double dz = p1 / pp;
+            z -= dz;
+            if (fabs(dz) < 1e-16) break;
+        }
+        const double w = 2.0 / ((1.0 - z * z) * pp * pp);
+        // map [-1, 1] to [0, L]
+        out[i] = 0.5 * kGlLength * (1.0 - z);
+        out[n - 1 - i] = 0.5 * kGlLength * (1.0 + z);
+        out[n + i] = out[2 * n - 1 - i] = 0.5 * kGlLength * w;
+    }
+}
+
+// g(x) = d(x f)/dx = f + x f' for f = STVH0(x) - Y0(x), the function V evaluates.
+//  x <= 5:      f + x (2/pi - H1 + Y1) with the H1 power series;
+//  5 < x <= 20: f and x f' cancel to O(x^-3) while each is O(1/x), so g comes from the
+//               integral form f(x) = (2/pi) int_0^inf e^{-xt} / sqrt(1 + t^2) dt, which gives
+//               g(x) = (2/pi) int_0^inf e^{-u} (u^2 / x^3) (1 + u^2/x^2)^(-3/2) du, a positive
+//               integrand, on the Gauss-Legendre rule (gl: nodes, then weights) over [0, 40]
+//               (the tail beyond is below e^-40 relative);
+//  x > 20:      the exact derivative of the asymptotic branch of struve_h0 minus Y0: with
+//               H0 = 2/(pi x) S + BY0 the S/x terms cancel analytically, leaving
+//               g = (2/pi) S'(x) + (BY0 - Y0) + x (BY0' + Y1) over the same series terms, so
+//               g is the derivative of the potential actually evaluated there.
+// Relative error against 40-digit values for x <= 20: below 4e-13 (series), 1e-14 (integral).
+__host__ __device__ inline double keldysh_dxf(double x, const double* gl) {
+    if (x <= 5.0) {
+        return struve_h0(x) - y0(x) + x * (2.0 / kPi - struve_h1_series(x) + y1(x));
+    }
+    if (x <= 20.0) {
+        const double ix = 1.0 / x;
+        double s = 0.0;
+        for (int i = 0; i < kGlNodes; ++i) {
+            const double u = gl[i];
+            const double q = 1.0 + (u * ix) * (u * ix);
+            s += gl[kGlNodes + i] * exp(-u) * (u * u) / (q * sqrt(q));
+        }
+        return 2.0 / kPi * s * ix * ix * ix;
+    }
+    int km = static_cast<int>(0.5 * (x + 1.0));
+    if (x >= 50.0) km = 25;
+    double S = 1.0, r = 1.0, dS = 0.0;
+    for (int k = 1; k <= km; ++k) {
+        const double tk = (2.0 * k - 1.0) / x;
+        r = -r * (tk * tk);  // (-1)^k ((2k-1)!!)^2 x^-2k
+        S += r;
+        dS += -2.0 * k * r / x;
+        if (fabs(r) < fabs(S) * 1.0e-12) break;
+    }
+    // BY0 = 2 x^-1/2 (P(t^2) sin th + t Qp(t^2) cos th), t = 4/x, th = x - pi/4
+    const double t = 4.0 / x, t2 = t * t, dt = -t * t / 4.0;
+    const double P =
+        ((((-.37043e-5 * t2 + .173565e-4) * t2 - .487613e-4) * t2 + .17343e-3) * t2 - 0.1753062e-2) * t2 +
+        .3989422793;
+    const double dP = (((5 * -.37043e-5 * t2 + 4 * .173565e-4) * t2 - 3 * .487613e-4) * t2 + 2 * .17343e-3) * t2 -
+                      0.1753062e-2;  // dP / d(t^2)
+    const double Qp = ((((.32312e-5 * t2 - 0.142078e-4) * t2 + 0.342468e-4) * t2 - 0.869791e-4) * t2 +
+                       0.4564324e-3) * t2 -
+                      0.0124669441;
+    const double dQp = (((5 * .32312e-5 * t2 - 4 * 0.142078e-4) * t2 + 3 * 0.342468e-4) * t2 - 2 * 0.869791e-4) * t2 +
+                       0.4564324e-3;  // dQp / d(t^2)
+    const double Q = t * Qp;
+    const double dPx = dP * 2.0 * t * dt;
+    const double dQx = (Qp + t * dQp * 2.0 * t) * dt;
+    const double th = x - 0.25 * kPi, sn = sin(th), cs = cos(th);
+    const double rx = 1.0 / sqrt(x);
+    const double by0 = 2.0 * rx * (P * sn + Q * cs);
+    const double dby0 = -rx / x * (P * sn + Q * cs) + 2.0 * rx * (dPx * sn + P * cs + dQx * cs - Q * sn);
+    return 2.0 / kPi * dS + (by0 - y0(x)) + x * (dby0 + y1(x));
+}
+
 // dV/dr0 for the Keldysh potential at distance r (same special cases as V):
 // with f = H0 - Y0 and x = r/r0, V = C f(x) / (eps_bar r0), so
-// dV/dr0 = -C / (eps_bar r0^2) * (f(x) + x f'(x)),  f'(x) = 2/pi - H1(x) + Y1(x).
-__device__ inline double keldysh_dr0(double r, const PotDesc& p) {
+// dV/dr0 = -C / (eps_bar r0^2) * (f(x) + x f'(x)) = -C / (eps_bar r0^2) * g(x).
+__device__ inline double keldysh_dr0(double r, const PotDesc& p, const double* gl) {
     double x;
     if (r == 0.0) {
         x = p.reg / p.r0;
@@ -101,16 +189,7 @@ __device__ inline double keldysh_dr0(double r, const PotDesc& p) {
         if (r > p.cutoff) return 0.0;
         x = fabs(r) / p.r0;
     }
-    const double f = struve_h0(x) - y0(x);
-    double fp;
-    if (x <= 20.0) {
-        fp = 2.0 / kPi - struve_h1_series(x) + y1(x);
-    } else {
-        // H1 - Y1 ~ (2/pi) (1 + 1/x^2 - 3/x^4 + 45/x^6)
-        const double ix2 = 1.0 / (x * x);
-        fp = 2.0 / kPi - (2.0 / kPi) * (1.0 + ix2 * (1.0 + ix2 * (-3.0 + ix2 * 45.0)));
-    }
-    return -keldysh_pref(p) / p.r0 * (f + x * fp);
+    return -keldysh_pref(p) / p.r0 * keldysh_dxf(x, gl);
 }
 
 }  // namespace xatu_dev

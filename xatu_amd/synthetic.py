@@ -11,7 +11,9 @@ every orbital pair to all cells R = m1 a1 + m2 a2 with |m1|, |m2| <= 3:
 t_ij(R) = (N(0, s) + i N(0, s)) / sqrt(2) * exp(-|R|/a) with s = 0.5 eV, and
 H(-R) = H(R)^H exactly (the R = 0 block is made Hermitian the same way).
 The larger model (36 orbitals) uses four atoms, nine orbitals each, the same
-distributions, and the additional atom at fractional (1/2, 0).
+distributions, and the additional atom at fractional (1/2, 0); half of its orbitals are
+valence-like as well (five on the first two atoms, four on the others: 18 of 36, the
+filling).
 """
 from __future__ import annotations
 
@@ -35,9 +37,12 @@ def model_arrays(natoms=3, orb_per_atom=6, seed=20240328, a=TRIANGULAR_A, reach=
     motif = np.array([[*(f1 * a1 + f2 * a2)[:3], 0.0] for f1, f2 in fractional])
     motif[:, 3] = 0.0
     norb = natoms * orb_per_atom
-    nval = orb_per_atom // 2
+    # half of all orbitals are valence-like: orb_per_atom // 2 on every atom, and with an
+    # odd orb_per_atom one more on the first atoms until the total is norb // 2
+    extra = norb // 2 - natoms * (orb_per_atom // 2)
     onsite = []
-    for _ in range(natoms):
+    for at in range(natoms):
+        nval = orb_per_atom // 2 + (1 if at < extra else 0)
         onsite += list(rng.uniform(-4.0, -2.0, nval)) + list(rng.uniform(2.0, 4.0, orb_per_atom - nval))
     onsite = np.array(onsite)
 
