@@ -166,10 +166,15 @@ int xatu_fourier_sums(xatu_ctx* ctx, const xatu_interaction* inter);
  * exchange factors and the diagonal so that solves and products can run matrix-free. */
 int xatu_build_bse(xatu_ctx* ctx, int precision, double scissor, void* hbs_out);
 
-/* Same, for an explicit subset basis (ExcitonTB::BShamiltonian(const arma::imat&)):
- * basis is nbasis x 3 row-major {valence column, conduction column, k index}, columns
- * indexing the band stacks (bandToIndex). FP64 only; the result is copied to hbs_out
- * (nbasis x nbasis) and not kept resident. */
+/* Same, for an explicit subset basis (ExcitonTB::BShamiltonian(const arma::imat&),
+ * src/ExcitonTB.cpp:840-921): basis is nbasis x 3 row-major {valence column, conduction
+ * column, k index}, columns indexing the band stacks (bandToIndex). FP64 only. The
+ * nbasis x nbasis matrix stays resident as the operator of the following calls (its
+ * dimension replaces N): xatu_solve (DIAG, or DAVIDSON with FP64 products on the resident
+ * matrix), xatu_get_bse(FP64), xatu_bse_matvec(FP64); the calls that need the full basis
+ * (FP32 and matrix-free products, gradients, oscillator strengths, build comparison)
+ * return XATU_ERR_LOGIC until the next xatu_build_bse, xatu_set_bands or
+ * xatu_fourier_sums restores it. hbs_out (nullable) receives a copy. */
 int xatu_build_bse_basis(xatu_ctx* ctx, int nbasis, const int* basis, double scissor, double* hbs_out);
 
 /* Copy a resident matrix to the host (same layouts as xatu_build_bse). */

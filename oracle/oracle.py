@@ -264,8 +264,19 @@ class System:
 
 # --------------------------------------------------------------------------- exciton
 
+def atomic_gauge(system, stack):
+    """SystemTB::latticeToAtomicGauge (src/SystemTB.cpp:166-182) on every state of a band
+    stack (norb x nb x nk): coefficient alpha at k-point k times exp(-i k . tau_atom(alpha)).
+    BShamiltonian applies it with the k-point of the row to the k and k+Q stacks alike
+    (src/ExcitonTB.cpp:877-884)."""
+    tau = np.repeat(system.motif[:, :3], system.orb_per_atom, axis=0)   # norb x 3
+    ph = np.exp(-1j * (system.kpoints @ tau.T))                           # nk x norb
+    return np.asfortranarray(stack * ph.T[:, None, :])
+
+
 class Exciton:
-    """The reference ExcitonTB pipeline on the CPU (realspace mode, lattice gauge)."""
+    """The reference ExcitonTB pipeline on the CPU (realspace mode; lattice gauge, or the
+    atomic gauge with gauge = "atomic")."""
 
     def __init__(self, system, ncell, nbands=1, nrmbands=0, parameters=(1, 5, 1), Q=(0, 0, 0), bands=None):
         self.sys = system if isinstance(system, System) else System(system)
@@ -280,6 +291,7 @@ class Exciton:
         self.potential = "keldysh"
         self.exchange_potential = "keldysh"
         self.regularization = 0.0
+        self.gauge = "lattice"
         f = self.sys.fermi
         if bands is None:
             self.valence = list(range(f - nbands - nrmbands + 1, f - nrmbands + 1))
@@ -353,6 +365,8 @@ class Exciton:
         ekq = self.eigval_kq if eigval_kq is None else eigval_kq
         vk = self.eigvec_k if eigvec_k is None else eigvec_k
         vkq = self.eigvec_kq if eigvec_kq is None else eigvec_kq
+        if self.gauge == "atomic":
+            vk, vkq = atomic_gauge(s, vk), atomic_gauge(s, vkq)
         sc = self.scissor if scissor is None else scissor
         ex = self.exchange if exchange is None else exchange
         orb_atom = np.repeat(np.arange(s.natoms), s.orb_per_atom).astype(np.int32)

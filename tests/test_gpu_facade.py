@@ -147,3 +147,59 @@ def test_result_observables(tmp_path):
     res.writeEigenvalues(str(f), 3)
     toks = f.read_text().split()
     assert int(toks[0]) == 400 and float(toks[1]) == pytest.approx(5.335690, abs=1e-6)
+
+
+@pytest.mark.parametrize("method", ["diag", "davidson"])
+def test_atomic_gauge(method):
+    # ExcitonTB::setGauge("atomic") (src/ExcitonTB.cpp:257-262, 877-884): the BSE built from
+    # the atomic-gauge coefficients, against the oracle in the same gauge; the exposed band
+    # stacks stay in the lattice gauge
+    ex = X.ExcitonTB(X.SystemConfiguration(MOS2), 6, 2, 0, [1., 4., 13.55], [0, 0.05, 0])
+    ex.setExchange(True)
+    ex.setGauge("atomic")
+    ex.brillouinZoneMesh(6)
+    ex.initializeHamiltonian()
+    ex.BShamiltonian()
+    ref = oracle_case(MOS2, 6, 2, 0, (1., 4., 13.55), exchange=True, Q=(0, 0.05, 0))
+    # MoS2 has degenerate bands: the oracle takes the facade's (lattice-gauge) stacks, whose
+    # phases inside degenerate subspaces are not pinned
+    stacks = dict(eigvec_k=np.asarray(ex.eigvecKStack), eigvec_kq=np.asarray(ex.eigvecKQStack))
+    H_lat = ref.bse(**stacks)
+    ref.gauge = "atomic"
+    H = ref.bse(**stacks)
+    assert np.abs(H - H_lat).max() > 1e-4
+    assert rel_err(ex.HBS, H) < 1e-10
+    res = ex.diagonalize(method, 6)
+    wr = np.linalg.eigvalsh(H)[:6]
+    assert np.abs(np.asarray(res.eigval)[:6] - wr).max() < 1e-6
+
+
+def test_subset_basis_hamiltonian():
+    # ExcitonTB::BShamiltonian(const arma::imat& basis) (src/ExcitonTB.cpp:840-921): the BSE over
+    # a subset of the basis rows {valence band, conduction band, k}, diagonalised through the
+    # facade; then the full basis again
+    ex = X.ExcitonTB(X.SystemConfiguration(MOS2), 6, 2, 0, [1., 4., 13.55])
+    ex.setExchange(True)
+    ex.brillouinZoneMesh(6)
+    ex.initializeHamiltonian()
+    ref = oracle_case(MOS2, 6, 2, 0, (1., 4., 13.55), exchange=True)
+    # the facade's stacks (phases inside MoS2's degenerate band pairs are not pinned)
+    H = ref.bse(eigvec_k=np.asarray(ex.eigvecKStack), eigvec_kq=np.asarray(ex.eigvecKQStack))
+    rng = np.random.default_rng(13)
+    idx = np.sort(rng.choice(ex.excitonbasisdim, size=61, replace=False))
+    basis = np.asarray(ex.basisStates)[idx]
+    ex.BShamiltonian(basis)
+    Hs = H[np.ix_(idx, idx)]
+    assert np.asarray(ex.HBS).shape == (61, 61)
+    assert rel_err(ex.HBS, Hs) < 1e-10
+    wr = np.linalg.eigvalsh(Hs)
+    res = ex.diagonalize("diag", 61)
+    assert np.abs(np.asarray(res.eigval) - wr).max() < 1e-9
+    res = ex.diagonalize("davidson", 4)
+    assert np.abs(np.asarray(res.eigval)[:4] - wr[:4]).max() < 1e-6
+    with pytest.raises(Exception, match="not among the exciton's bands"):
+        ex.BShamiltonian(np.array([[0, 1, 0]]))
+    ex.BShamiltonian()
+    assert rel_err(ex.HBS, H) < 1e-10
+    res = ex.diagonalize("davidson", 4)
+    assert np.abs(np.asarray(res.eigval)[:4] - np.linalg.eigvalsh(H)[:4]).max() < 1e-6

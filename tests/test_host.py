@@ -213,3 +213,38 @@ def test_crystal_needs_a_lattice(tmp_path):
     f.write_text("nothing here\n")
     with pytest.raises(Exception, match="Finite systems"):
         X.CRYSTALConfiguration(str(f), 10)
+
+
+def test_crystal_unrestricted_spin_blocks(tmp_path):
+    # an unrestricted (magnetic) run prints alpha Fock matrices, a line naming the beta
+    # electrons, then the beta ones; the reader makes one spinful system with the alpha and
+    # beta matrices as interleaved spin blocks and the overlap repeated for both spins.
+    # The run is made from the LDA output: its Fock matrices as alpha, twice them as beta.
+    src = (MODELS / "hBN_base_CREN_LDA.outp").read_text().split("\n")
+    first_fock = next(i for i, l in enumerate(src) if "FOCK MATRIX - CELL N." in l)
+    fock = src[first_fock:]
+    beta = []
+    for line in fock:
+        w = line.split()
+        if line.startswith("   ") and w and w[0].isdigit() and "E" in line:
+            beta.append(line[:6] + "".join(f"{2 * float(t):12.4E}" for t in w[1:]))
+        else:
+            beta.append(line)
+    text = src[:first_fock] + [" TYPE OF CALCULATION :  UNRESTRICTED OPEN SHELL"] + fock + \
+        [" BETA ELECTRONS", ""] + beta
+    f = tmp_path / "mag.outp"
+    f.write_text("\n".join(text))
+    plain = X.CRYSTALConfiguration(str(MODELS / "hBN_base_CREN_LDA.outp"), 12)
+    mag = X.CRYSTALConfiguration(str(f), 12)
+    assert mag.MAGNETIC_FLAG and not plain.MAGNETIC_FLAG
+    assert mag.nFock == plain.nFock == 12
+    assert mag.filling == 2 * plain.filling
+    assert list(mag.orbitalsPerSpecies) == list(plain.orbitalsPerSpecies)
+    assert list(mag.norbitals) == [2 * n for n in plain.norbitals]
+    H, H0 = np.asarray(mag.hamiltonian), np.asarray(plain.hamiltonian)
+    S, S0 = np.asarray(mag.overlap), np.asarray(plain.overlap)
+    assert H.shape == (16, 16, 12) and S.shape == (16, 16, 12)
+    assert np.allclose(H[0::2, 0::2], H0, atol=1e-12) and np.allclose(H[1::2, 1::2], 2 * H0, atol=1e-4)
+    assert np.all(H[0::2, 1::2] == 0) and np.all(H[1::2, 0::2] == 0)
+    assert np.array_equal(S[0::2, 0::2], S0) and np.array_equal(S[1::2, 1::2], S0)
+    assert np.all(S[0::2, 1::2] == 0)

@@ -506,3 +506,68 @@ def test_crystal_lda_bands():
     kmin = s.kpoints[np.argmin(gap)]
     assert abs(np.linalg.norm(kmin) - np.linalg.norm(s.recip[0]) / np.sqrt(3)) < 1e-5  # |K| = |b1|/sqrt(3)
     assert 4.4 < gap.min() < 4.7
+
+
+# ---------------------------------------------------------------- atomic gauge
+
+def _gauge_case(Q=(0.0, 0.0, 0.0)):
+    ex = oracle_case(MODELS / "hBN.model", 6, 1, 0, (1, 1, 10), exchange=True, Q=Q)
+    H_lat = ex.bse()
+    ex.gauge = "atomic"
+    return ex, H_lat, ex.bse()
+
+
+def test_atomic_gauge_direct_term_closed_form():
+    # with C_at(k) = exp(-i k.tau) C(k) the atom-resolved densities pick up
+    # exp(i (k - k').tau_a) (conduction pair) and exp(-i (k - k').tau_b) (valence pair), so the
+    # direct term is the lattice-gauge sum with M_ab(k' - k) exp(i (k - k').(tau_a - tau_b)), the
+    # phase of the raw (unfolded) difference; the exchange term pairs states at one k and does
+    # not change. H(i, j) = -(D - X) above the diagonal, so H_at - H_lat = D_lat - D_at there.
+    ex, H_lat, H_at = _gauge_case(Q=(0.03, 0.0, 0.0))
+    s = ex.sys
+    tau = s.motif[:, :3]
+    owner = np.repeat(np.arange(s.natoms), s.orb_per_atom)
+    assert np.abs(H_at - H_lat).max() > 1e-3  # the gauges differ for this two-atom motif
+    rng = np.random.default_rng(3)
+    checked = 0
+    for i, j in rng.integers(0, ex.N, size=(40, 2)):
+        i, j = int(min(i, j)), int(max(i, j))
+        if i == j:
+            continue
+        vi, ci, ki = ex.basis[i]
+        vj, cj, kj = ex.basis[j]
+        q = O.lib().oracle_find_equivalent(s.ndim, O._d(s.recip), O._d(s.inv),
+                                           O._d(s.kpoints[kj] - s.kpoints[ki]), ex.ncell, s.factor)
+        M = ex.ft[:, :, q]
+        rc = np.array([np.vdot(ex.eigvec_kq[owner == a, ci, ki], ex.eigvec_kq[owner == a, cj, kj])
+                       for a in range(s.natoms)])
+        rv = np.array([np.vdot(ex.eigvec_k[owner == b, vj, kj], ex.eigvec_k[owner == b, vi, ki])
+                       for b in range(s.natoms)])
+        ph = np.exp(1j * (tau @ (s.kpoints[ki] - s.kpoints[kj])))
+        D_lat = rc @ M @ rv
+        D_at = sum(rc[a] * M[a, b] * ph[a] / ph[b] * rv[b] for a in range(s.natoms) for b in range(s.natoms))
+        assert abs((H_at[i, j] - H_lat[i, j]) - (D_lat - D_at)) < 1e-12 * np.abs(H_lat).max(), (i, j)
+        checked += 1
+    assert checked > 20
+    # diagonal: D(k, k) has no phase; the matrices agree there
+    assert np.abs(np.diag(H_at) - np.diag(H_lat)).max() < 1e-12
+
+
+def test_atomic_gauge_exchange_unchanged():
+    # without the direct term only the exchange term and the diagonal remain: gauge invariant
+    ex = oracle_case(MODELS / "hBN.model", 6, 1, 0, (1, 1, 10), exchange=True, Q=(0.02, 0.01, 0.0))
+    zero = np.zeros_like(ex.ft)
+    H_lat = ex.bse(ft=zero)
+    ex.gauge = "atomic"
+    H_at = ex.bse(ft=zero)
+    assert np.abs(H_lat - np.diag(np.diag(H_lat))).max() > 1e-3
+    assert np.abs(H_at - H_lat).max() < 1e-13
+
+
+def test_atomic_gauge_single_position_is_lattice_gauge():
+    # every atom at one position: the phases are one per k-point and cancel in every density
+    ex = oracle_case(MODELS / "hBN.model", 6, 1, 0, (1, 1, 10), exchange=True)
+    H_lat = ex.bse()
+    ex.sys.motif[:, :3] = ex.sys.motif[0, :3]
+    ex.gauge = "atomic"
+    assert np.abs(ex.bse() - H_lat).max() < 1e-13

@@ -172,6 +172,17 @@ void reciprocal_tables(Context& c) {
     c.built64 = c.built32 = c.prepared = false;
 }
 
+// Back from a subset basis to the full one: the subset matrix is dropped.
+void leave_subset(Context& c) {
+    if (!c.subset) return;
+    XATU_HIP(hipStreamSynchronize(c.stream));
+    c.subset = false;
+    c.N = static_cast<int64_t>(c.nk) * c.nv * c.nc;
+    c.d_H64.release();
+    c.built64 = c.built32 = c.prepared = false;
+    c.drop_solution();
+}
+
 // Exchange factors and the FP64 diagonal (Davidson preconditioner), needed by
 // every product whether or not a matrix is resident.
 void prepare_operator(Context& c) {
@@ -330,6 +341,7 @@ int xatu_set_bands(xatu_ctx* ctx, int nk, int nv, int nc, const double* eigval_k
         if (nb > c.lat.norb) {
             throw Error(XATU_ERR_INVALID_ARGUMENT, "Error: Number of bands cannot be higher than actual material bands");
         }
+        c.subset = false;
         c.nk = nk;
         c.nv = nv;
         c.nc = nc;
@@ -371,6 +383,7 @@ int xatu_fourier_sums(xatu_ctx* ctx, const xatu_interaction* inter) {
         if (inter->mode != XATU_MODE_REALSPACE && inter->mode != XATU_MODE_RECIPROCAL) {
             throw Error(XATU_ERR_INVALID_ARGUMENT, "setMode(): mode must be either realspace or reciprocalspace");
         }
+        leave_subset(c);
         c.drop_solution();
         c.have_ft = false;
         c.built64 = c.built32 = c.prepared = false;
@@ -429,6 +442,7 @@ int xatu_build_bse(xatu_ctx* ctx, int precision, double scissor, void* hbs_out) 
         if (precision == XATU_MATRIX_FREE && hbs_out) {
             throw Error(XATU_ERR_INVALID_ARGUMENT, "a matrix-free build has no matrix to copy out");
         }
+        leave_subset(c);
         if (c.prepared && c.scissor != scissor) {
             // a resident matrix built with another scissor no longer matches the operator
             c.built64 = c.built32 = false;
@@ -477,22 +491,37 @@ int xatu_build_bse_basis(xatu_ctx* ctx, int nbasis, const int* basis, double sci
         c.require_lattice();
         c.require_bands();
         c.require_ft();
-        if (nbasis < 1 || !basis || !hbs_out) throw Error(XATU_ERR_INVALID_ARGUMENT, "bad basis");
+        if (nbasis < 1 || !basis) throw Error(XATU_ERR_INVALID_ARGUMENT, "bad basis");
         for (int i = 0; i < nbasis; ++i) {
             const int v = basis[3 * i], cc = basis[3 * i + 1], k = basis[3 * i + 2];
             if (v < 0 || v >= c.nb || cc < 0 || cc >= c.nb || k < 0 || k >= c.nk) {
                 throw Error(XATU_ERR_INVALID_ARGUMENT, "basis entry out of range");
             }
         }
+        // the exchange factors of the full basis are not used: the element kernel forms them
+        XATU_HIP(hipStreamSynchronize(c.stream));
+        c.built64 = c.built32 = c.prepared = false;
+        c.drop_solution();
+        c.d_H32.release();
         DevBuf<int> d_basis;
         d_basis.alloc(static_cast<size_t>(nbasis) * 3);
         XATU_HIP(hipMemcpyAsync(d_basis.p, basis, d_basis.bytes(), hipMemcpyHostToDevice, c.stream));
-        DevBuf<double2> H;
-        H.alloc(static_cast<size_t>(nbasis) * nbasis);
-        // exchange factors are computed on the fly by the element kernel
-        launch_build_basis(c, nbasis, d_basis.p, scissor, H.p, c.stream);
-        XATU_HIP(hipMemcpyAsync(hbs_out, H.p, H.bytes(), hipMemcpyDeviceToHost, c.stream));
+        c.d_H64.alloc(static_cast<size_t>(nbasis) * nbasis);
+        launch_build_basis(c, nbasis, d_basis.p, scissor, c.d_H64.p, c.stream);
+        // the subset matrix is now the operator: dense and FP64 Davidson solves run on it
+        c.subset = true;
+        c.N = nbasis;
+        c.d_diag.alloc(c.N);
+        launch_diag_extract(c, c.d_H64.p, c.d_diag.p, c.stream);
+        c.h_diag.resize(c.N);
+        XATU_HIP(hipMemcpyAsync(c.h_diag.data(), c.d_diag.p, c.N * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+        if (hbs_out) {
+            XATU_HIP(hipMemcpyAsync(hbs_out, c.d_H64.p, c.d_H64.bytes(), hipMemcpyDeviceToHost, c.stream));
+        }
         XATU_HIP(hipStreamSynchronize(c.stream));
+        c.built64 = true;
+        c.prepared = true;
+        ++c.prep_gen;
     });
 }
 
@@ -535,6 +564,9 @@ int xatu_solve(xatu_ctx* ctx, const xatu_solve_options* opt, double* eigval_out,
         if (opt->fp32_arith != XATU_ARITH_SPLIT16 && opt->fp32_arith != XATU_ARITH_FP32) {
             throw Error(XATU_ERR_INVALID_ARGUMENT, "fp32_arith must be XATU_ARITH_SPLIT16 or XATU_ARITH_FP32");
         }
+        if (c.subset && opt->method == XATU_SOLVE_DAVIDSON && (opt->precision != XATU_FP64 || opt->matrix_free)) {
+            throw Error(XATU_ERR_LOGIC, "a subset-basis Hamiltonian is solved with its resident FP64 matrix");
+        }
         xatu_solve_report rep{};
         c.tim.ms_matvec_kernel = 0.0;
         c.tim.n_matvec_kernel = 0;
@@ -565,6 +597,7 @@ int xatu_bse_matvec(xatu_ctx* ctx, int precision, int matrix_free, int nvec, con
     return guarded(ctx, [&](Context& c) {
         if (nvec < 1 || !x || !y) throw Error(XATU_ERR_INVALID_ARGUMENT, "bad vectors");
         c.require_bands();
+        if (precision != XATU_FP64 || matrix_free) c.require_full_basis("this product");
         const size_t n = static_cast<size_t>(c.N) * nvec;
         DevBuf<double2> X, Y;
         X.alloc(n);
@@ -580,6 +613,7 @@ int xatu_hf_gradients(xatu_ctx* ctx, int nstates, const double* eigvec, int npar
                       double* grad_out) {
     return guarded(ctx, [&](Context& c) {
         c.require_ft();
+        c.require_full_basis("xatu_hf_gradients");
         if (c.lat.recip) {
             throw Error(XATU_ERR_LOGIC, "interaction-parameter gradients are available in the real-space mode only");
         }
@@ -605,6 +639,7 @@ int xatu_hf_gradients(xatu_ctx* ctx, int nstates, const double* eigvec, int npar
 int xatu_band_gradients(xatu_ctx* ctx, int nstates, const double* eigvec, double* w_out, double* g_out) {
     return guarded(ctx, [&](Context& c) {
         c.require_ft();
+        c.require_full_basis("xatu_band_gradients");
         if (nstates < 1 || !w_out || !g_out) throw Error(XATU_ERR_INVALID_ARGUMENT, "bad args");
         DevBuf<double2> V;
         const double2* dv;
@@ -623,6 +658,7 @@ int xatu_oscillator_strengths(xatu_ctx* ctx, int nstates, const double* eigvec, 
                               double* out) {
     return guarded(ctx, [&](Context& c) {
         c.require_bands();
+        c.require_full_basis("xatu_oscillator_strengths");
         if (nstates < 1 || !velocity || !out) throw Error(XATU_ERR_INVALID_ARGUMENT, "bad args");
         DevBuf<double2> V, vel, o;
         const double2* dv;
@@ -666,6 +702,7 @@ int xatu_get_cells(xatu_ctx* ctx, int* n_cells, int* coefs) {
 
 int xatu_compare_builds(xatu_ctx* ctx, int kblock, double* block_err, double* max_abs) {
     return guarded(ctx, [&](Context& c) {
+        c.require_full_basis("xatu_compare_builds");
         if (!c.built64 || !c.built32) throw Error(XATU_ERR_LOGIC, "both FP64 and FP32 matrices must be resident");
         if (kblock < 1) throw Error(XATU_ERR_INVALID_ARGUMENT, "kblock must be positive");
         complete_matrix32(c);

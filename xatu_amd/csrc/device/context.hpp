@@ -158,6 +158,17 @@ struct Context {
     DevBuf<double2> d_ftq;      // natoms x natoms
     DevBuf<double2> d_e1, d_e2; // exchange factors: nk x natoms x (nc*nv)
 
+    // subset basis (xatu_build_bse_basis): the resident FP64 matrix and N are those of the
+    // subset until the next full build, set_bands or fourier_sums
+    bool subset = false;
+    void require_full_basis(const char* what) const {
+        if (subset) {
+            throw Error(XATU_ERR_LOGIC, std::string(what) +
+                                            " needs the full exciton basis; a subset-basis Hamiltonian is resident "
+                                            "(call xatu_build_bse)");
+        }
+    }
+
     // resident matrices
     double scissor = 0.0;
     bool built64 = false, built32 = false;

@@ -33,18 +33,7 @@ __global__ void potential_table_kernel(LatticeDesc lat, PotDesc pot, const int* 
         ++a;
     }
     const int b = a + rem;
-    double v[3] = {0.0, 0.0, 0.0};
-    for (int j = 0; j < lat.ndim; ++j) {
-        const double m = static_cast<double>(cells[3 * R + j]);
-        v[0] += m * lat.bravais[3 * j + 0];
-        v[1] += m * lat.bravais[3 * j + 1];
-        v[2] += m * lat.bravais[3 * j + 2];
-    }
-    const double dx = v[0] + lat.tau[3 * a + 0] - lat.tau[3 * b + 0];
-    const double dy = v[1] + lat.tau[3 * a + 1] - lat.tau[3 * b + 1];
-    const double dz = v[2] + lat.tau[3 * a + 2] - lat.tau[3 * b + 2];
-    const double r = sqrt(dx * dx + dy * dy + dz * dz);
-    out[static_cast<size_t>(p) * nR + R] = potential(r, pot);
+    out[static_cast<size_t>(p) * nR + R] = potential(cell_distance(lat, cells + 3 * R, a, b), pot);
 }
 
 __global__ void twiddle_kernel(int n, double2* __restrict__ tw) {

@@ -47,17 +47,7 @@ __global__ void dpotential_table_kernel(LatticeDesc lat, PotDesc pot, const int*
         ++a;
     }
     const int b = a + rem;
-    double v[3] = {0.0, 0.0, 0.0};
-    for (int j = 0; j < lat.ndim; ++j) {
-        const double m = static_cast<double>(cells[3 * R + j]);
-        v[0] += m * lat.bravais[3 * j + 0];
-        v[1] += m * lat.bravais[3 * j + 1];
-        v[2] += m * lat.bravais[3 * j + 2];
-    }
-    const double dx = v[0] + lat.tau[3 * a + 0] - lat.tau[3 * b + 0];
-    const double dy = v[1] + lat.tau[3 * a + 1] - lat.tau[3 * b + 1];
-    const double dz = v[2] + lat.tau[3 * a + 2] - lat.tau[3 * b + 2];
-    const double r = sqrt(dx * dx + dy * dy + dz * dz);
+    const double r = cell_distance(lat, cells + 3 * R, a, b);
     double d = 0.0;
     if (pot.kind == XATU_POTENTIAL_KELDYSH) {
         d = (which == 0) ? keldysh_dr0(r, pot, gl) : -potential(r, pot) / (2.0 * pot.eps_bar);

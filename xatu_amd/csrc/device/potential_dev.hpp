@@ -57,6 +57,24 @@ __host__ __device__ inline double struve_h0(double x) {
     return 2.0 / (kPi * x) * s + by0;
 }
 
+// |R + tau_a - tau_b| for the cell R = sum_j m_j a_j (integer coefficients m, padded to 3),
+// rounded operation by operation, without fused multiply-adds, in a fixed order:
+// R = m_0 a_1 + m_1 a_2 (+ m_2 a_3), d = (R + tau_a) - tau_b, r = sqrt((dx^2 + dy^2) + dz^2).
+// The Keldysh potential switches between two approximations of H0 at r / r0 = 20 (they
+// differ by ~1e-8 there), so a lattice vector that lies exactly on that radius must round
+// the same way wherever its distance is evaluated.
+__device__ inline double cell_distance(const LatticeDesc& lat, const int* m, int a, int b) {
+#pragma clang fp contract(off)
+    double v[3] = {0.0, 0.0, 0.0};
+    for (int j = 0; j < lat.ndim; ++j) {
+        const double mj = static_cast<double>(m[j]);
+        for (int x = 0; x < 3; ++x) v[x] = v[x] + mj * lat.bravais[3 * j + x];
+    }
+    double d[3];
+    for (int x = 0; x < 3; ++x) d[x] = (v[x] + lat.tau[3 * a + x]) - lat.tau[3 * b + x];
+    return sqrt((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]);
+}
+
 __device__ inline double keldysh_pref(const PotDesc& p) {
     return kEc / (8E-10 * kEps0 * p.eps_bar * p.r0);
 }

@@ -1,6 +1,6 @@
 // Reader of CRYSTAL (.outp) SCF outputs: lattice, motif, basis sizes, and the overlap
-// and Fock (Kohn-Sham) matrices of the first ncells cells, as
-// CRYSTALConfiguration (src/CRYSTALConfiguration.cpp:18-464) extracts them. The result
+// and Fock (Kohn-Sham) matrices of the first ncells cells, the data members
+// CRYSTALConfiguration (include/xatu/CRYSTALConfiguration.hpp) exposes. The result
 // is a SystemConfiguration whose systemInfo feeds SystemTB (non-orthogonal basis: the
 // bands are Lowdin-orthogonalised; the matrices are printed as lower triangles and are
 // in Hartree, see SystemTB::setAU).
@@ -33,15 +33,8 @@ public:
     int nOverlap = 0, nFock = 0, nAlpha = 0, nBeta = 0;
 
 private:
-    size_t pos_ = 0;  // next line to read
-    bool getline(std::string& line);
-    void parseContent(int ncells);
-    void parseBravaisLattice();
-    void extractDimension();
-    void parseAtoms();
-    void parseAtomicBasis();
-    std::vector<cdouble> parseMatrix();
-    void mapContent();
+    struct Scan;  // one pass over the file's lines
+    void fillSystemInfo();
 };
 
 }  // namespace xatu

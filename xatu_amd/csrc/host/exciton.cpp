@@ -264,9 +264,6 @@ void ExcitonTB::ensureContext() {
 void ExcitonTB::initializeHamiltonian() {
     if (bands.empty()) throw std::invalid_argument("Error: Exciton object must have some bands");
     if (system->nk == 0) throw std::invalid_argument("Error: BZ mesh must be initialized first");
-    if (gauge != "lattice") {
-        throw std::invalid_argument("initializeHamiltonian(): only the 'lattice' gauge is implemented by this build");
-    }
     if (valenceBands.empty() || conductionBands.empty()) {
         throw std::invalid_argument("Error: Exciton object must have valence and conduction bands");
     }
@@ -329,13 +326,38 @@ void ExcitonTB::initializeResultsH0() {
     std::vector<double> tau(static_cast<size_t>(system->natoms) * 3);
     for (int a = 0; a < system->natoms; ++a)
         for (int x = 0; x < 3; ++x) tau[3 * a + x] = system->motif[4 * a + x];
+    // Atomic gauge (SystemTB::latticeToAtomicGauge, src/SystemTB.cpp:166-182, applied in
+    // BShamiltonian, src/ExcitonTB.cpp:877-884): coefficient alpha of a state at k times
+    // exp(-i k . tau_atom(alpha)). The k+Q stack takes the phase of k as well, as the
+    // reference evaluates it at kpoints.row(k). The exposed stacks stay in the lattice gauge.
+    atomicK_.clear();
+    atomicKQ_.clear();
+    if (gauge == "atomic") {
+        const std::vector<int> opa = system->orbitalsPerAtom();
+        atomicK_ = eigvecKStack;
+        atomicKQ_ = eigvecKQStack;
+        for (int i = 0; i < nk; ++i) {
+            const double* k = system->kpoints.data() + 3 * static_cast<size_t>(i);
+            int alpha = 0;
+            for (int a = 0; a < system->natoms; ++a) {
+                const double kt = k[0] * tau[3 * a] + k[1] * tau[3 * a + 1] + k[2] * tau[3 * a + 2];
+                const cdouble ph(std::cos(kt), -std::sin(kt));
+                for (int o = 0; o < opa[a]; ++o, ++alpha)
+                    for (int b = 0; b < nb; ++b) {
+                        const size_t idx = (static_cast<size_t>(i) * nb + b) * bd + alpha;
+                        atomicK_[idx] *= ph;
+                        atomicKQ_[idx] *= ph;
+                    }
+            }
+        }
+    }
     const std::vector<int> opa = system->orbitalsPerAtom();
     check(xatu_set_lattice(ctx_, system->ndim, system->bravaisLattice.data(), system->natoms, tau.data(), opa.data(),
                            system->meshPoints));
     check(xatu_set_bands(ctx_, nk, static_cast<int>(valenceBands.size()), static_cast<int>(conductionBands.size()),
                          eigvalKStack.data(), eigvalKQStack.data(),
-                         reinterpret_cast<const double*>(eigvecKStack.data()),
-                         reinterpret_cast<const double*>(eigvecKQStack.data()), system->kpointOffsets.data()));
+                         reinterpret_cast<const double*>(bseStack(0).data()),
+                         reinterpret_cast<const double*>(bseStack(1).data()), system->kpointOffsets.data()));
     xatu_interaction in{};
     in.eps_m = eps_m;
     in.eps_s = eps_s;
@@ -352,12 +374,19 @@ void ExcitonTB::initializeResultsH0() {
     in.n_reciprocal = nReciprocalVectors;
     check(xatu_fourier_sums(ctx_, &in));
     built_ = built32_ = false;
+    subsetDim_ = 0;
+}
+
+const std::vector<cdouble>& ExcitonTB::bseStack(int which) const {
+    if (gauge == "atomic" && !atomicK_.empty()) return which ? atomicKQ_ : atomicK_;
+    return which ? eigvecKQStack : eigvecKStack;
 }
 
 // ExcitonTB::BShamiltonian (src/ExcitonTB.cpp:822-921).
 void ExcitonTB::BShamiltonian() {
     if (!initialized_) throw std::logic_error("BShamiltonian(): call initializeHamiltonian() first");
     std::cout << "BSE dimension: " << excitonbasisdim << std::endl;
+    subsetDim_ = 0;
     if (precision == "fp32") {
         check(xatu_build_bse(ctx_, XATU_FP32, scissor, nullptr));
         built32_ = true;
@@ -367,13 +396,34 @@ void ExcitonTB::BShamiltonian() {
     }
 }
 
+// ExcitonTB::BShamiltonian(const arma::imat& basis) (src/ExcitonTB.cpp:840-921): the BSE
+// over the given rows {valence band, conduction band, k index} (absolute band indices, as
+// in basisStates), built in FP64 and kept resident as the operator of diagonalize().
 void ExcitonTB::BShamiltonian(const std::vector<long long>& basis) {
     if (basis.empty()) {
         BShamiltonian();
         return;
     }
-    throw std::invalid_argument(
-        "BShamiltonian(basis): subset bases are exposed through buildSubsetHamiltonian in this build");
+    if (!initialized_) throw std::logic_error("BShamiltonian(): call initializeHamiltonian() first");
+    if (basis.size() % 3 != 0) throw std::invalid_argument("BShamiltonian(basis): basis must have 3 columns");
+    const long long n = static_cast<long long>(basis.size() / 3);
+    std::vector<int> cols(basis.size());
+    for (long long i = 0; i < n; ++i) {
+        for (int j = 0; j < 2; ++j) {
+            const auto it = bandToIndex.find(static_cast<int>(basis[3 * i + j]));
+            if (it == bandToIndex.end()) {
+                throw std::invalid_argument("BShamiltonian(basis): band " + std::to_string(basis[3 * i + j]) +
+                                            " is not among the exciton's bands");
+            }
+            cols[3 * i + j] = it->second;
+        }
+        cols[3 * i + 2] = static_cast<int>(basis[3 * i + 2]);
+    }
+    std::cout << "BSE dimension: " << n << std::endl;
+    check(xatu_build_bse_basis(ctx_, static_cast<int>(n), cols.data(), scissor, nullptr));
+    built_ = true;
+    built32_ = false;
+    subsetDim_ = n;
 }
 
 std::unique_ptr<ResultTB> ExcitonTB::diagonalize(const std::string& method, int nstates) {
@@ -381,7 +431,7 @@ std::unique_ptr<ResultTB> ExcitonTB::diagonalize(const std::string& method, int 
         throw std::invalid_argument("diagonalizeRaw(): BSE Hamiltonian is not initialized.");
     }
     xatu_solve_options opt{};
-    const long long N = excitonbasisdim;
+    const long long N = hbsDim();
     int ncols;
     if (method == "diag") {
         if (!built_) throw std::invalid_argument("diagonalize('diag') needs the FP64 matrix (setPrecision('fp64'))");
@@ -391,8 +441,9 @@ std::unique_ptr<ResultTB> ExcitonTB::diagonalize(const std::string& method, int 
     } else if (method == "davidson" || method == "sparse") {
         opt.method = XATU_SOLVE_DAVIDSON;
         opt.nstates = nstates;
-        opt.precision = (precision == "fp32") ? XATU_FP32 : XATU_FP64;
-        opt.matrix_free = matrixFree ? 1 : 0;
+        // a subset-basis Hamiltonian is solved with its resident FP64 matrix
+        opt.precision = (precision == "fp32" && subsetDim_ == 0) ? XATU_FP32 : XATU_FP64;
+        opt.matrix_free = (matrixFree && subsetDim_ == 0) ? 1 : 0;
         opt.block = davidsonBlock;
         opt.tol = davidsonTol;
         ncols = nstates;
@@ -409,7 +460,7 @@ std::unique_ptr<ResultTB> ExcitonTB::diagonalize(const std::string& method, int 
 
 std::vector<cdouble> ExcitonTB::HBS() const {
     if (!built_) throw std::logic_error("HBS: the FP64 BSE matrix has not been built");
-    std::vector<cdouble> h(static_cast<size_t>(excitonbasisdim) * excitonbasisdim);
+    std::vector<cdouble> h(static_cast<size_t>(hbsDim()) * hbsDim());
     check(xatu_get_bse(ctx_, XATU_FP64, h.data()));
     return h;
 }
@@ -505,6 +556,23 @@ std::vector<cdouble> ExcitonTB::hoppingGradients(const std::vector<cdouble>& eig
     std::vector<cdouble> g(static_cast<size_t>(nstates) * 2 * bd * nb * nk);
     const double* v = eigvec.empty() ? nullptr : reinterpret_cast<const double*>(eigvec.data());
     check(xatu_band_gradients(ctx_, nstates, v, w.data(), reinterpret_cast<double*>(g.data())));
+    if (gauge == "atomic") {
+        // the device differentiates with respect to the atomic-gauge coefficients
+        // C_at = p C, p = exp(-i k.tau): dE/d conj(C) = conj(p) dE/d conj(C_at)
+        const std::vector<int> opa = system->orbitalsPerAtom();
+        for (int k = 0; k < nk; ++k) {
+            const double* kp = system->kpoints.data() + 3 * static_cast<size_t>(k);
+            int alpha = 0;
+            for (int a = 0; a < system->natoms; ++a) {
+                const double* t = system->motif.data() + 4 * static_cast<size_t>(a);
+                const double kt = kp[0] * t[0] + kp[1] * t[1] + kp[2] * t[2];
+                const cdouble cp(std::cos(kt), std::sin(kt));
+                for (int o = 0; o < opa[a]; ++o, ++alpha)
+                    for (int sst = 0; sst < 2 * nstates; ++sst)
+                        for (int b = 0; b < nb; ++b) g[((static_cast<size_t>(sst) * nk + k) * nb + b) * bd + alpha] *= cp;
+            }
+        }
+    }
 
     const size_t bd2 = static_cast<size_t>(bd) * bd;
     // A[st][s][k] (bd x bd, column-major)

@@ -88,8 +88,11 @@ public:
     double davidsonTol = 0.0;
     int device = 0;
 
-    // The resident BSE matrix copied to the host as a full N x N column-major matrix.
+    // The resident BSE matrix copied to the host as a full column-major matrix of the
+    // current basis (hbsDim() x hbsDim(): the full basis, or the subset given to
+    // BShamiltonian(basis)).
     std::vector<cdouble> HBS() const;
+    long long hbsDim() const { return subsetDim_ > 0 ? subsetDim_ : excitonbasisdim; }
     std::vector<cdouble> HBS32() const;  // FP32 matrix (widened), when built in fp32
     bool hasHBS() const { return built_; }
     // Lattice Fourier stacks (ftMotifStack natoms x natoms x nmesh, ftMotifQ natoms x natoms).
@@ -127,7 +130,13 @@ private:
     void ensureContext();
     void check(int rc) const;
 
+    // band stacks in the gauge the BSE uses (a copy with the atomic-gauge phases, or the
+    // lattice-gauge stacks themselves)
+    const std::vector<cdouble>& bseStack(int which) const;
+    std::vector<cdouble> atomicK_, atomicKQ_;
+
     xatu_ctx* ctx_ = nullptr;
+    long long subsetDim_ = 0;  // > 0: the resident matrix is that of a subset basis
     bool built_ = false;
     bool built32_ = false;
     bool initialized_ = false;

@@ -326,6 +326,15 @@ PYBIND11_MODULE(_xatu, m) {
             py::gil_scoped_release rel;
             e.BShamiltonian();
         })
+        .def("BShamiltonian",
+             [](ExcitonTB& e, py::array_t<long long, py::array::c_style | py::array::forcecast> basis) {
+                 if (basis.ndim() != 2 || basis.shape(1) != 3) {
+                     throw std::invalid_argument("BShamiltonian(basis): basis must be an n x 3 integer array");
+                 }
+                 std::vector<long long> b(basis.data(), basis.data() + basis.size());
+                 py::gil_scoped_release rel;
+                 e.BShamiltonian(b);
+             })
         .def("diagonalize", &ExcitonTB::diagonalize, py::arg("method") = "diag", py::arg("nstates") = 8,
              py::call_guard<py::gil_scoped_release>())
         .def_property_readonly("system", [](const ExcitonTB& e) { return e.system; })
@@ -375,7 +384,7 @@ PYBIND11_MODULE(_xatu, m) {
                                })
         .def_property_readonly("HBS",
                                [](const ExcitonTB& e) {
-                                   const py::ssize_t n = e.excitonbasisdim;
+                                   const py::ssize_t n = e.hbsDim();
                                    return fortran(e.HBS(), {n, n});
                                })
         .def_property_readonly("HBS32",
