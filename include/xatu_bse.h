@@ -192,6 +192,16 @@ int xatu_solve(xatu_ctx* ctx, const xatu_solve_options* opt, double* eigval_out,
  * FP32 products use the default arithmetic (XATU_ARITH_SPLIT16). */
 int xatu_bse_matvec(xatu_ctx* ctx, int precision, int matrix_free, int nvec, const double* x, double* y);
 
+/* No counterpart in the reference: y = H x with the resident FP64 matrix for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). x and y are DEVICE pointers (N x nvec complex128,
+ * column-major, caller-allocated); the call allocates nothing: its staged vectors and
+ * partial sums live in a 64 MiB scratch array that each GPU holds from xatu_create until
+ * the process ends, so N is limited to 65536 (XATU_ERR_INVALID_ARGUMENT beyond). Every call
+ * on one GPU shares that GPU's scratch: calls that overlap on one GPU (other threads, other
+ * contexts) race. Synchronous on the context's stream; same kernels as xatu_bse_matvec. */
+int xatu_bse_matvec_noalloc(xatu_ctx* ctx, int nvec, const double* x, double* y);
+
 /* Hellmann-Feynman gradients dE_n/dtheta = psi_n^H (dH/dtheta) psi_n for the screening
  * parameters (XATU_PARAM_*), without forming dH. eigvec: N x nstates complex128 (null: the
  * eigenvectors of the last solve). grad_out: nparams x nstates, state index fastest.

@@ -35,7 +35,7 @@ EXPORTED = [
     "xatu_fourier_sums", "xatu_build_bse", "xatu_build_bse_basis", "xatu_get_bse", "xatu_free_bse",
     "xatu_solve", "xatu_bse_matvec", "xatu_hf_gradients", "xatu_get_fourier", "xatu_get_cells",
     "xatu_compare_builds", "xatu_get_timings", "xatu_band_gradients",
-    "xatu_oscillator_strengths",
+    "xatu_oscillator_strengths", "xatu_bse_matvec_noalloc",
 ]
 
 
@@ -93,6 +93,7 @@ def load(path: Path | str | None = None) -> ctypes.CDLL:
         L.xatu_free_bse.argtypes = [_vp, ctypes.c_int]
         L.xatu_solve.argtypes = [_vp, ctypes.POINTER(SolveOptions), _vp, _vp, ctypes.POINTER(SolveReport)]
         L.xatu_bse_matvec.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _vp]
+        L.xatu_bse_matvec_noalloc.argtypes = [_vp, ctypes.c_int, _vp, _vp]
         L.xatu_hf_gradients.argtypes = [_vp, ctypes.c_int, _vp, ctypes.c_int, _vp, _vp]
         L.xatu_band_gradients.argtypes = [_vp, ctypes.c_int, _vp, _vp, _vp]
         L.xatu_oscillator_strengths.argtypes = [_vp, ctypes.c_int, _vp, _vp, _vp]
@@ -218,6 +219,11 @@ class Context:
         self._check(self.L.xatu_bse_matvec(self._h, precision, int(matrix_free), x.shape[1], _ptr(x),
                                            _ptr(y)))
         return y
+
+    def matvec_noalloc(self, x_dev: int, y_dev: int, nvec: int):
+        """y = H x with the resident FP64 matrix, x and y device addresses of N x nvec complex128
+        column-major buffers (no allocation inside the call)."""
+        self._check(self.L.xatu_bse_matvec_noalloc(self._h, int(nvec), _vp(x_dev), _vp(y_dev)))
 
     def hf_gradients(self, nstates, params, eigvec=None):
         p = np.ascontiguousarray(params, dtype=np.int32)

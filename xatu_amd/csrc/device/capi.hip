@@ -4,17 +4,50 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <new>
 
 #include "context.hpp"
+#include "mfma.hpp"
 
 using namespace xatu_dev;
+
+// Scratch of the no-allocation product (xatu_bse_matvec_noalloc): staged vectors and the
+// partial sums of the column splits. Every GPU the library runs on holds its own copy at
+// its own address until the process ends; host code reaches it only through the address
+// hipGetSymbolAddress returns for the current device (set up by xatu_create).
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+constexpr int64_t kNoAllocScratchElems = static_cast<int64_t>(kNoAllocScratchBytes / sizeof(double2));
+static __device__ __align__(256) double2 scratch[kNoAllocScratchElems];
 
 struct xatu_ctx {
     Context c;
 };
 
 namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double2* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of the no-allocation scratch on the current device, looked up once per
+// device (the device must be current).
+void setup_scratch(int device) {
+    if (device < 0 || device >= kMaxDevices) throw Error(XATU_ERR_INVALID_ARGUMENT, "device index too large");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return;
+    void* p = nullptr;
+    XATU_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw Error(XATU_ERR_DEVICE, "no device address for the no-allocation scratch");
+    g_scratch_ptr[device] = static_cast<double2*>(p);
+}
+
+double2* scratch_address(int device) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    double2* p = (device >= 0 && device < kMaxDevices) ? g_scratch_ptr[device] : nullptr;
+    if (!p) throw Error(XATU_ERR_LOGIC, "no-allocation scratch not set up for this device");
+    return p;
+}
 
 template <typename F>
 int guarded(xatu_ctx* ctx, F&& f) {
@@ -269,6 +302,7 @@ int xatu_create(xatu_ctx** out, int device) {
         XATU_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
         XATU_BLAS(rocblas_create_handle(&c.blas));
         XATU_BLAS(rocblas_set_stream(c.blas, c.stream));
+        setup_scratch(device);  // guarded() made the device current
     });
     if (rc != XATU_OK) {
         xatu_destroy(ctx);
@@ -605,6 +639,23 @@ int xatu_bse_matvec(xatu_ctx* ctx, int precision, int matrix_free, int nvec, con
         XATU_HIP(hipMemcpyAsync(X.p, x, n * sizeof(double2), hipMemcpyHostToDevice, c.stream));
         apply_operator(c, precision, matrix_free, X.p, Y.p, nvec, nullptr);
         XATU_HIP(hipMemcpyAsync(y, Y.p, n * sizeof(double2), hipMemcpyDeviceToHost, c.stream));
+        XATU_HIP(hipStreamSynchronize(c.stream));
+    });
+}
+
+int xatu_bse_matvec_noalloc(xatu_ctx* ctx, int nvec, const double* x, double* y) {
+    return guarded(ctx, [&](Context& c) {
+        if (nvec < 1 || !x || !y) throw Error(XATU_ERR_INVALID_ARGUMENT, "bad vectors");
+        c.require_bands();
+        // the staged vectors and at least one split of partial sums must fit
+        if (c.N > kNoAllocScratchElems / (2 * kBlockVectors)) {
+            throw Error(XATU_ERR_INVALID_ARGUMENT,
+                        "BSE dimension " + std::to_string(c.N) + " exceeds the no-allocation product's scratch (N <= " +
+                            std::to_string(kNoAllocScratchElems / (2 * kBlockVectors)) + ")");
+        }
+        if (!c.built64) throw Error(XATU_ERR_LOGIC, "FP64 BSE matrix is not resident");
+        launch_matvec64_scratch(c, reinterpret_cast<const double2*>(x), reinterpret_cast<double2*>(y), nvec,
+                                scratch_address(c.device), kNoAllocScratchElems, c.stream);
         XATU_HIP(hipStreamSynchronize(c.stream));
     });
 }

@@ -287,6 +287,12 @@ void launch_compare(const Context& c, int kblock, double* d_blockerr, hipStream_
 // Dense products Y = H X (N x nvec, complex FP64 vectors) with the resident FP64 or
 // FP32 matrix (the FP32 product stages and accumulates in FP32).
 void launch_matvec64(Context& c, const double2* X, double2* Y, int nvec, hipStream_t s);
+// The FP64 product with a caller-provided scratch (staged vectors, then the partial sums
+// of as many column splits as fit, at most max_splits): no allocation.
+// matvec64_scratch_elems is the scratch the product uses with at most max_splits splits.
+int64_t matvec64_scratch_elems(int64_t N, int max_splits);
+void launch_matvec64_scratch(const Context& c, const double2* X, double2* Y, int nvec, double2* scratch,
+                             int64_t scratch_elems, hipStream_t s);
 void launch_matvec32(Context& c, const double2* X, double2* Y, int nvec, hipStream_t s);
 // Products with the resident FP32 matrix on the FP16 matrix cores, operands split into
 // hi + lo FP16 parts (split_product.hip). split_product_scale sets h16_scale from bounds

@@ -420,6 +420,21 @@ void launch_matvec64(Context& c, const double2* X, double2* Y, int nvec, hipStre
     dense_product<double, double, double>(c, c.d_H64.p, X, Y, nvec, s, c.s_xt64, c.s_yp64);
 }
 
+int64_t matvec64_scratch_elems(int64_t N, int max_splits) {
+    return N * kVec * (1 + product_splits<double>(N, max_splits));
+}
+
+void launch_matvec64_scratch(const Context& c, const double2* X, double2* Y, int nvec, double2* scratch,
+                             int64_t scratch_elems, hipStream_t s) {
+    const int64_t N = c.N;
+    // as many splits as the scratch holds beside the staged vectors (same kernels and, when
+    // they fit, the same splits as the allocating product)
+    const int max_splits = static_cast<int>(std::min<int64_t>(64, scratch_elems / (N * kVec) - 1));
+    if (max_splits < 1) throw Error(XATU_ERR_INVALID_ARGUMENT, "scratch too small for the product");
+    const int splits = product_splits<double>(N, max_splits);
+    dense_product_into<double, double, double>(N, c.d_H64.p, X, Y, nvec, s, scratch, scratch + N * kVec, splits);
+}
+
 // Product with the FP32 matrix: on the FP16 matrix cores with split operands where the
 // matrix layout allows it (even N), else staged in FP32 and accumulated in FP32 on the
 // FP32 matrix cores; widened to FP64 on output either way.
