@@ -12,6 +12,7 @@
 //   oracle_ft_stack        src/ExcitonTB.cpp:761-781   (initializeMotifFT loop, OpenMP over mesh)
 //   oracle_find_equivalent src/Lattice.cpp:228-279     (findEquivalentPointBZ)
 //   oracle_bse             src/ExcitonTB.cpp:520-548, 840-921 (realSpaceInteractionTerm, BShamiltonian)
+//   oracle_bse_columns, oracle_bse_apply: the same elements, selected columns and H X
 // Extra potential kinds (no reference counterpart) give the explicit derivatives
 // dV/dr0 and dV/deps_m used to build dH for the gradient checks.
 #include <cmath>
@@ -288,21 +289,27 @@ int oracle_find_equivalent(int ndim, const double* recip, const double* inv, con
 //  orb_atom: atom of every orbital (norb)
 //  ft: natoms x natoms x nmesh complex, ftq: natoms x natoms
 //  out: N x N complex, column-major
-void oracle_bse(int N, const int* basis, int nk, const double* kpoints, int norb, int nb, const double* evk,
-                const double* evkq, const double* ek, const double* ekq, int natoms, const int* orb_atom,
-                const double* ft, const double* ftq, int exchange, double scissor, int ndim, const double* recip,
-                const double* inv, int ncell, int factor, double* out) {
-    (void)nk;
-    const cd* EK = reinterpret_cast<const cd*>(evk);
-    const cd* EKQ = reinterpret_cast<const cd*>(evkq);
-    const cd* FT = reinterpret_cast<const cd*>(ft);
-    const cd* FTQ = reinterpret_cast<const cd*>(ftq);
-    cd* H = reinterpret_cast<cd*>(out);
-    const uint64_t n = static_cast<uint64_t>(N);
-    for (uint64_t i = 0; i < n * n; ++i) H[i] = 0.0;
+// The reference's BSE element rule (BShamiltonian, src/ExcitonTB.cpp:857-919) for the
+// upper triangle i <= j of the basis rows {v, c, k}: -(D - X) off the diagonal and
+// h + conj(h) with h = (scissor + Ec - Ev)/2 - (D - X)/2 on it, D and X from
+// realSpaceInteractionTerm (src/ExcitonTB.cpp:520-548: atom-resolved densities contracted
+// with the lattice-sum matrix by arma::dot, no conjugation).
+struct BseElements {
+    const int* basis;
+    const double* kpoints;
+    int norb, nb, natoms;
+    const cd *EK, *EKQ, *FT, *FTQ;
+    const double *ek, *ekq;
+    const int* orb_atom;
+    int exchange;
+    double scissor;
+    int ndim;
+    const double *recip, *inv;
+    int ncell, factor;
 
-    auto term = [&](const cd* c1, const cd* c2, const cd* c3, const cd* c4, const cd* M) {
-        std::vector<cd> r1(natoms, 0.0), r2(natoms, 0.0);
+    cd term(const cd* c1, const cd* c2, const cd* c3, const cd* c4, const cd* M) const {
+        cd r1[16], r2[16];
+        for (int a = 0; a < natoms; ++a) r1[a] = r2[a] = 0.0;
         for (int al = 0; al < norb; ++al) {
             r1[orb_atom[al]] += std::conj(c1[al]) * c3[al];
             r2[orb_atom[al]] += std::conj(c2[al]) * c4[al];
@@ -314,33 +321,83 @@ void oracle_bse(int N, const int* basis, int nk, const double* kpoints, int norb
             t += r1[a] * y;  // arma::dot: no conjugation
         }
         return t;
-    };
+    }
 
+    cd upper(long long i, long long j) const {
+        const int v = basis[3 * i], c = basis[3 * i + 1], k = basis[3 * i + 2];
+        const int v2 = basis[3 * j], c2 = basis[3 * j + 1], k2 = basis[3 * j + 2];
+        const cd* coefsK = EK + (static_cast<size_t>(k) * nb + v) * norb;
+        const cd* coefsKQ = EKQ + (static_cast<size_t>(k) * nb + c) * norb;
+        const cd* coefsK2 = EK + (static_cast<size_t>(k2) * nb + v2) * norb;
+        const cd* coefsK2Q = EKQ + (static_cast<size_t>(k2) * nb + c2) * norb;
+        double dk[3];
+        for (int x = 0; x < 3; ++x) dk[x] = kpoints[3 * k2 + x] - kpoints[3 * k + x];
+        const int q = oracle_find_equivalent(ndim, recip, inv, dk, ncell, factor);
+        const cd D = term(coefsKQ, coefsK2, coefsK2Q, coefsK, FT + static_cast<size_t>(q) * natoms * natoms);
+        cd X = 0.0;
+        if (exchange) X = term(coefsKQ, coefsK2, coefsK, coefsK2Q, FTQ);
+        if (i == j) {
+            const cd h = (scissor + ekq[static_cast<size_t>(k) * nb + c] - ek[static_cast<size_t>(k) * nb + v]) / 2. -
+                         (D - X) / 2.;
+            return h + std::conj(h);
+        }
+        return -(D - X);
+    }
+
+    // H(i, j) of the Hermitian matrix HBS + HBS^H
+    cd at(long long i, long long j) const { return i <= j ? upper(i, j) : std::conj(upper(j, i)); }
+};
+
+#define XATU_ORACLE_BSE_ARGS                                                                                 \
+    int N, const int *basis, int nk, const double *kpoints, int norb, int nb, const double *evk,             \
+        const double *evkq, const double *ek, const double *ekq, int natoms, const int *orb_atom,            \
+        const double *ft, const double *ftq, int exchange, double scissor, int ndim, const double *recip,    \
+        const double *inv, int ncell, int factor
+#define XATU_ORACLE_BSE_INIT                                                                                  \
+    BseElements E{basis, kpoints, norb, nb, natoms, reinterpret_cast<const cd*>(evk),                        \
+                  reinterpret_cast<const cd*>(evkq), reinterpret_cast<const cd*>(ft),                        \
+                  reinterpret_cast<const cd*>(ftq), ek, ekq, orb_atom, exchange, scissor, ndim, recip, inv,  \
+                  ncell, factor};                                                                            \
+    (void)nk;
+
+// The whole N x N matrix, column-major.
+void oracle_bse(XATU_ORACLE_BSE_ARGS, double* out) {
+    XATU_ORACLE_BSE_INIT
+    cd* H = reinterpret_cast<cd*>(out);
+    const uint64_t n = static_cast<uint64_t>(N);
 #pragma omp parallel for schedule(dynamic, 64)
     for (long long i = 0; i < N; ++i) {
         for (long long j = i; j < N; ++j) {
-            const int v = basis[3 * i], c = basis[3 * i + 1], k = basis[3 * i + 2];
-            const int v2 = basis[3 * j], c2 = basis[3 * j + 1], k2 = basis[3 * j + 2];
-            const cd* coefsK = EK + (static_cast<size_t>(k) * nb + v) * norb;
-            const cd* coefsKQ = EKQ + (static_cast<size_t>(k) * nb + c) * norb;
-            const cd* coefsK2 = EK + (static_cast<size_t>(k2) * nb + v2) * norb;
-            const cd* coefsK2Q = EKQ + (static_cast<size_t>(k2) * nb + c2) * norb;
-            double dk[3];
-            for (int x = 0; x < 3; ++x) dk[x] = kpoints[3 * k2 + x] - kpoints[3 * k + x];
-            const int q = oracle_find_equivalent(ndim, recip, inv, dk, ncell, factor);
-            const cd D = term(coefsKQ, coefsK2, coefsK2Q, coefsK, FT + static_cast<size_t>(q) * natoms * natoms);
-            cd X = 0.0;
-            if (exchange) X = term(coefsKQ, coefsK2, coefsK, coefsK2Q, FTQ);
-            if (i == j) {
-                const cd h = (scissor + ekq[static_cast<size_t>(k) * nb + c] - ek[static_cast<size_t>(k) * nb + v]) / 2. -
-                             (D - X) / 2.;
-                H[i + i * n] = h + std::conj(h);
-            } else {
-                const cd h = -(D - X);
-                H[i + j * n] = h;
-                H[j + i * n] = std::conj(h);
-            }
+            const cd h = E.upper(i, j);
+            H[i + j * n] = h;
+            if (i != j) H[j + i * n] = std::conj(h);
         }
+    }
+}
+
+// Columns cols[0..ncols) of the matrix (N x ncols, column-major).
+void oracle_bse_columns(XATU_ORACLE_BSE_ARGS, int ncols, const int* cols, double* out) {
+    XATU_ORACLE_BSE_INIT
+    cd* O = reinterpret_cast<cd*>(out);
+#pragma omp parallel for schedule(dynamic, 256)
+    for (long long i = 0; i < N; ++i)
+        for (int c = 0; c < ncols; ++c) O[i + static_cast<size_t>(c) * N] = E.at(i, cols[c]);
+}
+
+// Y = H X for nvec vectors (N x nvec, column-major) without storing H: every element is
+// evaluated by the element rule for its row.
+void oracle_bse_apply(XATU_ORACLE_BSE_ARGS, int nvec, const double* x, double* y) {
+    XATU_ORACLE_BSE_INIT
+    const cd* X = reinterpret_cast<const cd*>(x);
+    cd* Y = reinterpret_cast<cd*>(y);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (long long i = 0; i < N; ++i) {
+        std::vector<cd> acc(nvec, 0.0);
+        for (long long j = 0; j < N; ++j) {
+            const cd h = E.at(i, j);
+            for (int n = 0; n < nvec; ++n) acc[n] += h * X[j + static_cast<size_t>(n) * N];
+        }
+        for (int n = 0; n < nvec; ++n) Y[i + static_cast<size_t>(n) * N] = acc[n];
     }
 }
 

@@ -70,10 +70,15 @@ def lib():
                                       + [ctypes.c_double] * 6 + [_dp])
         L.oracle_find_equivalent.restype = ctypes.c_int
         L.oracle_find_equivalent.argtypes = [ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int]
+        bse_args = [ctypes.c_int, _ip, ctypes.c_int, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp,
+                    _dp, ctypes.c_int, _ip, _dp, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp,
+                    _dp, ctypes.c_int, ctypes.c_int]
         L.oracle_bse.restype = None
-        L.oracle_bse.argtypes = [ctypes.c_int, _ip, ctypes.c_int, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp,
-                                 _dp, ctypes.c_int, _ip, _dp, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp,
-                                 _dp, ctypes.c_int, ctypes.c_int, _dp]
+        L.oracle_bse.argtypes = bse_args + [_dp]
+        L.oracle_bse_columns.restype = None
+        L.oracle_bse_columns.argtypes = bse_args + [ctypes.c_int, _ip, _dp]
+        L.oracle_bse_apply.restype = None
+        L.oracle_bse_apply.argtypes = bse_args + [ctypes.c_int, _dp, _dp]
         _LIB = L
     return _LIB
 
@@ -355,9 +360,9 @@ class Exciton:
                               _d(np.asarray(k, float)), kind, *self._pot_args(), out.ctypes.data_as(_dp))
         return out
 
-    def bse(self, ft=None, ftq=None, eigval_k=None, eigval_kq=None, scissor=None, eigvec_k=None,
-            eigvec_kq=None, exchange=None):
-        """Full HBS (+ conjugate lower triangle) from the restated per-element loop."""
+    def _bse_args(self, ft=None, ftq=None, eigval_k=None, eigval_kq=None, scissor=None, eigvec_k=None,
+                  eigvec_kq=None, exchange=None):
+        """Arguments of the C restatement of the element rule, plus the arrays they point into."""
         s = self.sys
         ft = self.ft if ft is None else ft
         ftq = self.ftq if ftq is None else ftq
@@ -369,26 +374,41 @@ class Exciton:
             vk, vkq = atomic_gauge(s, vk), atomic_gauge(s, vkq)
         sc = self.scissor if scissor is None else scissor
         ex = self.exchange if exchange is None else exchange
-        orb_atom = np.repeat(np.arange(s.natoms), s.orb_per_atom).astype(np.int32)
+        keep = [np.repeat(np.arange(s.natoms), s.orb_per_atom).astype(np.int32), np.ascontiguousarray(s.kpoints)]
+        keep += [np.asfortranarray(a) for a in (vk, vkq, ek, ekq, ft, ftq)]
+        orb_atom, kp, vk_, vkq_, ek_, ekq_, ft_, ftq_ = keep
+        ptr = lambda a: a.ctypes.data_as(_dp)  # noqa: E731
+        args = (self.N, self.basis.ctypes.data_as(_ip), self.nk, ptr(kp), s.basisdim, len(self.band_list), ptr(vk_),
+                ptr(vkq_), ptr(ek_), ptr(ekq_), s.natoms, orb_atom.ctypes.data_as(_ip), ptr(ft_), ptr(ftq_),
+                int(bool(ex)), float(sc), s.ndim, _d(s.recip), _d(s.inv), self.ncell, s.factor)
+        return args, keep
+
+    def bse(self, **kw):
+        """Full HBS (+ conjugate lower triangle) from the restated per-element loop. Keywords
+        replace the stacks, lattice sums, scissor or exchange flag (see _bse_args)."""
+        args, keep = self._bse_args(**kw)
         out = np.zeros((self.N, self.N), complex, order="F")
-        kp = np.ascontiguousarray(s.kpoints)
-
-        def fp(a):
-            a = np.asfortranarray(a)
-            return a, a.ctypes.data_as(_dp)
-
-        ft_, ftp = fp(ft)
-        ftq_, ftqp = fp(ftq)
-        vk_, vkp = fp(vk)
-        vkq_, vkqp = fp(vkq)
-        ek_, ekp = fp(ek)
-        ekq_, ekqp = fp(ekq)
-        lib().oracle_bse(self.N, self.basis.ctypes.data_as(_ip), self.nk, _d(kp), s.basisdim,
-                         len(self.band_list), vkp, vkqp, ekp, ekqp, s.natoms,
-                         orb_atom.ctypes.data_as(_ip), ftp, ftqp, int(bool(ex)), float(sc), s.ndim,
-                         _d(s.recip), _d(s.inv), self.ncell, s.factor, out.ctypes.data_as(_dp))
+        lib().oracle_bse(*args, out.ctypes.data_as(_dp))
         return out
 
+    def bse_columns(self, cols, **kw):
+        """Columns cols of HBS (N x len(cols)) from the same element rule."""
+        args, keep = self._bse_args(**kw)
+        c = np.ascontiguousarray(cols, dtype=np.int32)
+        out = np.zeros((self.N, len(c)), complex, order="F")
+        lib().oracle_bse_columns(*args, len(c), c.ctypes.data_as(_ip), out.ctypes.data_as(_dp))
+        return out
+
+    def bse_apply(self, x, **kw):
+        """HBS x for x of shape N x nvec, every element evaluated by the element rule (no matrix
+        is stored)."""
+        args, keep = self._bse_args(**kw)
+        x = np.asfortranarray(x, dtype=complex)
+        if x.ndim == 1:
+            x = x[:, None]
+        y = np.zeros_like(x, order="F")
+        lib().oracle_bse_apply(*args, x.shape[1], x.ctypes.data_as(_dp), y.ctypes.data_as(_dp))
+        return y
 
 def davidson_reference(M, neigval, tol=1e-8):
     """davidson_method restated (src/davidson.cpp:5-61); returns (eigval[:neigval], eigvec (N x 2 neigval))."""

@@ -571,3 +571,14 @@ def test_atomic_gauge_single_position_is_lattice_gauge():
     ex.sys.motif[:, :3] = ex.sys.motif[0, :3]
     ex.gauge = "atomic"
     assert np.abs(ex.bse() - H_lat).max() < 1e-13
+
+
+def test_bse_columns_and_apply_match_matrix():
+    # the column and product routines evaluate the same element rule as the full matrix
+    ex = oracle_case(MODELS / "MoS2.model", 5, 2, 0, (1., 4., 13.55), exchange=True, Q=(0, 0.05, 0))
+    H = ex.bse()
+    cols = [0, 3, ex.N // 2, ex.N - 1]
+    assert np.array_equal(ex.bse_columns(cols), H[:, cols])
+    rng = np.random.default_rng(1)
+    x = rng.normal(size=(ex.N, 3)) + 1j * rng.normal(size=(ex.N, 3))
+    assert np.abs(ex.bse_apply(x) - H @ x).max() < 1e-12 * np.abs(H @ x).max()
