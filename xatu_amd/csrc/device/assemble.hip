@@ -72,8 +72,10 @@ __device__ inline void store_h(cx_t<R>* H, int64_t idx, cx_t<R> v) {
 }
 
 // Stage the coefficient vectors of TK row k-points (slots [0,TK)) and TK column
-// k-points (slots [TK,2TK)) in LDS: per slot [norb][NB], conduction columns first
-// (from the k+Q stack), then valence columns (from the k stack).
+// k-points (slots [TK,2TK)) in LDS, slot fastest: entry (orbital al, band) of slot s at
+// s_coef[(al NB + band) 2 TK + s], conduction bands first (from the k+Q stack), then the
+// valence bands (from the k stack). The threads of a wave own consecutive row slots, so
+// their reads of one entry fall on consecutive LDS words.
 template <typename R, int NC, int NV, int TK>
 __device__ inline void stage_coefficients(cx_t<R>* s_coef, int norb, int nk, int Kr, int Kc,
                                           const double2* __restrict__ evk, const double2* __restrict__ evkq,
@@ -92,16 +94,17 @@ __device__ inline void stage_coefficients(cx_t<R>* s_coef, int norb, int nk, int
                                           : evk[(static_cast<size_t>(k) * NB + (band - NC)) * norb + al];
             val = to_cx<R>(g);
         }
-        s_coef[(static_cast<size_t>(slot) * norb + al) * NB + band] = val;
+        s_coef[(static_cast<size_t>(al) * NB + band) * (2 * TK) + slot] = val;
     }
 }
 
 // The nc*nv x nc*nv block of (D - X) for the k-point pair (ka, kb), ka <= kb, from the
-// staged coefficients ca_ (ka) and cb_ (kb). sink(cA, vA, cB, vB, DX) receives every
-// element. The instruction sequence depends only on the data, so any two threads
-// evaluating the same pair produce bit-identical values.
+// staged coefficients ca_ (ka) and cb_ (kb): entry (orbital al, band) of a k-point at
+// ptr[(al NB + band) cs]. sink(cA, vA, cB, vB, DX) receives every element. The
+// instruction sequence depends only on the data, so any two threads evaluating the same
+// pair produce bit-identical values.
 template <typename R, int NA, int NC, int NV, typename Sink>
-__device__ inline void pair_block(const LatticeDesc& lat, const cx_t<R>* ca_, const cx_t<R>* cb_, int ka, int kb,
+__device__ inline void pair_block(const LatticeDesc& lat, const cx_t<R>* ca_, const cx_t<R>* cb_, int cs, int ka, int kb,
                                   const double2* __restrict__ ft, const int* __restrict__ kcoord,
                                   const double2* __restrict__ e1, const double2* __restrict__ e2, int exchange,
                                   Sink&& sink) {
@@ -126,9 +129,9 @@ __device__ inline void pair_block(const LatticeDesc& lat, const cx_t<R>* ca_, co
         for (int be = lat.orb_begin[b]; be < lat.orb_begin[b + 1]; ++be) {
 #pragma unroll
             for (int vb = 0; vb < NV; ++vb) {
-                const C xb = cb_[be * NB + NC + vb];
+                const C xb = cb_[(be * NB + NC + vb) * cs];
 #pragma unroll
-                for (int va = 0; va < NV; ++va) cfma_conj(rv[b][vb][va], xb, ca_[be * NB + NC + va]);
+                for (int va = 0; va < NV; ++va) cfma_conj(rv[b][vb][va], xb, ca_[(be * NB + NC + va) * cs]);
             }
         }
     }
@@ -141,9 +144,9 @@ __device__ inline void pair_block(const LatticeDesc& lat, const cx_t<R>* ca_, co
 #pragma unroll
             for (int cB = 0; cB < NC; ++cB) rc[a][cB] = mk(R(0), R(0));
             for (int al = lat.orb_begin[a]; al < lat.orb_begin[a + 1]; ++al) {
-                const C xa = ca_[al * NB + cA];
+                const C xa = ca_[(al * NB + cA) * cs];
 #pragma unroll
-                for (int cB = 0; cB < NC; ++cB) cfma_conj(rc[a][cB], xa, cb_[al * NB + cB]);
+                for (int cB = 0; cB < NC; ++cB) cfma_conj(rc[a][cB], xa, cb_[(al * NB + cB) * cs]);
             }
         }
         // T_b(cB) = sum_a rho^c_a(cB) M_ab
@@ -193,9 +196,14 @@ __device__ inline double diag_value(const double* __restrict__ eval_k, const dou
     return h + h;
 }
 
-// Structured build kernel: tile of TK x TK k-point pairs, one pair per thread.
+// Structured build kernel: tile of TK x TK k-point pairs, one pair per thread. With
+// STAGE the (TK NCV)^2 tile of H is assembled in LDS (column-major, one element of
+// padding per column) and then written column by column, consecutive threads on
+// consecutive rows, so that every store instruction writes whole contiguous column
+// segments; without it (the FP64 build, whose tile would cost occupancy) each pair
+// stores its block directly.
 // R: arithmetic type of the assembly (double or float).
-template <typename R, int NA, int NC, int NV, int TK>
+template <typename R, int NA, int NC, int NV, int TK, bool STAGE>
 __global__ __launch_bounds__(TK* TK) void bse_tile_kernel(LatticeDesc lat, int nk, const double2* __restrict__ evk,
                                                           const double2* __restrict__ evkq,
                                                           const double* __restrict__ eval_k,
@@ -204,20 +212,23 @@ __global__ __launch_bounds__(TK* TK) void bse_tile_kernel(LatticeDesc lat, int n
                                                           const double2* __restrict__ ft,
                                                           const double2* __restrict__ e1,
                                                           const double2* __restrict__ e2, int exchange,
-                                                          double scissor, int lower_rows,
+                                                          double scissor, int lower_rows, float hsplit,
                                                           cx_t<R>* __restrict__ H) {
     using C = cx_t<R>;
     constexpr int NB = NC + NV;
     constexpr int NCV = NC * NV;
+    constexpr int ROWS = TK * NCV;
+    constexpr int LDT = ROWS + 1;
     extern __shared__ unsigned char smem_raw[];
-    C* s_coef = reinterpret_cast<C*>(smem_raw);
     const int norb = lat.norb;
+    C* s_coef = reinterpret_cast<C*>(smem_raw);
+    C* s_tile = s_coef + static_cast<size_t>(2 * TK) * norb * NB;
     const int Kr = blockIdx.x, Kc = blockIdx.y;
     // block-lower build: skip tiles whose last row lies above the row block of their first
     // column (no entry of theirs is read)
     if (lower_rows > 0) {
-        const int64_t last_row = static_cast<int64_t>(Kr + 1) * TK * NCV - 1;
-        const int64_t first_col = static_cast<int64_t>(Kc) * TK * NCV;
+        const int64_t last_row = static_cast<int64_t>(Kr + 1) * ROWS - 1;
+        const int64_t first_col = static_cast<int64_t>(Kc) * ROWS;
         if (last_row < first_col / lower_rows * lower_rows) return;
     }
     stage_coefficients<R, NC, NV, TK>(s_coef, norb, nk, Kr, Kc, evk, evkq, TK * TK);
@@ -226,35 +237,80 @@ __global__ __launch_bounds__(TK* TK) void bse_tile_kernel(LatticeDesc lat, int n
     const int t = threadIdx.x;
     const int tr = t % TK, tc = t / TK;
     const int kr = Kr * TK + tr, kc = Kc * TK + tc;
-    if (kr >= nk || kc >= nk) return;
-    const bool sw = kr > kc;
-    const int ka = sw ? kc : kr;
-    const int kb = sw ? kr : kc;
-    const C* ca_ = s_coef + static_cast<size_t>(sw ? (TK + tc) : tr) * norb * NB;
-    const C* cb_ = s_coef + static_cast<size_t>(sw ? tr : (TK + tc)) * norb * NB;
-    const int64_t N = static_cast<int64_t>(nk) * NCV;
-    const bool diagblk = (ka == kb);
-    pair_block<R, NA, NC, NV>(lat, ca_, cb_, ka, kb, ft, kcoord, e1, e2, exchange,
-                              [&](int cA, int vA, int cB, int vB, C DX) {
-                                  const int ia = cA * NV + vA, ib = cB * NV + vB;
-                                  const int64_t i = static_cast<int64_t>(ka) * NCV + ia;
-                                  const int64_t j = static_cast<int64_t>(kb) * NCV + ib;
-                                  if (diagblk) {
-                                      if (ia < ib) {
-                                          const C f = mk(-DX.x, -DX.y);
-                                          H[i + j * N] = f;
-                                          H[j + i * N] = cconj(f);
-                                      } else if (ia == ib) {
-                                          const double h = diag_value(eval_k, eval_kq, NB, NV, ka, cA, vA, scissor,
-                                                                      static_cast<double>(DX.x));
-                                          H[i + i * N] = mk(R(h), R(0));
+    if (kr < nk && kc < nk) {
+        const bool sw = kr > kc;
+        const int ka = sw ? kc : kr;
+        const int kb = sw ? kr : kc;
+        const C* ca_ = s_coef + (sw ? (TK + tc) : tr);
+        const C* cb_ = s_coef + (sw ? tr : (TK + tc));
+        const bool diagblk = (ka == kb);
+        const int64_t N = static_cast<int64_t>(nk) * NCV;
+        // element (r, c) of this pair's block: to the LDS tile, or straight to H
+        const int r0 = tr * NCV, c0 = tc * NCV;
+        const int64_t g0 = static_cast<int64_t>(Kr) * ROWS, h0 = static_cast<int64_t>(Kc) * ROWS;
+        auto put = [&](int r, int c, C v) {
+            if (STAGE) {
+                s_tile[c * LDT + r] = v;
+            } else {
+                H[(g0 + r) + (h0 + c) * N] = v;
+            }
+        };
+        pair_block<R, NA, NC, NV>(lat, ca_, cb_, 2 * TK, ka, kb, ft, kcoord, e1, e2, exchange,
+                                  [&](int cA, int vA, int cB, int vB, C DX) {
+                                      const int ia = cA * NV + vA, ib = cB * NV + vB;
+                                      if (diagblk) {
+                                          if (ia < ib) {
+                                              const C f = mk(-DX.x, -DX.y);
+                                              put(r0 + ia, c0 + ib, f);
+                                              put(r0 + ib, c0 + ia, cconj(f));
+                                          } else if (ia == ib) {
+                                              const double h = diag_value(eval_k, eval_kq, NB, NV, ka, cA, vA, scissor,
+                                                                          static_cast<double>(DX.x));
+                                              put(r0 + ia, c0 + ia, mk(R(h), R(0)));
+                                          }
+                                      } else if (!sw) {
+                                          put(r0 + ia, c0 + ib, mk(-DX.x, -DX.y));
+                                      } else {
+                                          put(r0 + ib, c0 + ia, mk(-DX.x, DX.y));
                                       }
-                                  } else if (!sw) {
-                                      H[i + j * N] = mk(-DX.x, -DX.y);
-                                  } else {
-                                      H[j + i * N] = mk(-DX.x, DX.y);
-                                  }
-                              });
+                                  });
+    }
+    if (STAGE) {
+        __syncthreads();
+        const int64_t N = static_cast<int64_t>(nk) * NCV;
+        const int64_t row0 = static_cast<int64_t>(Kr) * ROWS, col0 = static_cast<int64_t>(Kc) * ROWS;
+        constexpr bool kSplittable = sizeof(R) == 4 && ROWS % 32 == 0;
+        if (kSplittable && hsplit > 0.0f) {
+            // split FP16 layout of the product on the FP16 matrix cores (ROWS a multiple of
+            // 32, N a multiple of 32): 32 entries x = a + i b of a column and a 32-row step
+            // occupy the same 256 bytes as in FP32, as 16 pieces of 8 halves, piece
+            // (part q, group g) at byte 16 (4 q + g) holding part q (a hi, a lo, b hi, b lo of
+            // hsplit x) of the rows 8 (e >> 1) + 2 g + (e & 1), e = 0..7
+            constexpr int kSteps = kSplittable ? ROWS / 32 : 1;
+            for (int e = t; e < ROWS * kSteps * 16; e += TK * TK) {
+                const int piece = e & 15, step = (e >> 4) % kSteps, c = (e >> 4) / kSteps;
+                const int q = piece >> 2, g = piece & 3;
+                // a step lies wholly inside or outside the matrix (N and row0 multiples of 32)
+                if (col0 + c >= N || row0 + 32 * step >= N) continue;
+                f16x8 out;
+#pragma unroll
+                for (int el = 0; el < 8; ++el) {
+                    const int r = 32 * step + 8 * (el >> 1) + 2 * g + (el & 1);
+                    const C v = s_tile[c * LDT + r];
+                    const float x = static_cast<float>((q < 2) ? v.x : v.y) * hsplit;
+                    const _Float16 hi = static_cast<_Float16>(x);
+                    out[el] = (q & 1) ? static_cast<_Float16>(x - static_cast<float>(hi)) : hi;
+                }
+                f16x8* dst = reinterpret_cast<f16x8*>(H + (row0 + 32 * step) + (col0 + c) * N) + piece;
+                *dst = out;
+            }
+        } else {
+            for (int e = t; e < ROWS * ROWS; e += TK * TK) {
+                const int r = e % ROWS, c = e / ROWS;
+                if (row0 + r < N && col0 + c < N) H[(row0 + r) + (col0 + c) * N] = s_tile[c * LDT + r];
+            }
+        }
+    }
 }
 
 // Diagonal of H only (Davidson preconditioner without a resident matrix): the
@@ -283,9 +339,9 @@ __global__ __launch_bounds__(TK* TK) void bse_diag_kernel(LatticeDesc lat, int n
     const int tr = t % TK, tc = t / TK;
     const int k = K * TK + tr;
     if (tr != tc || k >= nk) return;
-    const C* ca_ = s_coef + static_cast<size_t>(tr) * norb * NB;
-    const C* cb_ = s_coef + static_cast<size_t>(TK + tc) * norb * NB;
-    pair_block<R, NA, NC, NV>(lat, ca_, cb_, k, k, ft, kcoord, e1, e2, exchange,
+    const C* ca_ = s_coef + tr;
+    const C* cb_ = s_coef + TK + tc;
+    pair_block<R, NA, NC, NV>(lat, ca_, cb_, 2 * TK, k, k, ft, kcoord, e1, e2, exchange,
                               [&](int cA, int vA, int cB, int vB, C DX) {
                                   if (cA == cB && vA == vB) {
                                       diag[static_cast<int64_t>(k) * NCV + cA * NV + vA] =
@@ -389,7 +445,7 @@ __global__ __launch_bounds__(matfree_threads(TK)) void bse_matfree_kernel(
                 const C* ca_ = sw ? s_col + static_cast<size_t>(tc) * ss : s_row + static_cast<size_t>(tr) * ss;
                 const C* cb_ = sw ? s_row + static_cast<size_t>(tr) * ss : s_col + static_cast<size_t>(tc) * ss;
                 const bool diagblk = (ka == kb);
-                pair_block<R, NA, NC, NV>(lat, ca_, cb_, ka, kb, ft, kcoord, e1, e2, exchange,
+                pair_block<R, NA, NC, NV>(lat, ca_, cb_, 1, ka, kb, ft, kcoord, e1, e2, exchange,
                                           [&](int cA, int vA, int cB, int vB, C DX) {
                                               const int ia = cA * NV + vA, ib = cB * NV + vB;
                                               // element (row, col) of the tile at s_h[col * LDH + row]
@@ -732,9 +788,21 @@ __global__ void compare_kernel(const double2* __restrict__ H64, const float2* __
     if (threadIdx.x == 0) out[bi + static_cast<size_t>(nblk) * bj] = s[0];
 }
 
+// The FP32 build stages its output tile in LDS for coalesced stores; the FP64 tile would
+// halve the workgroups per CU.
+template <typename R>
+constexpr bool kStageTile = sizeof(R) == 4;
+
 template <typename R, int TK>
 size_t coef_bytes(const Context& c, int nb) {
     return static_cast<size_t>(2 * TK) * c.lat.norb * nb * sizeof(cx_t<R>);
+}
+
+// LDS of the tile kernel: staged coefficients and the padded output tile.
+template <typename R, int TK>
+size_t tile_bytes(const Context& c, int nb, int ncv) {
+    const size_t rows = static_cast<size_t>(TK) * ncv;
+    return coef_bytes<R, TK>(c, nb) + (kStageTile<R> ? rows * (rows + 1) * sizeof(cx_t<R>) : 0);
 }
 
 // Launchers for one (natoms, nc, nv) shape.
@@ -745,17 +813,31 @@ struct Shape {
     static bool matches(const Context& c) { return c.lat.natoms == NA && c.nc == NC && c.nv == NV; }
 
     template <int TK>
-    static void build_tk(const Context& c, cx_t<R>* H, int lower_rows, hipStream_t s) {
+    static void build_tk(const Context& c, cx_t<R>* H, int lower_rows, float hsplit, hipStream_t s) {
         const int nT = (c.nk + TK - 1) / TK;
-        const size_t lds = coef_bytes<R, TK>(c, NB);
-        hipLaunchKernelGGL((bse_tile_kernel<R, NA, NC, NV, TK>), dim3(nT, nT), dim3(TK * TK), lds, s, c.lat, c.nk, c.d_evec_k.p, c.d_evec_kq.p, c.d_eval_k.p, c.d_eval_kq.p, c.d_kcoord.p,
-                           c.d_ft.p, c.d_e1.p, c.d_e2.p, c.inter.exchange, c.scissor, lower_rows, H);
+        const size_t lds = tile_bytes<R, TK>(c, NB, NCV);
+        if (hsplit > 0.0f && ((TK * NCV) % 32 != 0 || c.N % 32 != 0 || !kStageTile<R>)) {
+            throw Error(XATU_ERR_LOGIC, "split FP16 layout needs 32-row tiles");
+        }
+        hipLaunchKernelGGL((bse_tile_kernel<R, NA, NC, NV, TK, kStageTile<R>>), dim3(nT, nT), dim3(TK * TK), lds, s, c.lat, c.nk, c.d_evec_k.p, c.d_evec_kq.p, c.d_eval_k.p, c.d_eval_kq.p, c.d_kcoord.p,
+                           c.d_ft.p, c.d_e1.p, c.d_e2.p, c.inter.exchange, c.scissor, lower_rows, hsplit, H);
     }
-    static void build(const Context& c, cx_t<R>* H, hipStream_t s, int lower_rows = 0) {
-        if (coef_bytes<R, 16>(c, NB) <= 64 * 1024) {
-            build_tk<16>(c, H, lower_rows, s);
-        } else if (coef_bytes<R, 8>(c, NB) <= 160 * 1024) {
-            build_tk<8>(c, H, lower_rows, s);
+    // The tile size the build uses (0: none fits).
+    static int build_tile(const Context& c) {
+        constexpr size_t kLds = 160 * 1024;
+        if (tile_bytes<R, 16>(c, NB, NCV) <= kLds) return 16;
+        if (tile_bytes<R, 8>(c, NB, NCV) <= kLds) return 8;
+        if (tile_bytes<R, 4>(c, NB, NCV) <= kLds) return 4;
+        return 0;
+    }
+    static void build(const Context& c, cx_t<R>* H, hipStream_t s, int lower_rows = 0, float hsplit = 0.0f) {
+        const int tk = build_tile(c);
+        if (tk == 16) {
+            build_tk<16>(c, H, lower_rows, hsplit, s);
+        } else if (tk == 8) {
+            build_tk<8>(c, H, lower_rows, hsplit, s);
+        } else if (tk == 4) {
+            build_tk<4>(c, H, lower_rows, hsplit, s);
         } else {
             throw Error(XATU_ERR_INVALID_ARGUMENT, "too many orbitals for the tile kernel");
         }
@@ -900,10 +982,23 @@ void launch_build_tiles64(const Context& c, double2* H, hipStream_t s) {
     build_generic<double>(c, H, 0, s);
 }
 
-void launch_build_tiles32(const Context& c, float2* H, bool lower, hipStream_t s) {
+void launch_build_tiles32(const Context& c, float2* H, bool lower, bool presplit, hipStream_t s) {
     const int lower_rows = lower ? kSplitRowBlock : 0;
-    if (for_shape<float>(c, [&](auto sh) { decltype(sh)::build(c, H, s, lower_rows); })) return;
+    const float hsplit = presplit ? c.h16_scale : 0.0f;
+    if (for_shape<float>(c, [&](auto sh) { decltype(sh)::build(c, H, s, lower_rows, hsplit); })) return;
+    if (presplit) throw Error(XATU_ERR_LOGIC, "split FP16 layout needs a specialised tile kernel");
     build_generic<float>(c, H, lower_rows, s);
+}
+
+bool presplit_build_available(const Context& c) {
+    if (c.N % 32 != 0) return false;
+    bool ok = false;
+    for_shape<float>(c, [&](auto sh) {
+        using S = decltype(sh);
+        const int tk = S::build_tile(c);
+        ok = tk > 0 && (tk * S::NCV) % 32 == 0;
+    });
+    return ok;
 }
 
 void launch_diag_direct(const Context& c, double* diag, hipStream_t s) {

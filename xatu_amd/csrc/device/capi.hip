@@ -501,8 +501,10 @@ int xatu_build_bse(xatu_ctx* ctx, int precision, double scissor, void* hbs_out) 
             // only the block-lower part when the split product will read it (and nothing is
             // copied out); completed on demand for the FP32 matrix-core product and copies
             c.h32_lower = !hbs_out && split_layout_available(c);
+            // ... and in the split FP16 layout the product on the FP16 matrix cores loads
+            c.h32_presplit = c.h32_lower && presplit_build_available(c);
             if (c.h32_lower) clear_upper_heads(c);
-            launch_build_tiles32(c, c.d_H32.p, c.h32_lower, c.stream);
+            launch_build_tiles32(c, c.d_H32.p, c.h32_lower, c.h32_presplit, c.stream);
         }
         evk.stop();
         ev.stop();

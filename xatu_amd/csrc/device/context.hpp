@@ -184,6 +184,7 @@ struct Context {
     DevBuf<double2> s_xt64, s_yp64;   // staged vectors / split partial sums, FP64 dense product
     DevBuf<float2> s_xt32, s_yp32;    // same, FP32 dense product (partial sums also of the split product)
     bool h32_lower = false;           // the FP32 matrix holds only its block-lower part (split product)
+    bool h32_presplit = false;        // ... and holds it as scaled split FP16 parts (the product's load order)
     DevBuf<float4> s_xs16;            // vectors staged as split-FP16 fragments
     DevBuf<float4> s_xt16;            // same, in the row order of the transposed-block products
     DevBuf<float2> s_pt32;            // partial sums of the transposed-block products (split product)
@@ -268,7 +269,11 @@ void launch_exchange_factors(const Context& c, hipStream_t s);
 void launch_build_tiles64(const Context& c, double2* H, hipStream_t s);
 // lower: only the tiles holding entries (k, i) with k >= kSplitRowBlock floor(i / kSplitRowBlock)
 // (the block-lower part of column-major H read by the split product)
-void launch_build_tiles32(const Context& c, float2* H, bool lower, hipStream_t s);
+// presplit: write the block-lower part in the split FP16 layout the product on the FP16
+// matrix cores reads (see presplit_build_available).
+void launch_build_tiles32(const Context& c, float2* H, bool lower, bool presplit, hipStream_t s);
+// The FP32 build can write the split FP16 layout: N and the tile rows multiples of 32.
+bool presplit_build_available(const Context& c);
 // Fills the block-upper part of a FP32 matrix built block-lower (Hermitian mirror).
 void complete_matrix32(Context& c);
 // Before a block-lower build: zeroes the never-built entries that the split product reads

@@ -165,6 +165,17 @@ __global__ void stage_split_kernel(const double2* __restrict__ X, int64_t N, int
     }
 }
 
+// The four loads of a tile step already hold the parts (a hi, a lo, b hi, b lo) when the
+// matrix is stored split (the FP32 build writes that layout for this product).
+__device__ inline HFrag16 presplit_h(const float4 (&r)[4]) {
+    HFrag16 f;
+    f.ah = __builtin_bit_cast(f16x8, r[0]);
+    f.al = __builtin_bit_cast(f16x8, r[1]);
+    f.bh = __builtin_bit_cast(f16x8, r[2]);
+    f.bl = __builtin_bit_cast(f16x8, r[3]);
+    return f;
+}
+
 __device__ inline HFrag16 split_h(const float4 (&r)[4], float sH) {
     HFrag16 f;
 #pragma unroll
@@ -219,7 +230,7 @@ __device__ inline void wave_lds_sync() {
 //  sums over their 64 rows are combined in LDS and stored per step to Pt[b][k][32].
 // Xt holds the vectors in the i order of the transposed fragments: element e of lane
 // (n, g) of fragment jj of the wave's 64 rows holds row 32 jj + 16 (e & 1) + 4 g + (e >> 1).
-template <int TW>
+template <int TW, bool PRESPLIT>
 __global__ __launch_bounds__(256, 1) void herm_product_kernel(const float2* __restrict__ H,
                                                               const XFrag16* __restrict__ Xs,
                                                               const XFrag16* __restrict__ Xt,
@@ -321,7 +332,7 @@ __global__ __launch_bounds__(256, 1) void herm_product_kernel(const float2* __re
         for (int tp = 0; tp < TW / 2; ++tp) {
             HFrag16 hf[2];
 #pragma unroll
-            for (int j = 0; j < 2; ++j) hf[j] = split_h(rb[2 * tp + j], sH);
+            for (int j = 0; j < 2; ++j) hf[j] = PRESPLIT ? presplit_h(rb[2 * tp + j]) : split_h(rb[2 * tp + j], sH);
 #pragma unroll
             for (int j = 0; j < 2; ++j)
 #pragma unroll
@@ -445,6 +456,229 @@ __global__ __launch_bounds__(256, 1) void herm_product_kernel(const float2* __re
         }
 }
 
+// The same product for a matrix stored in the split layout, organised per wave: wave w of
+// the workgroup takes the steps w, w + 4, ... of the item and, for each, all 256 columns i of
+// the row block (eight groups of 32 = two tiles of 16). Direct sums accumulate per wave over
+// its steps for all 256 rows (combined across the four waves once, at the end of the item);
+// the transposed sums of a step are complete inside the wave (its 256 i) and are stored to
+// Pt[b][k][32] at once, so the waves never wait for each other inside the loop. The vectors
+// of the row block in the transposed-fragment order are staged in LDS once per item; each
+// wave transposes its split parts through its own LDS buffer, one group at a time. Two
+// register sets of two tiles alternate, so one group is loaded while the other is used.
+struct Herm2Lds {
+    union {
+        struct {
+            XFrag16 xi[4][2][kVec][4];       // [group][jj][n][g]
+            _Float16 tr[4][4][32 * kTrRow];  // per wave, per part: [rho][pair][li][tile & 1]
+        } loop;
+        float2 sum[2][kHermRows][kVec];      // end of item: direct sums of waves 0 + 2, 1 + 3
+    };
+};
+
+__global__ __launch_bounds__(256, 1) void herm_product_wave_kernel(const float2* __restrict__ H,
+                                                                   const XFrag16* __restrict__ Xs,
+                                                                   const XFrag16* __restrict__ Xt,
+                                                                   const int2* __restrict__ items, int64_t N,
+                                                                   float2* __restrict__ Pn, float2* __restrict__ Pt) {
+    __shared__ Herm2Lds lds;
+    const int2 bc = items[blockIdx.x];
+    const int b = bc.x, c = bc.y;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int li = lane & 15, g = lane >> 4;
+    const int64_t rb0 = static_cast<int64_t>(b) * kHermRows;
+    const int64_t cbeg = static_cast<int64_t>(c) * kHermChunk;
+    const int64_t kend = min(N, cbeg + kHermChunk);
+    const bool diag = cbeg <= rb0;
+    const int64_t kbeg = diag ? rb0 : cbeg;
+    const int nd = diag ? static_cast<int>((min(kend, rb0 + kHermRows) - rb0 + 31) / 32) : 0;
+    const int64_t klow = diag ? rb0 + kHermRows : cbeg;
+    const int nl = kend > klow ? static_cast<int>((kend - klow + 31) / 32) : 0;
+    const int nsteps = nd + nl;
+    const int mine = wave < nsteps ? (nsteps - wave + 3) / 4 : 0;  // steps of this wave
+
+    // vectors of the row block in the i order of the transposed fragments
+    {
+        const XFrag16* src = Xt + static_cast<int64_t>(b) * 4 * 2 * kVec * 4;
+        XFrag16* dst = &lds.loop.xi[0][0][0][0];
+        for (int e = threadIdx.x; e < 4 * 2 * kVec * 4; e += 256) dst[e] = src[e];
+    }
+    __syncthreads();
+
+    f32x4 acc[16][2][2];
+#pragma unroll
+    for (int t = 0; t < 16; ++t)
+#pragma unroll
+        for (int u = 0; u < 2; ++u) acc[t][u][0] = acc[t][u][1] = f32x4{0, 0, 0, 0};
+    f32x4 acct[2][2][2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int u = 0; u < 2; ++u) acct[h][u][0] = acct[h][u][1] = f32x4{0, 0, 0, 0};
+    // column i = rb0 + 16 t + li of tile t (rows past N clamped to N - 1: their outputs are
+    // not stored and their vector entries are staged as zero)
+    const int64_t i0 = rb0 + li;
+    const float4* hbase = reinterpret_cast<const float4*>(H + (i0 < N ? i0 : N - 1) * N) + g;
+    const int64_t tstride = 16 * N / 2;  // float4 per 16 columns
+    auto kstep = [&](int st) -> int64_t { return st < nd ? kbeg + 32 * st : klow + 32 * (st - nd); };
+    auto load = [&](float4 (&rb)[2][4], int j, int G) {
+        // step j of this wave (clamped to its last; to the item's last when it has none)
+        const int st = min(wave + 4 * min(j, max(mine - 1, 0)), nsteps - 1);
+        const int64_t k0 = kstep(st);
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            const int64_t i = rb0 + 16 * (2 * G + t) + li;
+            const float4* col = (i < N) ? hbase + (2 * G + t) * tstride : hbase;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) rb[t][q] = col[k0 / 2 + 4 * q];
+        }
+    };
+    _Float16* trw = &lds.loop.tr[wave][0][0];
+    // one group of two tiles of step st: direct products, and (below the diagonal block)
+    // the transposed products through the wave's LDS buffer
+    auto group = [&](const float4 (&rb)[2][4], const XFrag16 (&xb)[2], int G, bool low) {
+        HFrag16 hf[2];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) hf[t] = presplit_h(rb[t]);
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            const HFrag16& h = hf[t];
+            const f16x8 nbh = -h.bh, nbl = -h.bl;
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const XFrag16& x = xb[u];
+                f32x4 re = acc[2 * G + t][u][0], im = acc[2 * G + t][u][1];
+                re = mma_f16(h.al, x.ch, re);
+                re = mma_f16(h.ah, x.cl, re);
+                re = mma_f16(h.ah, x.ch, re);
+                re = mma_f16(h.bl, x.dh, re);
+                re = mma_f16(h.bh, x.dl, re);
+                re = mma_f16(h.bh, x.dh, re);
+                im = mma_f16(h.al, x.dh, im);
+                im = mma_f16(h.ah, x.dl, im);
+                im = mma_f16(h.ah, x.dh, im);
+                im = mma_f16(nbl, x.ch, im);
+                im = mma_f16(nbh, x.cl, im);
+                im = mma_f16(nbh, x.ch, im);
+                acc[2 * G + t][u][0] = re;
+                acc[2 * G + t][u][1] = im;
+            }
+        }
+        if (!low) return;
+        wave_lds_sync();  // the previous group's reads of the buffer are done
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+                const int rho = 8 * (e >> 1) + 4 * (e & 1) + g;
+                _Float16* dst = trw + rho * kTrRow + 2 * li;
+                const HFrag16& h0 = hf[0];
+                const HFrag16& h1 = hf[1];
+                *reinterpret_cast<f16x2*>(dst) = f16x2{h0.ah[e], h1.ah[e]};
+                *reinterpret_cast<f16x2*>(dst + 32 * kTrRow) = f16x2{h0.al[e], h1.al[e]};
+                *reinterpret_cast<f16x2*>(dst + 64 * kTrRow) = f16x2{h0.bh[e], h1.bh[e]};
+                *reinterpret_cast<f16x2*>(dst + 96 * kTrRow) = f16x2{h0.bl[e], h1.bl[e]};
+            }
+        wave_lds_sync();
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+                const _Float16* src = trw + (16 * h + li) * kTrRow + 8 * g;
+                const f16x8 ah = *reinterpret_cast<const f16x8*>(src);
+                const f16x8 al = *reinterpret_cast<const f16x8*>(src + 32 * kTrRow);
+                const f16x8 bh = *reinterpret_cast<const f16x8*>(src + 64 * kTrRow);
+                const f16x8 bl = *reinterpret_cast<const f16x8*>(src + 96 * kTrRow);
+                const f16x8 nbh = -bh, nbl = -bl;
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    const XFrag16 x = lds.loop.xi[G >> 1][G & 1][16 * u + li][g];
+                    f32x4 re = acct[h][u][0], im = acct[h][u][1];
+                    re = mma_f16(al, x.ch, re);
+                    re = mma_f16(ah, x.cl, re);
+                    re = mma_f16(ah, x.ch, re);
+                    re = mma_f16(nbl, x.dh, re);
+                    re = mma_f16(nbh, x.dl, re);
+                    re = mma_f16(nbh, x.dh, re);
+                    im = mma_f16(al, x.dh, im);
+                    im = mma_f16(ah, x.dl, im);
+                    im = mma_f16(ah, x.dh, im);
+                    im = mma_f16(bl, x.ch, im);
+                    im = mma_f16(bh, x.cl, im);
+                    im = mma_f16(bh, x.ch, im);
+                    acct[h][u][0] = re;
+                    acct[h][u][1] = im;
+                }
+            }
+    };
+    float4 bufa[2][4], bufb[2][4];
+    load(bufa, 0, 0);
+    load(bufb, 0, 1);
+    for (int j = 0; j < mine; ++j) {
+        const int st = wave + 4 * j;
+        const int64_t k0 = kstep(st);
+        const bool low = st >= nd;
+        XFrag16 xb[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) xb[u] = Xs[((k0 / 32) * kVec + 16 * u + li) * 4 + g];
+        // groups 0..7 alternate between the two register sets; each set is reloaded with the
+        // group two ahead (in the next step for groups 6 and 7) as soon as it is consumed
+        group(bufa, xb, 0, low);
+        load(bufa, j, 2);
+        group(bufb, xb, 1, low);
+        load(bufb, j, 3);
+        group(bufa, xb, 2, low);
+        load(bufa, j, 4);
+        group(bufb, xb, 3, low);
+        load(bufb, j, 5);
+        group(bufa, xb, 4, low);
+        load(bufa, j, 6);
+        group(bufb, xb, 5, low);
+        load(bufb, j, 7);
+        group(bufa, xb, 6, low);
+        load(bufa, j + 1, 0);
+        group(bufb, xb, 7, low);
+        load(bufb, j + 1, 1);
+        if (low) {
+            float2* out = Pt + static_cast<int64_t>(b) * N * kVec;
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+#pragma unroll
+                for (int u = 0; u < 2; ++u)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int64_t k = k0 + 16 * h + 8 * (g >> 1) + (g & 1) + 2 * v;
+                        if (k < N) out[k * kVec + 16 * u + li] = make_float2(acct[h][u][0][v], acct[h][u][1][v]);
+                        acct[h][u][0][v] = 0.0f;
+                        acct[h][u][1][v] = 0.0f;
+                    }
+        }
+    }
+    // direct sums of the four waves: waves 0, 1 store, waves 2, 3 add, all write Pn
+    __syncthreads();  // every wave is done with the loop's LDS
+    auto deposit = [&](bool add) {
+        float2(&slot)[kHermRows][kVec] = lds.sum[wave & 1];
+#pragma unroll
+        for (int t = 0; t < 16; ++t)
+#pragma unroll
+            for (int u = 0; u < 2; ++u)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    float2& d = slot[16 * t + 4 * g + v][16 * u + li];
+                    const float2 x = make_float2(acc[t][u][0][v], acc[t][u][1][v]);
+                    d = add ? make_float2(d.x + x.x, d.y + x.y) : x;
+                }
+    };
+    if (wave < 2) deposit(false);
+    __syncthreads();
+    if (wave >= 2) deposit(true);
+    __syncthreads();
+    float2* out = Pn + static_cast<int64_t>(c) * N * kVec;
+    for (int e = threadIdx.x; e < kHermRows * kVec; e += 256) {
+        const int r = e / kVec, n = e % kVec;
+        const int64_t i = rb0 + r;
+        if (i < N) {
+            const float2 a = lds.sum[0][r][n], d = lds.sum[1][r][n];
+            out[i * kVec + n] = make_float2(a.x + d.x, a.y + d.y);
+        }
+    }
+}
+
 // Y[r][n] (column-major, FP64) = scale[n] (sum over the items' direct partial sums of r and
 // the transposed partial sums of r from the row blocks above it); 64 rows per workgroup
 // (all in one row block), fixed summation order.
@@ -504,6 +738,38 @@ __global__ __launch_bounds__(256) void herm_reduce_kernel(const float2* __restri
         const int rr = f % kRows, n = f / kRows;
         const int64_t r = r0 + rr;
         if (r < N && n < nvec) Y[r + static_cast<int64_t>(n) * N] = tile[rr][n];
+    }
+}
+
+// Back from the split layout to FP32 complex entries, in place: a wave converts four
+// consecutive 256-byte column steps (128 entries; N is a multiple of 32, so the steps of a
+// column are consecutive in memory) per pass, entry = (hi + lo) / scale for both parts.
+// Steps of the block-upper part (never built) are rewritten the same way; the mirror of
+// the block-lower part overwrites them afterwards.
+__global__ __launch_bounds__(256) void unsplit_kernel(float2* __restrict__ H, int64_t N, float inv_scale) {
+    __shared__ f16x8 piece[4][64];
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int64_t nsteps = (N / 32) * N;  // 32-entry steps of the whole matrix, in memory order
+    for (int64_t s0 = (static_cast<int64_t>(blockIdx.x) * 4 + w) * 4; s0 < nsteps;
+         s0 += static_cast<int64_t>(gridDim.x) * 16) {
+        const int ns = static_cast<int>(min<int64_t>(4, nsteps - s0));
+        float2* base = H + 32 * s0;
+        if (l < 16 * ns) piece[w][l] = reinterpret_cast<const f16x8*>(base)[l];
+        wave_lds_sync();
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int o = l + 64 * h;  // entry o of the four steps: step o / 32, row o % 32
+            const int st = o >> 5, row = o & 31;
+            if (st < ns) {
+                // row = 8 (e >> 1) + 2 g + (e & 1)
+                const int g = (row >> 1) & 3, e = 2 * (row >> 3) + (row & 1);
+                const f16x8* p = piece[w] + 16 * st;
+                const float a = static_cast<float>(p[g][e]) + static_cast<float>(p[4 + g][e]);
+                const float b = static_cast<float>(p[8 + g][e]) + static_cast<float>(p[12 + g][e]);
+                base[o] = make_float2(a * inv_scale, b * inv_scale);
+            }
+        }
+        wave_lds_sync();
     }
 }
 
@@ -613,6 +879,11 @@ void clear_upper_heads(Context& c) {
 
 void complete_matrix32(Context& c) {
     if (!c.built32 || !c.h32_lower) return;
+    if (c.h32_presplit) {
+        hipLaunchKernelGGL(unsplit_kernel, dim3(2048), dim3(256), 0, c.stream, c.d_H32.p, c.N, 1.0f / c.h16_scale);
+        XATU_HIP(hipGetLastError());
+        c.h32_presplit = false;
+    }
     const unsigned t = static_cast<unsigned>((c.N + 63) / 64);
     hipLaunchKernelGGL(mirror_upper_kernel, dim3(t, t), dim3(256), 0, c.stream, c.d_H32.p, c.N);
     XATU_HIP(hipGetLastError());
@@ -656,8 +927,13 @@ void launch_matvec_split16(Context& c, const double2* X, double2* Y, int nvec, h
                            c.s_xinv.p);
         hipLaunchKernelGGL(stage_split_kernel, dim3((nstage + 255) / 256), dim3(256), 0, s, Xb, N, nv, nsteps, ngroups,
                            c.s_xscale.p, xs, xt);
-        hipLaunchKernelGGL((herm_product_kernel<kHermTiles>), dim3(c.herm_items), dim3(256), 0, s, c.d_H32.p, xs, xt,
-                           c.d_herm_items.p, N, sH, c.s_yp32.p, c.s_pt32.p);
+        if (c.h32_presplit) {
+            hipLaunchKernelGGL(herm_product_wave_kernel, dim3(c.herm_items), dim3(256), 0, s, c.d_H32.p, xs, xt,
+                               c.d_herm_items.p, N, c.s_yp32.p, c.s_pt32.p);
+        } else {
+            hipLaunchKernelGGL((herm_product_kernel<kHermTiles, false>), dim3(c.herm_items), dim3(256), 0, s,
+                               c.d_H32.p, xs, xt, c.d_herm_items.p, N, sH, c.s_yp32.p, c.s_pt32.p);
+        }
         hipLaunchKernelGGL(herm_reduce_kernel, dim3((N + 63) / 64), dim3(256), 0, s, c.s_yp32.p, c.s_pt32.p, N, nv, C,
                            c.s_xinv.p, Y + static_cast<int64_t>(v0) * N);
     }
