@@ -39,6 +39,7 @@ DEVICE_SOURCES = [
     "device/band_adjoint.hip",
     "device/convolution.hip",
     "device/split_product.hip",
+    "device/projected_eigen.hip",
     "device/capi.hip",
     "device/davidson.cpp",
 ]

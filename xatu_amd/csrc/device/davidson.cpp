@@ -7,7 +7,8 @@
 // the basis V, its image W = H V, Ritz vectors and residuals live in HBM in FP64;
 // products H V use the FP64 or the FP32 matrix (or the matrix-free product);
 // the projected matrix V^H W is formed incrementally (Gram kernel, FP64) and
-// diagonalised on the host (Rayleigh-Ritz); corrections use the diagonal
+// diagonalised on the device (Rayleigh-Ritz, projected_eigen.hip; on the host for
+// bases above 128 or blocks above 64); corrections use the diagonal
 // preconditioner r / (theta - diag H), are orthogonalised twice against V
 // (classical Gram-Schmidt, FP64) and orthonormalised by an eigen-decomposition
 // of their Gram matrix, dropping dependent directions; the block products with the
@@ -17,6 +18,10 @@
 // entries (the reference uses the first 2*nstates unit vectors, :19), no dense
 // (M - theta I) is ever formed (:37), and convergence is on residual norms
 // rather than on the change of the eigenvalues (:43).
+// With the device Rayleigh-Ritz an iteration synchronises with the host once: the
+// residual norms, the Ritz values and the correction block's Gram matrix arrive together
+// (the corrections are formed before the convergence test; the block is whitened on the
+// host, a 32 x 32 eigenproblem).
 // The FP32 form runs this loop with FP32 products to a loose tolerance and then
 // refines in FP64: it restarts from the FP32 Ritz block and iterates with FP64
 // products until the FP64 residuals meet the tolerance.
@@ -40,6 +45,12 @@ void launch_precondition(const double2* R, const double* theta, const double* di
                          hipStream_t s);
 void launch_unit_vectors(const int* idx, int64_t N, int nvec, double2* V, hipStream_t s);
 void launch_copy_vectors(const double2* src, int64_t n, double2* dst, hipStream_t s);
+int projected_eigen_max_dim();
+int projected_eigen_max_want();
+void launch_projected_eigen(const double2* T, int ldt, int m, int nwant, double2* refl, double* theta, double2* Y,
+                            int ldy, hipStream_t s);
+void launch_extend_projected(const double2* P, int m0, int nb, double2* T, int ldt, hipStream_t s);
+void launch_restart_projected(double2* T, int ld, const double* theta, int nb, hipStream_t s);
 
 namespace {
 
@@ -63,6 +74,8 @@ struct Workspace {
     DevBuf<double> theta;    // b
     DevBuf<double> partial;  // residual partial sums
     DevBuf<float2> x32, y32;
+    DevBuf<double2> Tm;      // projected matrix (mmax x mmax), device Rayleigh-Ritz
+    DevBuf<double2> refl;    // its Householder reflectors
 };
 
 class Davidson {
@@ -86,6 +99,13 @@ public:
         c_.h_whiten.reserve(static_cast<size_t>(b_) * b_);
         c_.h_theta.reserve(b_);
         c_.h_part.reserve(static_cast<size_t>(b_) * ((N_ + 255) / 256));
+        dev_rr_ = mmax_ <= projected_eigen_max_dim() && b_ <= projected_eigen_max_want();
+        if (dev_rr_) {
+            const int md = projected_eigen_max_dim();
+            ws_.Tm.alloc(static_cast<size_t>(mmax_) * mmax_);
+            ws_.refl.alloc(static_cast<size_t>(md) * md);
+            XATU_HIP(hipMemsetAsync(ws_.Tm.p, 0, ws_.Tm.bytes(), c_.stream));
+        }
     }
 
     // Start from the unit vectors on the b smallest diagonal entries.
@@ -117,6 +137,7 @@ public:
 
     // Iterate to convergence; returns true if converged.
     bool run(double tol, int maxit) {
+        if (dev_rr_) return run_device(tol, maxit);
         for (int it = 0; it < maxit; ++it) {
             rayleigh_ritz();
             ritz_and_residuals();
@@ -144,10 +165,113 @@ public:
     double projections_ms() const { return proj_ms_; }
 
 private:
+    // Event pairs read after the next synchronisation (reading them adds no wait).
+    struct Pending {
+        std::unique_ptr<EventTimer> ev;
+        int kind;  // 0 product (nvec), 1 projection, 2 Rayleigh-Ritz
+        int nvec;
+    };
+    EventTimer* timer(int kind, int nvec = 0) {
+        pending_.push_back({std::make_unique<EventTimer>(c_.stream), kind, nvec});
+        return pending_.back().ev.get();
+    }
+    void drain_timers() {
+        for (Pending& p : pending_) {
+            const double ms = p.ev->ms();
+            if (p.kind == 0) {
+                prod_ms_ += ms;
+                record_product(c_, mf_, ms, p.nvec);
+            } else if (p.kind == 1) {
+                proj_ms_ += ms;
+            } else {
+                rr_ms_ += ms;
+            }
+        }
+        pending_.clear();
+    }
+
+    // One iteration per synchronisation: Rayleigh-Ritz on the device, Ritz block and
+    // residuals, restart when the basis is full, preconditioned corrections and their
+    // Gram matrix against the basis; then residual norms, Ritz values and the Gram matrix
+    // to the host together.
+    bool run_device(double tol, int maxit) {
+        const int n = static_cast<int>(N_);
+        for (int it = 0; it < maxit; ++it) {
+            EventTimer* er = timer(2);
+            er->start();
+            launch_projected_eigen(ws_.Tm.p, mmax_, m_, b_, ws_.refl.p, ws_.theta.p, ws_.Y.p, m_, c_.stream);
+            er->stop();
+            EventTimer* ep = timer(1);
+            ep->start();
+            launch_block_gemm(ws_.V.p, n, ws_.Y.p, m_, N_, m_, b_, 1.0, 0.0, ws_.X.p, n, c_.stream);
+            launch_block_gemm(ws_.W.p, n, ws_.Y.p, m_, N_, m_, b_, 1.0, 0.0, ws_.AX.p, n, c_.stream);
+            ep->stop();
+            int nblk = 0;
+            launch_residual(ws_.X.p, ws_.AX.p, ws_.theta.p, N_, b_, ws_.R.p, ws_.partial.p, &nblk, c_.stream);
+            rep_->iterations++;
+            if (m_ + b_ > mmax_) {
+                launch_copy_vectors(ws_.X.p, static_cast<int64_t>(N_) * b_, ws_.V.p, c_.stream);
+                launch_copy_vectors(ws_.AX.p, static_cast<int64_t>(N_) * b_, ws_.W.p, c_.stream);
+                launch_restart_projected(ws_.Tm.p, mmax_, ws_.theta.p, b_, c_.stream);
+                m_ = b_;
+                rep_->restarts++;
+            }
+            const int nb = std::min(b_, mmax_ - m_);
+            if (nb > 0) {
+                launch_precondition(ws_.R.p, ws_.theta.p, c_.d_diag.p, N_, nb, ws_.V.p + static_cast<size_t>(m_) * N_,
+                                    c_.stream);
+                ortho_gram(nb);
+                XATU_HIP(hipMemcpyAsync(c_.h_proj.p, ws_.P.p, static_cast<size_t>(m_ + nb) * nb * sizeof(cd),
+                                        hipMemcpyDeviceToHost, c_.stream));
+            }
+            XATU_HIP(hipMemcpyAsync(c_.h_part.p, ws_.partial.p, static_cast<size_t>(nblk) * b_ * sizeof(double),
+                                    hipMemcpyDeviceToHost, c_.stream));
+            XATU_HIP(hipMemcpyAsync(c_.h_theta.p, ws_.theta.p, b_ * sizeof(double), hipMemcpyDeviceToHost, c_.stream));
+            XATU_HIP(hipStreamSynchronize(c_.stream));
+            drain_timers();
+            theta_.assign(c_.h_theta.p, c_.h_theta.p + b_);
+            residual_norms(nblk);
+            double worst = 0.0;
+            for (int i = 0; i < nev_; ++i) worst = std::max(worst, res_[i]);
+            last_worst_ = worst;
+            if (worst < tol) return true;
+            if (nb <= 0) return false;
+            const int kept = ortho_whiten(nb, 0);
+            if (kept == 0) return false;
+            append_block(kept);
+        }
+        return false;
+    }
+
+    void residual_norms(int nblk) {
+        const double* part = c_.h_part.p;
+        res_.assign(b_, 0.0);
+        for (int j = 0; j < b_; ++j) {
+            double s = 0.0;
+            for (int i = 0; i < nblk; ++i) s += part[i + static_cast<size_t>(j) * nblk];
+            res_[j] = std::sqrt(s);
+        }
+    }
+
     // W[:, m:m+nb] = H V[:, m:m+nb], then extend the projected matrix.
     void append_block(int nb) {
         double2* Vn = ws_.V.p + static_cast<size_t>(m_) * N_;
         double2* Wn = ws_.W.p + static_cast<size_t>(m_) * N_;
+        if (dev_rr_) {
+            EventTimer* ev = timer(0, nb);
+            ev->start();
+            launch_operator(c_, prec_, mf_, Vn, Wn, nb);
+            ev->stop();
+            rep_->products += nb;
+            const int mn = m_ + nb;
+            EventTimer* eg = timer(1);
+            eg->start();
+            launch_gram(c_, ws_.V.p, static_cast<int>(N_), Wn, static_cast<int>(N_), mn, nb, ws_.P.p, mn, c_.stream);
+            eg->stop();
+            launch_extend_projected(ws_.P.p, m_, nb, ws_.Tm.p, mmax_, c_.stream);
+            m_ = mn;
+            return;
+        }
         // event pairs are read after the synchronisation below, so that timing adds no waits
         EventTimer evp(c_.stream), evg(c_.stream);
         evp.start();
@@ -211,17 +335,11 @@ private:
         ev.stop();
         int nblk = 0;
         launch_residual(ws_.X.p, ws_.AX.p, ws_.theta.p, N_, b_, ws_.R.p, ws_.partial.p, &nblk, c_.stream);
-        const double* part = c_.h_part.p;
         XATU_HIP(hipMemcpyAsync(c_.h_part.p, ws_.partial.p, static_cast<size_t>(nblk) * b_ * sizeof(double),
                                 hipMemcpyDeviceToHost, c_.stream));
         XATU_HIP(hipStreamSynchronize(c_.stream));
         proj_ms_ += ev.ms();
-        res_.assign(b_, 0.0);
-        for (int j = 0; j < b_; ++j) {
-            double s = 0.0;
-            for (int i = 0; i < nblk; ++i) s += part[i + static_cast<size_t>(j) * nblk];
-            res_[j] = std::sqrt(s);
-        }
+        residual_norms(nblk);
     }
 
     void restart() {
@@ -242,23 +360,36 @@ private:
     // dropping dependent directions. An ill-conditioned block (kept eigenvalues spanning
     // more than 1e8) repeats the whole procedure once on the whitened vectors.
     int orthonormalise_slot(int nb, int round = 0) {
+        ortho_gram(nb);
+        XATU_HIP(hipMemcpyAsync(c_.h_proj.p, ws_.P.p, static_cast<size_t>(m_ + nb) * nb * sizeof(cd),
+                                hipMemcpyDeviceToHost, c_.stream));
+        XATU_HIP(hipStreamSynchronize(c_.stream));
+        drain_timers();
+        return ortho_whiten(nb, round);
+    }
+
+    // The device part: two classical Gram-Schmidt passes; P = [V, T]^H T of the second.
+    void ortho_gram(int nb) {
         const int n = static_cast<int>(N_);
         const int mn = m_ + nb;
         double2* T = ws_.V.p + static_cast<size_t>(m_) * N_;
-        EventTimer ev(c_.stream);
-        ev.start();
+        EventTimer* ev = timer(1);
+        ev->start();
         for (int pass = 0; pass < 2; ++pass) {
             launch_gram(c_, ws_.V.p, n, T, n, mn, nb, ws_.P.p, mn, c_.stream);
             if (m_ > 0) {
                 launch_block_gemm(ws_.V.p, n, ws_.P.p, mn, N_, m_, nb, -1.0, 1.0, T, n, c_.stream);
             }
         }
-        ev.stop();
+        ev->stop();
+    }
+
+    // The host part, with P on the host: whiten the block (its Gram matrix rows m..m+nb).
+    int ortho_whiten(int nb, int round) {
+        const int n = static_cast<int>(N_);
+        const int mn = m_ + nb;
+        double2* T = ws_.V.p + static_cast<size_t>(m_) * N_;
         const cd* P = reinterpret_cast<const cd*>(c_.h_proj.p);
-        XATU_HIP(hipMemcpyAsync(c_.h_proj.p, ws_.P.p, static_cast<size_t>(mn) * nb * sizeof(cd), hipMemcpyDeviceToHost,
-                                c_.stream));
-        XATU_HIP(hipStreamSynchronize(c_.stream));
-        proj_ms_ += ev.ms();
         // column-normalised Gram matrix of the block
         std::vector<cd> G(static_cast<size_t>(nb) * nb);
         std::vector<double> dsc(nb);
@@ -306,6 +437,8 @@ private:
 
     Context& c_;
     int prec_, mf_, nev_, b_, mmax_;
+    bool dev_rr_ = false;
+    std::vector<Pending> pending_;
     xatu_solve_report* rep_;
     int64_t N_ = 0;
     int m_ = 0;
