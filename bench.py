@@ -60,6 +60,20 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def cpu_model():
+    """The host CPU's model name (first processor in /proc/cpuinfo) and its logical CPU count."""
+    name = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    name = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return f"{name} ({os.cpu_count()} logical CPUs on the host)"
+
+
 def cpu_baseline(model_path, cfg, n_products_fp64, nstates, block):
     """Time the CPU oracle (the reference algorithm restated in C++, OpenMP) on a
     bounded sample of the same workload and scale it to one full step."""
@@ -95,6 +109,7 @@ def cpu_baseline(model_path, cfg, n_products_fp64, nstates, block):
     total = t_ft + t_bse + t_solve
     return {
         "value": round(total, 3), "unit": "s per build+solve (extrapolated)", "cores": threads, "kind": "port",
+        "cpu_model": cpu_model(),
         "sample": (f"oracle at a {ns}x{ns} mesh of the same model (N={ex.N}): lattice sums {t_ft_sample:.2f}s, "
                    f"per-element assembly {t_bse_sample:.2f}s, one {block}-vector FP64 product {t_mv_sample*1e3:.1f}ms; "
                    f"scaled to N={N_full} (assembly and products x(N/Ns)^2, sums x mesh x cells) with "
@@ -256,6 +271,22 @@ def main():
             out["mixed_fp32mfma_eig_dev_meV_max"] = float(np.abs((q["w"] - results["fp64"]["w"]) * 1e3).max())
     rep = mixed["rep"]
     tim = mixed["tim"]
+    # throughput figures of the headline step: converged eigenpairs per second of the whole
+    # step; vector products H x per second (all of the solve's products over their kernel
+    # time, FP32 phase and FP64 refinement); lattice Fourier sums as algorithmic bytes
+    # (8 natoms^2 nR of V_ab(R) + 16 natoms^2 nMesh of sums + 24 nR of cell vectors) per
+    # second of their kernels
+    out["eigenpairs_per_s"] = round(nstates / (ms_step / 1e3), 2)
+    if rep.ms_products > 0:
+        out["matvecs_per_s"] = round(rep.products / (rep.ms_products / 1e3), 1)
+    n_r = len(ctx.get_cells())
+    ft_bytes = 8.0 * natoms ** 2 * n_r + 16.0 * natoms ** 2 * n * n + 24.0 * n_r
+    if mixed["build"].ms_fourier > 0:
+        out["fourier_GBs"] = round(ft_bytes / (mixed["build"].ms_fourier / 1e3) / 1e9, 2)
+        out["fourier_bytes"] = ft_bytes
+    if "fp64" in results and "mixed_fp32mfma" in results:
+        out["speedup_fp32mfma_vs_fp64"] = round(float(np.median(results["fp64"]["times"])) /
+                                                float(np.median(results["mixed_fp32mfma"]["times"])), 3)
     out["mixed_parts_ms"] = {"fourier": round(mixed["build"].ms_fourier, 3),
                              "build": round(mixed["build"].ms_build, 3), "solve": round(rep.ms_total, 2),
                              "products": round(rep.ms_products, 2), "rayleigh_ritz": round(rep.ms_rayleigh_ritz, 2),

@@ -112,6 +112,22 @@ def test_davidson_fp64(name, nstates, request):
     ctx.close()
 
 
+# The Rayleigh-Ritz step runs on the device for blocks up to 32 and bases up to 128, on the
+# host otherwise; both, at several basis sizes (restarts), against the oracle's eigenvalues.
+@pytest.mark.parametrize("nstates,block,max_basis", [(16, 32, 96), (20, 40, 120), (6, 12, 128), (16, 32, 160)])
+def test_davidson_rayleigh_ritz_paths(nstates, block, max_basis, request):
+    ex, H = get_case("random18", request)
+    ctx = gpu_context(ex)
+    ctx.build(0, scissor=ex.scissor)
+    w, v, rep = ctx.solve(method=1, nstates=nstates, precision=0, block=block, max_basis=max_basis, tol=1e-7)
+    wr = np.linalg.eigvalsh(H)[:nstates]
+    assert rep.converged == 1
+    assert np.abs(w - wr).max() < 1e-6
+    assert np.abs(H @ v - v * w).max() < 1e-6
+    assert np.abs(v.conj().T @ v - np.eye(nstates)).max() < 1e-8
+    ctx.close()
+
+
 @pytest.mark.parametrize("arith", [0, 1])  # XATU_ARITH_SPLIT16 (FP16 matrix cores, split operands), XATU_ARITH_FP32
 @pytest.mark.parametrize("name,nstates", [("hbn30_cut20_x", 8), ("mos2_q_x", 4), ("random18", 16), ("random36", 8)])
 def test_fp32_build_and_mixed_solve(name, nstates, arith, request):
