@@ -165,6 +165,9 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the FP64 plain path")
     ap.add_argument("--max-basis", type=int, default=0, help="Davidson basis size before restart (0: library default)")
     ap.add_argument("--block", type=int, default=0, help="Davidson block size (0: the workload's)")
+    ap.add_argument("--time-tiles", action="store_true",
+                    help="matrix-free workloads: also time one 32-vector product of the recomputed-tiles "
+                         "operator (FP32 tiles) against the convolution operator on the same vectors")
     args = ap.parse_args()
     # exactly one JSON line on stdout: everything else (including what the native library
     # prints) goes to stderr
@@ -322,6 +325,18 @@ def main():
                                "kernel": "FP32 dense product H X (N x N by N x 32)",
                                "hbm_GBs": round(8.0 * N * N / t / 1e9, 1), "algorithmic_bytes": 8.0 * N * N,
                                **common}
+    if args.time_tiles and mf:
+        X = np.asfortranarray(np.random.default_rng(1).normal(size=(N, 32)) + 0j)
+        forms_mf = (("conv", capi.XATU_MATRIX_FREE_CONV), ("tiles", capi.XATU_MATRIX_FREE_TILES))
+        got = {}
+        for name, form in forms_mf:
+            log(f"one 32-vector product, matrix-free form {name}")
+            t0 = time.perf_counter()
+            got[name] = ctx.matvec(X, precision=capi.XATU_FP32, matrix_free=form)
+            got[name + "_s"] = time.perf_counter() - t0
+        out["matrix_free_product_s"] = {"convolution": round(got["conv_s"], 4), "tiles": round(got["tiles_s"], 3),
+                                        "vectors": 32, "rel_diff": float(np.abs(got["conv"] - got["tiles"]).max() /
+                                                                         np.abs(got["tiles"]).max())}
     if args.config in ("c3", "c5"):
         ctx.close()
         out["extras"] = extras(ex, args.config, nstates, mf)
