@@ -468,7 +468,7 @@ __global__ __launch_bounds__(256, 1) void herm_product_kernel(const float2* __re
 struct Herm2Lds {
     union {
         struct {
-            XFrag16 xi[4][2][kVec][4];       // [group][jj][n][g]
+            f16x8 xi[4][4][2][4][kVec];      // [part ch, cl, dh, dl][group][jj][g][n]: 16 lanes read 256 B
             _Float16 tr[4][4][32 * kTrRow];  // per wave, per part: [rho][pair][li][tile & 1]
         } loop;
         float2 sum[2][kHermRows][kVec];      // end of item: direct sums of waves 0 + 2, 1 + 3
@@ -499,8 +499,14 @@ __global__ __launch_bounds__(256, 1) void herm_product_wave_kernel(const float2*
     // vectors of the row block in the i order of the transposed fragments
     {
         const XFrag16* src = Xt + static_cast<int64_t>(b) * 4 * 2 * kVec * 4;
-        XFrag16* dst = &lds.loop.xi[0][0][0][0];
-        for (int e = threadIdx.x; e < 4 * 2 * kVec * 4; e += 256) dst[e] = src[e];
+        for (int e = threadIdx.x; e < 4 * 2 * kVec * 4; e += 256) {  // src order [group][jj][n][g]
+            const XFrag16 x = src[e];
+            const int g4 = e & 3, n = (e >> 2) % kVec, jj = (e / (4 * kVec)) & 1, G = e / (8 * kVec);
+            lds.loop.xi[0][G][jj][g4][n] = x.ch;
+            lds.loop.xi[1][G][jj][g4][n] = x.cl;
+            lds.loop.xi[2][G][jj][g4][n] = x.dh;
+            lds.loop.xi[3][G][jj][g4][n] = x.dl;
+        }
     }
     __syncthreads();
 
@@ -587,7 +593,11 @@ __global__ __launch_bounds__(256, 1) void herm_product_wave_kernel(const float2*
                 const f16x8 nbh = -bh, nbl = -bl;
 #pragma unroll
                 for (int u = 0; u < 2; ++u) {
-                    const XFrag16 x = lds.loop.xi[G >> 1][G & 1][16 * u + li][g];
+                    XFrag16 x;
+                    x.ch = lds.loop.xi[0][G >> 1][G & 1][g][16 * u + li];
+                    x.cl = lds.loop.xi[1][G >> 1][G & 1][g][16 * u + li];
+                    x.dh = lds.loop.xi[2][G >> 1][G & 1][g][16 * u + li];
+                    x.dl = lds.loop.xi[3][G >> 1][G & 1][g][16 * u + li];
                     f32x4 re = acct[h][u][0], im = acct[h][u][1];
                     re = mma_f16(al, x.ch, re);
                     re = mma_f16(ah, x.cl, re);
