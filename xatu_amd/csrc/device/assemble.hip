@@ -196,9 +196,22 @@ __device__ inline double diag_value(const double* __restrict__ eval_k, const dou
     return h + h;
 }
 
+// LDS layout of the staged tile: column c at c * tile_pitch, its row r at position
+// (r mod NCV) TK + r / NCV, so that the threads of a wave (consecutive pair rows tr, then
+// pair columns tc) write consecutive slots for each element of their blocks; the pitch is
+// the smallest >= ROWS that moves the next pair column by one run of TK slots modulo the
+// 64 LDS banks (2 TK banks for 8-byte entries), which spreads a wave's 64 writes evenly.
+constexpr int tile_pitch(int tk, int ncv) {
+    const int rows = tk * ncv;
+    for (int l = rows; l < rows + 64; ++l)
+        if ((2 * l * ncv - 2 * tk) % 64 == 0) return l;
+    return rows + 1;
+}
+__host__ __device__ constexpr int tile_slot(int r, int tk, int ncv) { return (r % ncv) * tk + r / ncv; }
+
 // Structured build kernel: tile of TK x TK k-point pairs, one pair per thread. With
-// STAGE the (TK NCV)^2 tile of H is assembled in LDS (column-major, one element of
-// padding per column) and then written column by column, consecutive threads on
+// STAGE the (TK NCV)^2 tile of H is assembled in LDS (column-major, rows permuted as
+// tile_slot gives) and then written column by column, consecutive threads on
 // consecutive rows, so that every store instruction writes whole contiguous column
 // segments; without it (the FP64 build, whose tile would cost occupancy) each pair
 // stores its block directly.
@@ -218,7 +231,7 @@ __global__ __launch_bounds__(TK* TK) void bse_tile_kernel(LatticeDesc lat, int n
     constexpr int NB = NC + NV;
     constexpr int NCV = NC * NV;
     constexpr int ROWS = TK * NCV;
-    constexpr int LDT = ROWS + 1;
+    constexpr int LDT = tile_pitch(TK, NCV);
     extern __shared__ unsigned char smem_raw[];
     const int norb = lat.norb;
     C* s_coef = reinterpret_cast<C*>(smem_raw);
@@ -250,7 +263,7 @@ __global__ __launch_bounds__(TK* TK) void bse_tile_kernel(LatticeDesc lat, int n
         const int64_t g0 = static_cast<int64_t>(Kr) * ROWS, h0 = static_cast<int64_t>(Kc) * ROWS;
         auto put = [&](int r, int c, C v) {
             if (STAGE) {
-                s_tile[c * LDT + r] = v;
+                s_tile[c * LDT + tile_slot(r, TK, NCV)] = v;
             } else {
                 H[(g0 + r) + (h0 + c) * N] = v;
             }
@@ -296,7 +309,7 @@ __global__ __launch_bounds__(TK* TK) void bse_tile_kernel(LatticeDesc lat, int n
 #pragma unroll
                 for (int el = 0; el < 8; ++el) {
                     const int r = 32 * step + 8 * (el >> 1) + 2 * g + (el & 1);
-                    const C v = s_tile[c * LDT + r];
+                    const C v = s_tile[c * LDT + tile_slot(r, TK, NCV)];
                     const float x = static_cast<float>((q < 2) ? v.x : v.y) * hsplit;
                     const _Float16 hi = static_cast<_Float16>(x);
                     out[el] = (q & 1) ? static_cast<_Float16>(x - static_cast<float>(hi)) : hi;
@@ -307,7 +320,7 @@ __global__ __launch_bounds__(TK* TK) void bse_tile_kernel(LatticeDesc lat, int n
         } else {
             for (int e = t; e < ROWS * ROWS; e += TK * TK) {
                 const int r = e % ROWS, c = e / ROWS;
-                if (row0 + r < N && col0 + c < N) H[(row0 + r) + (col0 + c) * N] = s_tile[c * LDT + r];
+                if (row0 + r < N && col0 + c < N) H[(row0 + r) + (col0 + c) * N] = s_tile[c * LDT + tile_slot(r, TK, NCV)];
             }
         }
     }
@@ -802,7 +815,7 @@ size_t coef_bytes(const Context& c, int nb) {
 template <typename R, int TK>
 size_t tile_bytes(const Context& c, int nb, int ncv) {
     const size_t rows = static_cast<size_t>(TK) * ncv;
-    return coef_bytes<R, TK>(c, nb) + (kStageTile<R> ? rows * (rows + 1) * sizeof(cx_t<R>) : 0);
+    return coef_bytes<R, TK>(c, nb) + (kStageTile<R> ? rows * static_cast<size_t>(tile_pitch(TK, ncv)) * sizeof(cx_t<R>) : 0);
 }
 
 // Launchers for one (natoms, nc, nv) shape.

@@ -692,11 +692,12 @@ __global__ __launch_bounds__(256, 1) void herm_product_wave_kernel(const float2*
 // Y[r][n] (column-major, FP64) = scale[n] (sum over the items' direct partial sums of r and
 // the transposed partial sums of r from the row blocks above it); 64 rows per workgroup
 // (all in one row block), fixed summation order.
+constexpr int kReduceRows = 8;  // rows per workgroup of the reduction (256 threads = 8 rows x 32 vectors)
 __global__ __launch_bounds__(256) void herm_reduce_kernel(const float2* __restrict__ Pn,
                                                           const float2* __restrict__ Pt, int64_t N, int nvec,
                                                           int nchunks, const double* __restrict__ scale,
                                                           double2* __restrict__ Y) {
-    constexpr int kRows = 64;
+    constexpr int kRows = kReduceRows;  // one output (row, vector) per thread
     __shared__ double2 tile[kRows][kVec + 1];
     // the last rows (the most transposed-block slots) first
     const int64_t r0 = static_cast<int64_t>(gridDim.x - 1 - blockIdx.x) * kRows;
@@ -944,7 +945,7 @@ void launch_matvec_split16(Context& c, const double2* X, double2* Y, int nvec, h
             hipLaunchKernelGGL((herm_product_kernel<kHermTiles, false>), dim3(c.herm_items), dim3(256), 0, s,
                                c.d_H32.p, xs, xt, c.d_herm_items.p, N, sH, c.s_yp32.p, c.s_pt32.p);
         }
-        hipLaunchKernelGGL(herm_reduce_kernel, dim3((N + 63) / 64), dim3(256), 0, s, c.s_yp32.p, c.s_pt32.p, N, nv, C,
+        hipLaunchKernelGGL(herm_reduce_kernel, dim3((N + kReduceRows - 1) / kReduceRows), dim3(256), 0, s, c.s_yp32.p, c.s_pt32.p, N, nv, C,
                            c.s_xinv.p, Y + static_cast<int64_t>(v0) * N);
     }
     XATU_HIP(hipGetLastError());
