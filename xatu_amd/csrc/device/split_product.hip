@@ -465,6 +465,10 @@ __global__ __launch_bounds__(256, 1) void herm_product_kernel(const float2* __re
 // of the row block in the transposed-fragment order are staged in LDS once per item; each
 // wave transposes its split parts through its own LDS buffer, one group at a time. Two
 // register sets of two tiles alternate, so one group is loaded while the other is used.
+// Vector views of the transposition buffer that may alias each other (the writes and the
+// reads of one wave rely on their program order, with no fence in between).
+typedef _Float16 f16x2_alias __attribute__((ext_vector_type(2), may_alias));
+typedef _Float16 f16x8_alias __attribute__((ext_vector_type(8), may_alias));
 struct Herm2Lds {
     union {
         struct {
@@ -570,26 +574,26 @@ __global__ __launch_bounds__(256, 1) void herm_product_wave_kernel(const float2*
             }
         }
         if (!low) return;
-        wave_lds_sync();  // the previous group's reads of the buffer are done
+        // the wave's LDS operations execute in order and these writes and reads address the
+        // same buffer, so no fence separates them (the MFMAs around them may interleave)
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
                 const int rho = 8 * (e >> 1) + 4 * (e & 1) + g;
                 _Float16* dst = trw + rho * kTrRow + 2 * li;
                 const HFrag16& h0 = hf[0];
                 const HFrag16& h1 = hf[1];
-                *reinterpret_cast<f16x2*>(dst) = f16x2{h0.ah[e], h1.ah[e]};
-                *reinterpret_cast<f16x2*>(dst + 32 * kTrRow) = f16x2{h0.al[e], h1.al[e]};
-                *reinterpret_cast<f16x2*>(dst + 64 * kTrRow) = f16x2{h0.bh[e], h1.bh[e]};
-                *reinterpret_cast<f16x2*>(dst + 96 * kTrRow) = f16x2{h0.bl[e], h1.bl[e]};
+                *reinterpret_cast<f16x2_alias*>(dst) = f16x2{h0.ah[e], h1.ah[e]};
+                *reinterpret_cast<f16x2_alias*>(dst + 32 * kTrRow) = f16x2{h0.al[e], h1.al[e]};
+                *reinterpret_cast<f16x2_alias*>(dst + 64 * kTrRow) = f16x2{h0.bh[e], h1.bh[e]};
+                *reinterpret_cast<f16x2_alias*>(dst + 96 * kTrRow) = f16x2{h0.bl[e], h1.bl[e]};
             }
-        wave_lds_sync();
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
                 const _Float16* src = trw + (16 * h + li) * kTrRow + 8 * g;
-                const f16x8 ah = *reinterpret_cast<const f16x8*>(src);
-                const f16x8 al = *reinterpret_cast<const f16x8*>(src + 32 * kTrRow);
-                const f16x8 bh = *reinterpret_cast<const f16x8*>(src + 64 * kTrRow);
-                const f16x8 bl = *reinterpret_cast<const f16x8*>(src + 96 * kTrRow);
+                const f16x8 ah = *reinterpret_cast<const f16x8_alias*>(src);
+                const f16x8 al = *reinterpret_cast<const f16x8_alias*>(src + 32 * kTrRow);
+                const f16x8 bh = *reinterpret_cast<const f16x8_alias*>(src + 64 * kTrRow);
+                const f16x8 bl = *reinterpret_cast<const f16x8_alias*>(src + 96 * kTrRow);
                 const f16x8 nbh = -bh, nbl = -bl;
 #pragma unroll
                 for (int u = 0; u < 2; ++u) {
