@@ -399,6 +399,48 @@ private:
         }
         for (int j = 0; j < nb; ++j)
             for (int i = 0; i < nb; ++i) G[i + static_cast<size_t>(j) * nb] = P[(m_ + i) + static_cast<size_t>(j) * mn] * dsc[i] * dsc[j];
+        // Well-conditioned blocks (every Cholesky pivot^2 of the unit-diagonal G above 1e-6, so
+        // cond(G) stays below ~1e6): S = D^-1/2 L^-H from G = L L^H, a tenth of the cost of the
+        // eigen-decomposition below, which handles the rest (dependent directions dropped).
+        {
+            std::vector<cd> L(G);
+            bool ok = true;
+            for (int j = 0; j < nb && ok; ++j) {
+                double d = L[j + static_cast<size_t>(j) * nb].real();
+                for (int k = 0; k < j; ++k) d -= std::norm(L[j + static_cast<size_t>(k) * nb]);
+                if (!(d > 1e-6)) {
+                    ok = false;
+                    break;
+                }
+                const double ljj = std::sqrt(d);
+                L[j + static_cast<size_t>(j) * nb] = ljj;
+                for (int i = j + 1; i < nb; ++i) {
+                    cd sum = L[i + static_cast<size_t>(j) * nb];
+                    for (int k = 0; k < j; ++k) sum -= L[i + static_cast<size_t>(k) * nb] * std::conj(L[j + static_cast<size_t>(k) * nb]);
+                    L[i + static_cast<size_t>(j) * nb] = sum / ljj;
+                }
+            }
+            if (ok) {
+                std::vector<cd> Li(static_cast<size_t>(nb) * nb, cd(0.0, 0.0));  // L^-1, lower
+                for (int j = 0; j < nb; ++j) {
+                    Li[j + static_cast<size_t>(j) * nb] = 1.0 / L[j + static_cast<size_t>(j) * nb].real();
+                    for (int i = j + 1; i < nb; ++i) {
+                        cd sum(0.0, 0.0);
+                        for (int k = j; k < i; ++k) sum += L[i + static_cast<size_t>(k) * nb] * Li[k + static_cast<size_t>(j) * nb];
+                        Li[i + static_cast<size_t>(j) * nb] = -sum / L[i + static_cast<size_t>(i) * nb].real();
+                    }
+                }
+                cd* S = reinterpret_cast<cd*>(c_.h_whiten.p);
+                for (int j = 0; j < nb; ++j)
+                    for (int i = 0; i < nb; ++i)
+                        S[i + static_cast<size_t>(j) * nb] = i <= j ? dsc[i] * std::conj(Li[j + static_cast<size_t>(i) * nb]) : cd(0.0, 0.0);
+                XATU_HIP(hipMemcpyAsync(ws_.P.p, c_.h_whiten.p, static_cast<size_t>(nb) * nb * sizeof(cd),
+                                        hipMemcpyHostToDevice, c_.stream));
+                launch_block_gemm(T, n, ws_.P.p, nb, N_, nb, nb, 1.0, 0.0, ws_.Tb.p, n, c_.stream);
+                launch_copy_vectors(ws_.Tb.p, static_cast<int64_t>(N_) * nb, T, c_.stream);
+                return nb;
+            }
+        }
         std::vector<double> lam(nb);
         std::vector<cd> U(static_cast<size_t>(nb) * nb);
         xatu_linalg::heev(nb, G.data(), nb, lam.data(), U.data());
