@@ -47,8 +47,9 @@ void launch_unit_vectors(const int* idx, int64_t N, int nvec, double2* V, hipStr
 void launch_copy_vectors(const double2* src, int64_t n, double2* dst, hipStream_t s);
 int projected_eigen_max_dim();
 int projected_eigen_max_want();
+// fp32_reduction: the Householder reduction in FP32 (the FP32 phase of the mixed form)
 void launch_projected_eigen(const double2* T, int ldt, int m, int nwant, double2* refl, double* theta, double2* Y,
-                            int ldy, hipStream_t s);
+                            int ldy, bool fp32_reduction, hipStream_t s);
 void launch_extend_projected(const double2* P, int m0, int nb, double2* T, int ldt, hipStream_t s);
 void launch_restart_projected(double2* T, int ld, const double* theta, int nb, hipStream_t s);
 
@@ -127,11 +128,14 @@ public:
         XATU_HIP(hipStreamSynchronize(c_.stream));
     }
 
-    // Start from b orthonormal vectors given on the device (N x b): V = X0.
+    // Start from b vectors given on the device (N x b), orthonormalised (they come from a
+    // solve whose Rayleigh-Ritz step may have been FP32): V = X0 S.
     void init_from(const double2* X0) {
         XATU_HIP(hipMemcpyAsync(ws_.V.p, X0, static_cast<size_t>(N_) * b_ * sizeof(double2),
                                 hipMemcpyDeviceToDevice, c_.stream));
         m_ = 0;
+        const int kept = orthonormalise_slot(b_);
+        if (kept < b_) throw Error(XATU_ERR_NOT_CONVERGED, "Davidson: the refinement's starting block is rank deficient");
         append_block(b_);
     }
 
@@ -199,7 +203,8 @@ private:
         for (int it = 0; it < maxit; ++it) {
             EventTimer* er = timer(2);
             er->start();
-            launch_projected_eigen(ws_.Tm.p, mmax_, m_, b_, ws_.refl.p, ws_.theta.p, ws_.Y.p, m_, c_.stream);
+            launch_projected_eigen(ws_.Tm.p, mmax_, m_, b_, ws_.refl.p, ws_.theta.p, ws_.Y.p, m_, prec_ == XATU_FP32,
+                                   c_.stream);
             er->stop();
             EventTimer* ep = timer(1);
             ep->start();

@@ -43,13 +43,14 @@ constexpr int kEWant = 32;       // most eigenpairs
 constexpr int kEPad = kEM + 1;   // padded row of the per-eigenvector arrays
 constexpr int kEStage = 32;      // reflectors per LDS stage of the back-transformation
 
+template <typename TR>
 struct EigLds {
     union {
         struct {
-            double2 v[2][16][kEM];        // per-wave copy of the reflector, by step parity
-            double2 p[2][kEM];            // p of the step, by step parity
-            double2 row[2][kEM];          // row k + 1 before the update of step k, by parity of k + 1
-            double cpart[2][16];          // v^H p over the rows of each wave, by step parity
+            cx_t<TR> v[2][16][kEM];       // per-wave copy of the reflector, by step parity
+            cx_t<TR> p[2][kEM];           // p of the step, by step parity
+            cx_t<TR> row[2][kEM];         // row k + 1 before the update of step k, by parity of k + 1
+            TR cpart[2][16];              // v^H p over the rows of each wave, by step parity
         } tri;
         struct {
             double ui[kEWant][kEPad], u1[kEWant][kEPad], u2[kEWant][kEPad];  // U rows; ui = 1 / u0
@@ -73,6 +74,17 @@ __device__ inline double dpp(double x) {
     const int hi = __builtin_amdgcn_update_dpp(0, static_cast<int>(b >> 32), kCtrl, kRowMask, 0xF, false);
     return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
 }
+template <int kCtrl, int kRowMask = 0xF>
+__device__ inline float dpp(float x) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), kCtrl, kRowMask, 0xF, false));
+}
+__device__ inline double lane63(double x) {
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readlane(static_cast<int>(b), 63);
+    const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), 63);
+    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
+}
+__device__ inline float lane63(float x) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 63)); }
 constexpr int kDppXor1 = 0xB1;        // quad_perm [1, 0, 3, 2]
 constexpr int kDppXor2 = 0x4E;        // quad_perm [2, 3, 0, 1]
 constexpr int kDppHalfMirror = 0x141; // lane i <-> 7 - i within 8
@@ -81,36 +93,41 @@ constexpr int kDppBcast15 = 0x142;    // lane 15 of a row to the next row
 constexpr int kDppBcast31 = 0x143;    // lane 31 to rows 2 and 3
 
 // Sum over the 8 lanes l ^ 0..7 (every lane of the group receives it).
-__device__ inline double sum8(double x) {
+template <typename T>
+__device__ inline T sum8(T x) {
     x += dpp<kDppXor1>(x);
     x += dpp<kDppXor2>(x);
     x += dpp<kDppHalfMirror>(x);
     return x;
 }
 // Sum over the 16 lanes of a DPP row (every lane of the row receives it).
-__device__ inline double sum16_row(double x) {
+template <typename T>
+__device__ inline T sum16_row(T x) {
     x = sum8(x);
     x += dpp<kDppMirror>(x);
     return x;
 }
 // Sum over the wave of a value that is the same within groups of 8 lanes: one term per group.
-__device__ inline double wave_sum_groups8(double x) {
+template <typename T>
+__device__ inline T wave_sum_groups8(T x) {
     x += dpp<kDppMirror>(x);             // two groups, every lane of a row of 16
     x += dpp<kDppBcast15, 0xA>(x);       // rows 1, 3 += rows 0, 2
     x += dpp<kDppBcast31, 0xC>(x);       // rows 2, 3 += row 1
-    const long long b = __double_as_longlong(x);
-    const int lo = __builtin_amdgcn_readlane(static_cast<int>(b), 63);
-    const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), 63);
-    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
+    return lane63(x);
 }
 // Sum over the wave, the same in every lane.
-__device__ inline double wave_sum(double x) { return wave_sum_groups8(sum8(x)); }
+template <typename T>
+__device__ inline T wave_sum(T x) { return wave_sum_groups8(sum8(x)); }
 
 __device__ inline double2 csc(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+__device__ inline float2 csc(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+__device__ inline float rsq(float x) { return rsqrtf(x); }
+__device__ inline double rsq(double x) { return rsqrt(x); }
 
 // Pairwise sum of 16 values (one term per wave).
-__device__ inline double sum16(const double* a) {
-    double t[8];
+template <typename T>
+__device__ inline T sum16(const T* a) {
+    T t[8];
 #pragma unroll
     for (int i = 0; i < 8; ++i) t[i] = a[i] + a[i + 8];
 #pragma unroll
@@ -134,33 +151,37 @@ __device__ inline double recip(double q) {
     return fma(y, e, y);
 }
 
+// TR: arithmetic of the tridiagonalisation (float or double); the eigenvalues, eigenvectors
+// and back-transformation are FP64.
+template <typename TR>
 __global__ __launch_bounds__(kEThreads) void projected_eigen_kernel(const double2* __restrict__ A, int lda, int m,
-                                                                    int nwant, double2* __restrict__ refl,
+                                                                    int nwant, cx_t<TR>* __restrict__ refl,
                                                                     double* __restrict__ theta,
                                                                     double2* __restrict__ Y, int ldy) {
-    __shared__ EigLds s;
+    using CT = cx_t<TR>;
+    __shared__ EigLds<TR> s;
     const int t = threadIdx.x, w = t >> 6, lane = t & 63;
     const int r = 8 * w + (lane >> 3), cg = lane & 7;
     const int duty = (m - 1) >> 3;  // the wave of row m - 1 stores the reflectors and the tridiagonal
 
     // ---- 1. tridiagonalisation -------------------------------------------------------
-    double2 B[kECols];  // A[r][cg + 8 j], Hermitian from the lower triangle
+    CT B[kECols];  // A[r][cg + 8 j], Hermitian from the lower triangle
 #pragma unroll
     for (int j = 0; j < kECols; ++j) {
         const int c = cg + kEGroups * j;
         double2 a = make_double2(0.0, 0.0);
         if (r < m && c < m) a = (r >= c) ? A[r + static_cast<size_t>(c) * lda] : cconj(A[c + static_cast<size_t>(r) * lda]);
-        if (r == c) a.y = 0.0;
-        B[j] = a;
+        if (r == c) a.y = TR(0);
+        B[j] = mk(static_cast<TR>(a.x), static_cast<TR>(a.y));
     }
     // the state of a step -1 that changes nothing
 #pragma unroll
     for (int b = 0; b < 2; ++b) {
-        s.tri.v[b][w][lane] = make_double2(0.0, 0.0);
-        s.tri.v[b][w][lane + 64] = make_double2(0.0, 0.0);
+        s.tri.v[b][w][lane] = mk(TR(0), TR(0));
+        s.tri.v[b][w][lane + 64] = mk(TR(0), TR(0));
     }
-    if (t < 2 * kEM) s.tri.p[t >> 7][t & (kEM - 1)] = make_double2(0.0, 0.0);
-    if (t < 32) s.tri.cpart[t >> 4][t & 15] = 0.0;
+    if (t < 2 * kEM) s.tri.p[t >> 7][t & (kEM - 1)] = mk(TR(0), TR(0));
+    if (t < 32) s.tri.cpart[t >> 4][t & 15] = TR(0);
     if (r == 0) {
 #pragma unroll
         for (int j = 0; j < kECols; ++j) s.tri.row[0][cg + kEGroups * j] = B[j];
@@ -173,41 +194,41 @@ __global__ __launch_bounds__(kEThreads) void projected_eigen_kernel(const double
     // row k before that update (row buffer) and v_{k-1}, p_{k-1}
     auto reflector = [&](int k) {
         const int cur = k & 1, prv = cur ^ 1;
-        const double cprev = sum16(s.tri.cpart[prv]);
-        const double2* vp = s.tri.v[prv][w];
-        const double2 vpk = vp[k];
-        const double2 wpk = csub(s.tri.p[prv][k], csc(vpk, cprev));
-        auto row_k = [&](int i) -> double2 {
-            const double2 vpi = vp[i];
-            const double2 wpi = csub(s.tri.p[prv][i], csc(vpi, cprev));
-            double2 a = s.tri.row[cur][i];
-            a = csub(a, csc(cmul(vpk, cconj(wpi)), 2.0));
-            a = csub(a, csc(cmul(wpk, cconj(vpi)), 2.0));
+        const TR cprev = sum16(s.tri.cpart[prv]);
+        const CT* vp = s.tri.v[prv][w];
+        const CT vpk = vp[k];
+        const CT wpk = csub(s.tri.p[prv][k], csc(vpk, cprev));
+        auto row_k = [&](int i) -> CT {
+            const CT vpi = vp[i];
+            const CT wpi = csub(s.tri.p[prv][i], csc(vpi, cprev));
+            CT a = s.tri.row[cur][i];
+            a = csub(a, csc(cmul(vpk, cconj(wpi)), TR(2)));
+            a = csub(a, csc(cmul(wpk, cconj(vpi)), TR(2)));
             return a;
         };
-        const double dk = row_k(k).x;
-        const double2 x0 = cconj(row_k(k + 1));
-        const double2 xa = cconj(row_k(lane)), xb = cconj(row_k(lane + 64));  // x_i = conj(A[k][i])
+        const TR dk = row_k(k).x;
+        const CT x0 = cconj(row_k(k + 1));
+        const CT xa = cconj(row_k(lane)), xb = cconj(row_k(lane + 64));  // x_i = conj(A[k][i])
         const bool ina = lane > k + 1 && lane < m, inb = lane + 64 > k + 1 && lane + 64 < m;
-        double tl = 0.0;
+        TR tl = TR(0);
         if (ina) tl += xa.x * xa.x + xa.y * xa.y;
         if (inb) tl += xb.x * xb.x + xb.y * xb.y;
         tl = wave_sum(tl);  // |x|^2 without x_0
-        const double a0 = x0.x * x0.x + x0.y * x0.y;
-        const double xn2 = tl + a0;
-        double2 va = make_double2(0.0, 0.0), vb = make_double2(0.0, 0.0), ek = x0;
-        if (!(xn2 == 0.0 || tl <= 0.0)) {
+        const TR a0 = x0.x * x0.x + x0.y * x0.y;
+        const TR xn2 = tl + a0;
+        CT va = mk(TR(0), TR(0)), vb = mk(TR(0), TR(0)), ek = x0;
+        if (!(xn2 == TR(0) || tl <= TR(0))) {
             // alpha = -x0 |x| / |x0| (= -|x| for x0 = 0), u0 = x0 - alpha
-            double2 alpha, u0;
-            if (a0 > 0.0) {
-                const double f = sqrt(xn2 / a0);
+            CT alpha, u0;
+            if (a0 > TR(0)) {
+                const TR f = sqrt(xn2 / a0);
                 alpha = csc(x0, -f);
-                u0 = csc(x0, 1.0 + f);
+                u0 = csc(x0, TR(1) + f);
             } else {
-                alpha = make_double2(-sqrt(xn2), 0.0);
-                u0 = make_double2(-alpha.x, 0.0);
+                alpha = mk(-sqrt(xn2), TR(0));
+                u0 = mk(-alpha.x, TR(0));
             }
-            const double inv = rsqrt(tl + u0.x * u0.x + u0.y * u0.y);
+            const TR inv = rsq(tl + u0.x * u0.x + u0.y * u0.y);
             if (ina) va = csc(xa, inv);
             if (inb) vb = csc(xb, inv);
             if (lane == k + 1) va = csc(u0, inv);
@@ -221,7 +242,7 @@ __global__ __launch_bounds__(kEThreads) void projected_eigen_kernel(const double
             refl[static_cast<size_t>(k) * kEM + lane + 64] = vb;
             if (lane == 0) {
                 s.d[k] = dk;
-                s.ec[k] = ek;
+                s.ec[k] = make_double2(ek.x, ek.y);
             }
         }
     };
@@ -229,20 +250,20 @@ __global__ __launch_bounds__(kEThreads) void projected_eigen_kernel(const double
     auto update = [&](int k) {
         const int cur = k & 1;
         const int jlo = (k + 1) >> 3;  // columns cg + 8 j, j < jlo, are all <= k
-        const double cc = sum16(s.tri.cpart[cur]);
-        const double2* vk = s.tri.v[cur][w];
-        const double2 vr = vk[r];
-        const double2 wr = csub(s.tri.p[cur][r], csc(vr, cc));
-        const double2 v2 = csc(vr, 2.0), w2 = csc(wr, 2.0);
+        const TR cc = sum16(s.tri.cpart[cur]);
+        const CT* vk = s.tri.v[cur][w];
+        const CT vr = vk[r];
+        const CT wr = csub(s.tri.p[cur][r], csc(vr, cc));
+        const CT v2 = csc(vr, TR(2)), w2 = csc(wr, TR(2));
 #pragma unroll
         for (int j = 0; j < kECols; ++j) {
             const int c = cg + kEGroups * j;
             if (j >= jlo && j < jhi && c > k) {
-                const double2 vc = vk[c];
-                const double2 pc = s.tri.p[cur][c];
-                const double2 wc = make_double2(fma(-cc, vc.x, pc.x), fma(-cc, vc.y, pc.y));
+                const CT vc = vk[c];
+                const CT pc = s.tri.p[cur][c];
+                const CT wc = mk(fma(-cc, vc.x, pc.x), fma(-cc, vc.y, pc.y));
                 // a -= 2 v_r conj(w_c) + 2 w_r conj(v_c)
-                double2 a = B[j];
+                CT a = B[j];
                 a.x = fma(-v2.x, wc.x, a.x);
                 a.x = fma(-v2.y, wc.y, a.x);
                 a.x = fma(-w2.x, vc.x, a.x);
@@ -251,7 +272,7 @@ __global__ __launch_bounds__(kEThreads) void projected_eigen_kernel(const double
                 a.y = fma(-v2.y, wc.x, a.y);
                 a.y = fma(w2.x, vc.y, a.y);
                 a.y = fma(-w2.y, vc.x, a.y);
-                if (r == c) a.y = 0.0;
+                if (r == c) a.y = TR(0);
                 B[j] = a;
             }
         }
@@ -271,8 +292,8 @@ __global__ __launch_bounds__(kEThreads) void projected_eigen_kernel(const double
                 for (int j = 0; j < kECols; ++j) s.tri.row[(k + 1) & 1][cg + kEGroups * j] = B[j];
             }
             // p = A v_k for the rows of this wave: over the thread's columns, then the row's 8 lanes
-            const double2* vk = s.tri.v[cur][w];
-            double2 acc = make_double2(0.0, 0.0), acc2 = make_double2(0.0, 0.0);
+            const CT* vk = s.tri.v[cur][w];
+            CT acc = mk(TR(0), TR(0)), acc2 = mk(TR(0), TR(0));
 #pragma unroll
             for (int j = 0; j < kECols; j += 2) {
                 if (j >= jlo && j < jhi) cfma(acc, B[j], vk[cg + kEGroups * j]);
@@ -281,12 +302,12 @@ __global__ __launch_bounds__(kEThreads) void projected_eigen_kernel(const double
             acc.x = sum8(acc.x + acc2.x);
             acc.y = sum8(acc.y + acc2.y);
             if (cg == 0) s.tri.p[cur][r] = acc;
-            const double2 vr = vk[r];
+            const CT vr = vk[r];
             // Re(conj(v_r) p_r), the same in the row's 8 lanes, summed over the wave's rows
-            const double cs = wave_sum_groups8(vr.x * acc.x + vr.y * acc.y);
+            const TR cs = wave_sum_groups8(vr.x * acc.x + vr.y * acc.y);
             if (lane == 0) s.tri.cpart[cur][w] = cs;
         } else if (lane == 0) {
-            s.tri.cpart[cur][w] = 0.0;
+            s.tri.cpart[cur][w] = TR(0);
         }
         lds_barrier();
         const bool next = k + 3 < m && alive_at(k + 1);
@@ -305,7 +326,7 @@ __global__ __launch_bounds__(kEThreads) void projected_eigen_kernel(const double
         const int c = cg + kEGroups * j;
         if (m >= 2) {
             if (r == m - 2 && c == m - 2) s.d[m - 2] = B[j].x;
-            if (r == m - 1 && c == m - 2) s.ec[m - 2] = B[j];
+            if (r == m - 1 && c == m - 2) s.ec[m - 2] = make_double2(B[j].x, B[j].y);
         }
         if (r == m - 1 && c == m - 1) s.d[m - 1] = B[j].x;
     }
@@ -552,7 +573,8 @@ __global__ __launch_bounds__(kEThreads) void projected_eigen_kernel(const double
         __syncthreads();  // the previous stage (or z) is no longer read
         for (int e = t; e < (k1 - k0) * kEM; e += kEThreads) {
             const int kk = e / kEM, i = e % kEM;
-            s.refl[kk][i] = refl[static_cast<size_t>(k0 + kk) * kEM + i];
+            const CT rv = refl[static_cast<size_t>(k0 + kk) * kEM + i];
+            s.refl[kk][i] = make_double2(rv.x, rv.y);
         }
         __syncthreads();
         if (vec < nwant) {
@@ -618,10 +640,16 @@ int projected_eigen_max_dim() { return kEM; }
 int projected_eigen_max_want() { return kEWant; }
 
 void launch_projected_eigen(const double2* T, int ldt, int m, int nwant, double2* refl, double* theta, double2* Y,
-                            int ldy, hipStream_t s) {
+                            int ldy, bool fp32_reduction, hipStream_t s) {
     if (m < 1 || m > kEM || nwant < 1 || nwant > kEWant || nwant > m || ldt < m || ldy < m)
         throw Error(XATU_ERR_INVALID_ARGUMENT, "projected eigensolver: unsupported dimensions");
-    hipLaunchKernelGGL(projected_eigen_kernel, dim3(1), dim3(kEThreads), 0, s, T, ldt, m, nwant, refl, theta, Y, ldy);
+    if (fp32_reduction) {
+        hipLaunchKernelGGL(projected_eigen_kernel<float>, dim3(1), dim3(kEThreads), 0, s, T, ldt, m, nwant,
+                           reinterpret_cast<float2*>(refl), theta, Y, ldy);
+    } else {
+        hipLaunchKernelGGL(projected_eigen_kernel<double>, dim3(1), dim3(kEThreads), 0, s, T, ldt, m, nwant, refl,
+                           theta, Y, ldy);
+    }
     XATU_HIP(hipGetLastError());
 }
 
