@@ -159,6 +159,9 @@ __global__ __launch_bounds__(kEThreads) void projected_eigen_kernel(const double
                                                                     double* __restrict__ theta,
                                                                     double2* __restrict__ Y, int ldy) {
     using CT = cx_t<TR>;
+    // working accuracy of the tridiagonal: the eigenvalues are resolved to it (multisection
+    // tolerance, shift separation, inverse-iteration growth test), not beyond
+    constexpr double kTolEps = sizeof(TR) == 4 ? static_cast<double>(FLT_EPSILON) : DBL_EPSILON;
     __shared__ EigLds<TR> s;
     const int t = threadIdx.x, w = t >> 6, lane = t & 63;
     const int r = 8 * w + (lane >> 3), cg = lane & 7;
@@ -374,7 +377,7 @@ __global__ __launch_bounds__(kEThreads) void projected_eigen_kernel(const double
             const int gshift = lane - gs;  // first lane of the group in the wave
             const unsigned long long gmask = (G == 64) ? ~0ull : ((1ull << G) - 1ull);
             for (int it = 0; it < 64; ++it) {
-                if (hi - lo <= DBL_EPSILON * (2.0 * fmax(fabs(lo), fabs(hi)) + tnorm)) break;
+                if (hi - lo <= kTolEps * (2.0 * fmax(fabs(lo), fabs(hi)) + tnorm)) break;
                 const double step = (hi - lo) / (G + 1);
                 const double x = lo + step * (gs + 1);
                 // Sturm count: eigenvalues below x
@@ -402,7 +405,7 @@ __global__ __launch_bounds__(kEThreads) void projected_eigen_kernel(const double
     if (t == 0) {
         // clusters of close eigenvalues and separated shifts for coincident ones
         const double cluster = 1e-3 * fmax(s.tnorm, 1e-300);
-        const double sep = 10.0 * DBL_EPSILON * s.tnorm;
+        const double sep = 10.0 * kTolEps * s.tnorm;
         int cs = 0;
         for (int i = 0; i < nwant; ++i) {
             if (i > 0 && s.lam[i] - s.lam[i - 1] > cluster) cs = i;
@@ -431,9 +434,10 @@ __global__ __launch_bounds__(kEThreads) void projected_eigen_kernel(const double
             z[i] = 0.5 + (sd >> 8) * (1.0 / 16777216.0);
         }
         double scale = 1.0;  // the previous solve's normalisation, applied on the way in
-        // a solve whose growth |(T - lam I)^-1 z| reaches 1 / (64 sqrt(m) eps |T|) has
-        // converged (the test of LAPACK dstein); one more solve follows it, at most four
-        const double grow = 1.0 / (64.0 * sqrt(static_cast<double>(m)) * eps);
+        // a solve whose growth |(T - lam I)^-1 z| reaches 1 / (64 sqrt(m) eps |T|), eps the
+        // working accuracy, has converged (the test of LAPACK dstein); one more solve follows
+        // it, at most four
+        const double grow = 1.0 / (64.0 * sqrt(static_cast<double>(m)) * kTolEps * fmax(s.tnorm, DBL_MIN));
         int extra = -1;
         for (int it = 0; it < 4 && extra != 0; ++it) {
             // LU with partial pivoting of T - lam I (pending row: alpha at i, beta at i + 1)
