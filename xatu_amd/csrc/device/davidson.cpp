@@ -179,8 +179,15 @@ private:
         pending_.push_back({std::make_unique<EventTimer>(c_.stream), kind, nvec});
         return pending_.back().ev.get();
     }
-    void drain_timers() {
+    // Reads the event pairs; with completed_only, only those already complete (the others
+    // stay pending), so that it can run while the device works.
+    void drain_timers(bool completed_only = false) {
+        std::vector<Pending> later;
         for (Pending& p : pending_) {
+            if (completed_only && hipEventQuery(p.ev->b) != hipSuccess) {
+                later.push_back(std::move(p));
+                continue;
+            }
             const double ms = p.ev->ms();
             if (p.kind == 0) {
                 prod_ms_ += ms;
@@ -191,7 +198,7 @@ private:
                 rr_ms_ += ms;
             }
         }
-        pending_.clear();
+        pending_ = std::move(later);
     }
 
     // One iteration per synchronisation: Rayleigh-Ritz on the device, Ritz block and
@@ -233,18 +240,24 @@ private:
                                     hipMemcpyDeviceToHost, c_.stream));
             XATU_HIP(hipMemcpyAsync(c_.h_theta.p, ws_.theta.p, b_ * sizeof(double), hipMemcpyDeviceToHost, c_.stream));
             XATU_HIP(hipStreamSynchronize(c_.stream));
-            drain_timers();
             theta_.assign(c_.h_theta.p, c_.h_theta.p + b_);
             residual_norms(nblk);
             double worst = 0.0;
             for (int i = 0; i < nev_; ++i) worst = std::max(worst, res_[i]);
             last_worst_ = worst;
-            if (worst < tol) return true;
-            if (nb <= 0) return false;
+            if (worst < tol || nb <= 0) {
+                drain_timers();
+                return worst < tol;
+            }
             const int kept = ortho_whiten(nb, 0);
-            if (kept == 0) return false;
+            if (kept == 0) {
+                drain_timers();
+                return false;
+            }
             append_block(kept);
+            drain_timers(true);  // the timings of this iteration, while the next one runs
         }
+        drain_timers();
         return false;
     }
 
