@@ -98,14 +98,21 @@ __device__ inline void stage_coefficients(cx_t<R>* s_coef, int norb, int nk, int
     }
 }
 
+template <typename R>
+__device__ inline cx_t<R> table_cx(double2 v) { return to_cx<R>(v); }
+template <typename R>
+__device__ inline cx_t<R> table_cx(float2 v) { return mk(static_cast<R>(v.x), static_cast<R>(v.y)); }
+
 // The nc*nv x nc*nv block of (D - X) for the k-point pair (ka, kb), ka <= kb, from the
 // staged coefficients ca_ (ka) and cb_ (kb): entry (orbital al, band) of a k-point at
 // ptr[(al NB + band) cs]. sink(cA, vA, cB, vB, DX) receives every element. The
 // instruction sequence depends only on the data, so any two threads evaluating the same
 // pair produce bit-identical values.
-template <typename R, int NA, int NC, int NV, typename Sink>
+// ft: the lattice-sum table, entry (a, b) of table row q at ft[q fts + a + NA b] (FP64, or the
+// FP32 copy of the FP32 build with rows padded to an even length).
+template <typename R, int NA, int NC, int NV, typename FT, typename Sink>
 __device__ inline void pair_block(const LatticeDesc& lat, const cx_t<R>* ca_, const cx_t<R>* cb_, int cs, int ka, int kb,
-                                  const double2* __restrict__ ft, const int* __restrict__ kcoord,
+                                  const FT* __restrict__ ft, int fts, const int* __restrict__ kcoord,
                                   const double2* __restrict__ e1, const double2* __restrict__ e2, int exchange,
                                   Sink&& sink) {
     using C = cx_t<R>;
@@ -116,7 +123,7 @@ __device__ inline void pair_block(const LatticeDesc& lat, const cx_t<R>* ca_, co
 #pragma unroll
     for (int a = 0; a < NA; ++a)
 #pragma unroll
-        for (int b = 0; b < NA; ++b) M[a][b] = to_cx<R>(ft[static_cast<size_t>(q) * NA * NA + a + NA * b]);
+        for (int b = 0; b < NA; ++b) M[a][b] = table_cx<R>(ft[static_cast<size_t>(q) * fts + a + NA * b]);
 
     // rho^v_b(vb, va) = sum_{beta in b} conj(V_kb[beta, vb]) V_ka[beta, va]
     C rv[NA][NV][NV];
@@ -222,7 +229,7 @@ __global__ __launch_bounds__(TK* TK) void bse_tile_kernel(LatticeDesc lat, int n
                                                           const double* __restrict__ eval_k,
                                                           const double* __restrict__ eval_kq,
                                                           const int* __restrict__ kcoord,
-                                                          const double2* __restrict__ ft,
+                                                          const cx_t<R>* __restrict__ ft, int fts,
                                                           const double2* __restrict__ e1,
                                                           const double2* __restrict__ e2, int exchange,
                                                           double scissor, int lower_rows, float hsplit,
@@ -268,7 +275,7 @@ __global__ __launch_bounds__(TK* TK) void bse_tile_kernel(LatticeDesc lat, int n
                 H[(g0 + r) + (h0 + c) * N] = v;
             }
         };
-        pair_block<R, NA, NC, NV>(lat, ca_, cb_, 2 * TK, ka, kb, ft, kcoord, e1, e2, exchange,
+        pair_block<R, NA, NC, NV>(lat, ca_, cb_, 2 * TK, ka, kb, ft, fts, kcoord, e1, e2, exchange,
                                   [&](int cA, int vA, int cB, int vB, C DX) {
                                       const int ia = cA * NV + vA, ib = cB * NV + vB;
                                       if (diagblk) {
@@ -354,7 +361,7 @@ __global__ __launch_bounds__(TK* TK) void bse_diag_kernel(LatticeDesc lat, int n
     if (tr != tc || k >= nk) return;
     const C* ca_ = s_coef + tr;
     const C* cb_ = s_coef + TK + tc;
-    pair_block<R, NA, NC, NV>(lat, ca_, cb_, 2 * TK, k, k, ft, kcoord, e1, e2, exchange,
+    pair_block<R, NA, NC, NV>(lat, ca_, cb_, 2 * TK, k, k, ft, NA * NA, kcoord, e1, e2, exchange,
                               [&](int cA, int vA, int cB, int vB, C DX) {
                                   if (cA == cB && vA == vB) {
                                       diag[static_cast<int64_t>(k) * NCV + cA * NV + vA] =
@@ -458,7 +465,7 @@ __global__ __launch_bounds__(matfree_threads(TK)) void bse_matfree_kernel(
                 const C* ca_ = sw ? s_col + static_cast<size_t>(tc) * ss : s_row + static_cast<size_t>(tr) * ss;
                 const C* cb_ = sw ? s_row + static_cast<size_t>(tr) * ss : s_col + static_cast<size_t>(tc) * ss;
                 const bool diagblk = (ka == kb);
-                pair_block<R, NA, NC, NV>(lat, ca_, cb_, 1, ka, kb, ft, kcoord, e1, e2, exchange,
+                pair_block<R, NA, NC, NV>(lat, ca_, cb_, 1, ka, kb, ft, NA * NA, kcoord, e1, e2, exchange,
                                           [&](int cA, int vA, int cB, int vB, C DX) {
                                               const int ia = cA * NV + vA, ib = cB * NV + vB;
                                               // element (row, col) of the tile at s_h[col * LDH + row]
@@ -801,6 +808,21 @@ __global__ void compare_kernel(const double2* __restrict__ H64, const float2* __
     if (threadIdx.x == 0) out[bi + static_cast<size_t>(nblk) * bj] = s[0];
 }
 
+// out[q fts + j] = ft[q na2 + j] rounded to FP32 (j < na2), zero padding up to fts.
+__global__ void table_to_fp32_kernel(const double2* __restrict__ ft, int64_t rows, int na2, int fts,
+                                     float2* __restrict__ out) {
+    const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= rows * fts) return;
+    const int64_t q = e / fts;
+    const int j = static_cast<int>(e - q * fts);
+    if (j < na2) {
+        const double2 v = ft[q * na2 + j];
+        out[e] = make_float2(static_cast<float>(v.x), static_cast<float>(v.y));
+    } else {
+        out[e] = make_float2(0.0f, 0.0f);
+    }
+}
+
 // The FP32 build stages its output tile in LDS for coalesced stores; the FP64 tile would
 // halve the workgroups per CU.
 template <typename R>
@@ -832,8 +854,23 @@ struct Shape {
         if (hsplit > 0.0f && ((TK * NCV) % 32 != 0 || c.N % 32 != 0 || !kStageTile<R>)) {
             throw Error(XATU_ERR_LOGIC, "split FP16 layout needs 32-row tiles");
         }
+        const cx_t<R>* ft;
+        int fts;
+        if constexpr (sizeof(R) == 4) {
+            // FP32 copy of the table, rows padded to an even length (16-byte aligned rows)
+            constexpr int na2 = NA * NA;
+            fts = (na2 + 1) & ~1;
+            const int64_t rows = static_cast<int64_t>(c.d_ft.n / na2);
+            c.d_ft32.alloc(static_cast<size_t>(rows) * fts);
+            hipLaunchKernelGGL(table_to_fp32_kernel, dim3(static_cast<unsigned>((rows * fts + 255) / 256)), dim3(256), 0, s,
+                               c.d_ft.p, rows, na2, fts, c.d_ft32.p);
+            ft = c.d_ft32.p;
+        } else {
+            fts = NA * NA;
+            ft = c.d_ft.p;
+        }
         hipLaunchKernelGGL((bse_tile_kernel<R, NA, NC, NV, TK, kStageTile<R>>), dim3(nT, nT), dim3(TK * TK), lds, s, c.lat, c.nk, c.d_evec_k.p, c.d_evec_kq.p, c.d_eval_k.p, c.d_eval_kq.p, c.d_kcoord.p,
-                           c.d_ft.p, c.d_e1.p, c.d_e2.p, c.inter.exchange, c.scissor, lower_rows, hsplit, H);
+                           ft, fts, c.d_e1.p, c.d_e2.p, c.inter.exchange, c.scissor, lower_rows, hsplit, H);
     }
     // The tile size the build uses (0: none fits).
     static int build_tile(const Context& c) {

@@ -155,6 +155,7 @@ struct Context {
     DevBuf<double> d_vtabx;     // npairs x nR exchange potential table
     DevBuf<double2> d_twiddle;  // 2*mesh_n roots of unity
     DevBuf<double2> d_ft;       // natoms x natoms x nmesh
+    mutable DevBuf<float2> d_ft32;  // its FP32 copy for the FP32 build (rows padded to an even length)
     DevBuf<double2> d_ftq;      // natoms x natoms
     DevBuf<double2> d_e1, d_e2; // exchange factors: nk x natoms x (nc*nv)
 
