@@ -202,7 +202,7 @@ __device__ inline HFrag16 split_h(const float4 (&r)[4], float sH) {
 constexpr int kHermTiles = 4;
 constexpr int kHermRows = 4 * 16 * kHermTiles;
 static_assert(kHermRows == kSplitRowBlock, "row block of the split product");
-constexpr int kHermChunk = 2048;   // columns per work item, a multiple of kHermRows
+constexpr int kHermChunk = 4096;   // columns per work item, a multiple of kHermRows
 constexpr int kTrRow = 72;         // FP16 entries per row of the transposition buffer (64 + 8 padding)
 constexpr int kCbRow = 34;         // complex entries per row of the combine buffer (32 + 2 padding)
 
