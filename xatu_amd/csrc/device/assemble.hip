@@ -105,12 +105,13 @@ __device__ inline cx_t<R> table_cx(float2 v) { return mk(static_cast<R>(v.x), st
 
 // The nc*nv x nc*nv block of (D - X) for the k-point pair (ka, kb), ka <= kb, from the
 // staged coefficients ca_ (ka) and cb_ (kb): entry (orbital al, band) of a k-point at
-// ptr[(al NB + band) cs]. sink(cA, vA, cB, vB, DX) receives every element. The
-// instruction sequence depends only on the data, so any two threads evaluating the same
-// pair produce bit-identical values.
+// ptr[(al NB + band) cs]. sink(cA, vA, cB, vB, DX) receives every element, (cB, vB) fastest,
+// or with COLS_OUTER (cA, vA) fastest (the order in which a pair's rows of one column of H
+// are stored together). Each element's instruction sequence depends only on the data, not
+// on that order, so any two threads evaluating the same pair produce bit-identical values.
 // ft: the lattice-sum table, entry (a, b) of table row q at ft[q fts + a + NA b] (FP64, or the
 // FP32 copy of the FP32 build with rows padded to an even length).
-template <typename R, int NA, int NC, int NV, typename FT, typename Sink>
+template <typename R, int NA, int NC, int NV, bool COLS_OUTER = false, typename FT, typename Sink>
 __device__ inline void pair_block(const LatticeDesc& lat, const cx_t<R>* ca_, const cx_t<R>* cb_, int cs, int ka, int kb,
                                   const FT* __restrict__ ft, int fts, const int* __restrict__ kcoord,
                                   const double2* __restrict__ e1, const double2* __restrict__ e2, int exchange,
@@ -142,9 +143,8 @@ __device__ inline void pair_block(const LatticeDesc& lat, const cx_t<R>* ca_, co
             }
         }
     }
-#pragma unroll
-    for (int cA = 0; cA < NC; ++cA) {
-        // rho^c_a(cB) = sum_{alpha in a} conj(C_ka[alpha, cA]) C_kb[alpha, cB]
+    // T_b(cA, cB) = sum_a rho^c_a(cA, cB) M_ab, rho^c_a(cA, cB) = sum_{alpha in a} conj(C_ka[alpha, cA]) C_kb[alpha, cB]
+    auto t_of = [&](int cA, C (&T)[NA][NC]) {
         C rc[NA][NC];
 #pragma unroll
         for (int a = 0; a < NA; ++a) {
@@ -156,8 +156,6 @@ __device__ inline void pair_block(const LatticeDesc& lat, const cx_t<R>* ca_, co
                 for (int cB = 0; cB < NC; ++cB) cfma_conj(rc[a][cB], xa, cb_[(al * NB + cB) * cs]);
             }
         }
-        // T_b(cB) = sum_a rho^c_a(cB) M_ab
-        C T[NA][NC];
 #pragma unroll
         for (int b = 0; b < NA; ++b)
 #pragma unroll
@@ -167,29 +165,56 @@ __device__ inline void pair_block(const LatticeDesc& lat, const cx_t<R>* ca_, co
                 for (int a = 0; a < NA; ++a) cfma(s, rc[a][cB], M[a][b]);
                 T[b][cB] = s;
             }
+    };
+    auto element = [&](const C (&T)[NA][NC], const C* ex1, int cA, int vA, int cB, int vB) {
+        C D = mk(R(0), R(0));
 #pragma unroll
-        for (int vA = 0; vA < NV; ++vA) {
-            C ex1[NA];
-            if (exchange) {
+        for (int b = 0; b < NA; ++b) cfma(D, T[b][cB], rv[b][vB][vA]);
+        C X = mk(R(0), R(0));
+        if (exchange) {
 #pragma unroll
-                for (int a = 0; a < NA; ++a)
-                    ex1[a] = to_cx<R>(e1[(static_cast<size_t>(ka) * NA + a) * NCV + cA * NV + vA]);
+            for (int a = 0; a < NA; ++a)
+                cfma(X, ex1[a], to_cx<R>(e2[(static_cast<size_t>(kb) * NA + a) * NCV + cB * NV + vB]));
+        }
+        sink(cA, vA, cB, vB, csub(D, X));
+    };
+    auto ex1_of = [&](int cA, int vA, C* ex1) {
+        if (exchange) {
+#pragma unroll
+            for (int a = 0; a < NA; ++a) ex1[a] = to_cx<R>(e1[(static_cast<size_t>(ka) * NA + a) * NCV + cA * NV + vA]);
+        }
+    };
+    if constexpr (COLS_OUTER) {
+        C T[NC][NA][NC];
+        C ex1[NC][NV][NA];
+#pragma unroll
+        for (int cA = 0; cA < NC; ++cA) {
+            t_of(cA, T[cA]);
+#pragma unroll
+            for (int vA = 0; vA < NV; ++vA) ex1_of(cA, vA, ex1[cA][vA]);
+        }
+#pragma unroll
+        for (int cB = 0; cB < NC; ++cB)
+#pragma unroll
+            for (int vB = 0; vB < NV; ++vB)
+#pragma unroll
+                for (int cA = 0; cA < NC; ++cA)
+#pragma unroll
+                    for (int vA = 0; vA < NV; ++vA) element(T[cA], ex1[cA][vA], cA, vA, cB, vB);
+    } else {
+#pragma unroll
+        for (int cA = 0; cA < NC; ++cA) {
+            C T[NA][NC];
+            t_of(cA, T);
+#pragma unroll
+            for (int vA = 0; vA < NV; ++vA) {
+                C ex1[NA];
+                ex1_of(cA, vA, ex1);
+#pragma unroll
+                for (int cB = 0; cB < NC; ++cB)
+#pragma unroll
+                    for (int vB = 0; vB < NV; ++vB) element(T, ex1, cA, vA, cB, vB);
             }
-#pragma unroll
-            for (int cB = 0; cB < NC; ++cB)
-#pragma unroll
-                for (int vB = 0; vB < NV; ++vB) {
-                    C D = mk(R(0), R(0));
-#pragma unroll
-                    for (int b = 0; b < NA; ++b) cfma(D, T[b][cB], rv[b][vB][vA]);
-                    C X = mk(R(0), R(0));
-                    if (exchange) {
-#pragma unroll
-                        for (int a = 0; a < NA; ++a)
-                            cfma(X, ex1[a], to_cx<R>(e2[(static_cast<size_t>(kb) * NA + a) * NCV + cB * NV + vB]));
-                    }
-                    sink(cA, vA, cB, vB, csub(D, X));
-                }
         }
     }
 }
@@ -223,7 +248,9 @@ __host__ __device__ constexpr int tile_slot(int r, int tk, int ncv) { return (r 
 // segments; without it (the FP64 build, whose tile would cost occupancy) each pair
 // stores its block directly.
 // R: arithmetic type of the assembly (double or float).
-template <typename R, int NA, int NC, int NV, int TK, bool STAGE>
+// PART: 0 every tile; 1 the tiles above the block diagonal only, their pairs' elements
+// produced column by column (rows together, for the direct stores); 2 the other tiles.
+template <typename R, int NA, int NC, int NV, int TK, bool STAGE, int PART = 0>
 __global__ __launch_bounds__(TK* TK) void bse_tile_kernel(LatticeDesc lat, int nk, const double2* __restrict__ evk,
                                                           const double2* __restrict__ evkq,
                                                           const double* __restrict__ eval_k,
@@ -244,6 +271,7 @@ __global__ __launch_bounds__(TK* TK) void bse_tile_kernel(LatticeDesc lat, int n
     C* s_coef = reinterpret_cast<C*>(smem_raw);
     C* s_tile = s_coef + static_cast<size_t>(2 * TK) * norb * NB;
     const int Kr = blockIdx.x, Kc = blockIdx.y;
+    if ((PART == 1 && Kr >= Kc) || (PART == 2 && Kr < Kc)) return;
     // block-lower build: skip tiles whose last row lies above the row block of their first
     // column (no entry of theirs is read)
     if (lower_rows > 0) {
@@ -275,8 +303,7 @@ __global__ __launch_bounds__(TK* TK) void bse_tile_kernel(LatticeDesc lat, int n
                 H[(g0 + r) + (h0 + c) * N] = v;
             }
         };
-        pair_block<R, NA, NC, NV>(lat, ca_, cb_, 2 * TK, ka, kb, ft, fts, kcoord, e1, e2, exchange,
-                                  [&](int cA, int vA, int cB, int vB, C DX) {
+        auto sink = [&](int cA, int vA, int cB, int vB, C DX) {
                                       const int ia = cA * NV + vA, ib = cB * NV + vB;
                                       if (diagblk) {
                                           if (ia < ib) {
@@ -293,7 +320,11 @@ __global__ __launch_bounds__(TK* TK) void bse_tile_kernel(LatticeDesc lat, int n
                                       } else {
                                           put(r0 + ib, c0 + ia, mk(-DX.x, DX.y));
                                       }
-                                  });
+                                  };
+        // above the diagonal (PART 1) a pair's direct stores go column by column, its rows
+        // together (their 128-byte lines filled by consecutive stores); below it the
+        // transposed stores are in that order already
+        pair_block<R, NA, NC, NV, PART == 1>(lat, ca_, cb_, 2 * TK, ka, kb, ft, fts, kcoord, e1, e2, exchange, sink);
     }
     if (STAGE) {
         __syncthreads();
@@ -869,8 +900,19 @@ struct Shape {
             fts = NA * NA;
             ft = c.d_ft.p;
         }
-        hipLaunchKernelGGL((bse_tile_kernel<R, NA, NC, NV, TK, kStageTile<R>>), dim3(nT, nT), dim3(TK * TK), lds, s, c.lat, c.nk, c.d_evec_k.p, c.d_evec_kq.p, c.d_eval_k.p, c.d_eval_kq.p, c.d_kcoord.p,
-                           ft, fts, c.d_e1.p, c.d_e2.p, c.inter.exchange, c.scissor, lower_rows, hsplit, H);
+        if constexpr (kStageTile<R>) {
+            hipLaunchKernelGGL((bse_tile_kernel<R, NA, NC, NV, TK, true>), dim3(nT, nT), dim3(TK * TK), lds, s, c.lat, c.nk,
+                               c.d_evec_k.p, c.d_evec_kq.p, c.d_eval_k.p, c.d_eval_kq.p, c.d_kcoord.p, ft, fts, c.d_e1.p,
+                               c.d_e2.p, c.inter.exchange, c.scissor, lower_rows, hsplit, H);
+        } else {
+            // the tiles above the diagonal and the others in two launches, one element order each
+            hipLaunchKernelGGL((bse_tile_kernel<R, NA, NC, NV, TK, false, 1>), dim3(nT, nT), dim3(TK * TK), lds, s, c.lat,
+                               c.nk, c.d_evec_k.p, c.d_evec_kq.p, c.d_eval_k.p, c.d_eval_kq.p, c.d_kcoord.p, ft, fts,
+                               c.d_e1.p, c.d_e2.p, c.inter.exchange, c.scissor, lower_rows, hsplit, H);
+            hipLaunchKernelGGL((bse_tile_kernel<R, NA, NC, NV, TK, false, 2>), dim3(nT, nT), dim3(TK * TK), lds, s, c.lat,
+                               c.nk, c.d_evec_k.p, c.d_evec_kq.p, c.d_eval_k.p, c.d_eval_kq.p, c.d_kcoord.p, ft, fts,
+                               c.d_e1.p, c.d_e2.p, c.inter.exchange, c.scissor, lower_rows, hsplit, H);
+        }
     }
     // The tile size the build uses (0: none fits).
     static int build_tile(const Context& c) {
