@@ -60,14 +60,23 @@ struct DevBuf {
         XATU_HIP(hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)));
         n = count;
     }
+    // grow only: keeps an allocation of at least count entries (its contents are undefined
+    // after a regrow), so that repeated solves and steps allocate and free nothing
+    void reserve(size_t count) {
+        if (count <= n && p) return;
+        alloc(count);
+    }
     size_t bytes() const { return n * sizeof(T); }
 };
 
 // Owning pinned host allocation, grown on demand: small transfers to and from it need no
-// staging copy.
+// staging copy. reserve_mapped: fine-grained (coherent) memory mapped into the device's
+// address space at dev, which kernels store into directly (visible to the host once the
+// kernel has completed, with no copy).
 template <typename T>
 struct HostBuf {
     T* p = nullptr;
+    T* dev = nullptr;
     size_t n = 0;
     HostBuf() = default;
     HostBuf(const HostBuf&) = delete;
@@ -76,14 +85,33 @@ struct HostBuf {
     void release() {
         if (p) (void)hipHostFree(p);
         p = nullptr;
+        dev = nullptr;
         n = 0;
     }
     void reserve(size_t count) {
-        if (count <= n && p) return;
+        if (count <= n && p && !dev) return;
         release();
         XATU_HIP(hipHostMalloc(reinterpret_cast<void**>(&p), count * sizeof(T), hipHostMallocDefault));
         n = count;
     }
+    void reserve_mapped(size_t count) {
+        if (count <= n && p && dev) return;
+        release();
+        XATU_HIP(hipHostMalloc(reinterpret_cast<void**>(&p), count * sizeof(T), hipHostMallocMapped | hipHostMallocCoherent));
+        n = count;
+        XATU_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&dev), p, 0));
+    }
+};
+
+// Device workspace of the block Davidson solver: basis V, products W = H V (N x mmax), Ritz
+// block X, AX, residuals R, scratch block Tb (N x b), projections P, Ritz coefficients Y
+// (mmax x b), Ritz values theta (b), residual partial sums, the projected matrix Tm (mmax x
+// mmax) and its reflectors (device Rayleigh-Ritz). Held by the context between solves.
+struct SolverWorkspace {
+    DevBuf<double2> V, W, X, AX, R, Tb;
+    DevBuf<double2> P, Y;
+    DevBuf<double> theta, partial;
+    DevBuf<double2> Tm, refl;
 };
 
 // Geometry and lattice-sum description shared with the kernels (by value).
@@ -194,6 +222,7 @@ struct Context {
     int64_t herm_n = 0;               // matrix dimension the work items were laid out for
     DevBuf<float> s_colmax, s_xscale; // column maxima and power-of-two scales of the staged vectors
     DevBuf<double> s_xinv;            // inverse combined scales applied by the split-product reduction
+    DevBuf<double> s_bounds;          // per-block partial bounds of the split product's scales
     float h16_scale = 0.0f;           // power-of-two scale of H for the split product (0: unavailable)
     bool fp32_arith_only = false;     // products with the FP32 matrix stay on the FP32 matrix cores
     DevBuf<double2> s_gram;           // split partial sums of the Gram kernel
@@ -201,17 +230,21 @@ struct Context {
     // Ritz coefficients and values, residual partial sums, block whitening coefficients
     HostBuf<double2> h_proj, h_coef, h_whiten;
     HostBuf<double> h_theta, h_part;
+    HostBuf<double> h_gather;  // mapped: the per-iteration results of the device solver
+    // solver workspaces, kept (grow only) between solves: [0] the FP64 solve or refinement,
+    // [1] the FP32 phase of the mixed form, which both exist during a mixed solve
+    SolverWorkspace solver_ws[2];
 
     // last solve
     int last_nstates = 0;
     int64_t last_n = 0;          // dimension of the operator the last solve ran on
-    DevBuf<double2> d_last_vec;  // N x last_nstates
+    DevBuf<double2> d_last_vec;  // N x last_nstates (allocation kept, grow only)
     std::vector<double> last_val;
-    // forget the last solve's eigenpairs: they belong to an operator that has changed
+    // forget the last solve's eigenpairs: they belong to an operator that has changed (the
+    // buffer stays allocated for the next solve; last_vectors refuses it until then)
     void drop_solution() {
         last_nstates = 0;
         last_n = 0;
-        d_last_vec.release();
         last_val.clear();
     }
     // eigenvectors of the last solve for nstates states of the current operator

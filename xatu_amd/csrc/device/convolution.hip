@@ -704,6 +704,7 @@ struct ConvState {
     DevBuf<double2> G64, U64, Ct64, Vt64;
     DevBuf<float2> G32, U32, Ct32, Vt32;
     DevBuf<double2> part, total;
+    DevBuf<double2> Mt;  // interaction table on the mesh (set-up scratch, kept between set-ups)
     // A transform over the whole mesh: one rocFFT plan per direction (entry-major layout),
     // or for the row-interleaved layout two batched 1-D plans per direction (along o0,
     // then along o1)
@@ -823,7 +824,7 @@ void ensure_conv(Context& c) {
                        c.d_evec_k.p, c.d_evec_kq.p, cs.Ct32.p, cs.Vt32.p);
     XATU_HIP(hipGetLastError());
     const int na2 = c.lat.natoms * c.lat.natoms;
-    DevBuf<double2> Mt;
+    DevBuf<double2>& Mt = cs.Mt;
     Mt.alloc(static_cast<size_t>(na2) * M);
     hipLaunchKernelGGL(table_grid_kernel, dim3((Mt.n + 255) / 256), dim3(256), 0, c.stream, c.lat, c.d_ft.p, Mt.p);
     XATU_HIP(hipGetLastError());
@@ -838,7 +839,6 @@ void ensure_conv(Context& c) {
     hipLaunchKernelGGL((scale_spectrum_kernel<float>), dim3((Mt.n + 255) / 256), dim3(256), 0, c.stream, Mt.p,
                        static_cast<int64_t>(Mt.n), scale, cs.P32.p);
     XATU_HIP(hipGetLastError());
-    XATU_HIP(hipStreamSynchronize(c.stream));  // Mt goes out of scope
     cs.gen = c.prep_gen;
 }
 

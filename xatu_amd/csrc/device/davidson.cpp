@@ -45,6 +45,8 @@ void launch_precondition(const double2* R, const double* theta, const double* di
                          hipStream_t s);
 void launch_unit_vectors(const int* idx, int64_t N, int nvec, double2* V, hipStream_t s);
 void launch_copy_vectors(const double2* src, int64_t n, double2* dst, hipStream_t s);
+void launch_gather3(const double* a, int na, const double* b, int nb, const double* c, int nc, double* dst,
+                    hipStream_t s);
 int projected_eigen_max_dim();
 int projected_eigen_max_want();
 // fp32_reduction: the Householder reduction in FP32 (the FP32 phase of the mixed form)
@@ -63,48 +65,51 @@ inline const rocblas_double_complex* bz(const double2* p) {
     return reinterpret_cast<const rocblas_double_complex*>(p);
 }
 
+// Waits for the stream by polling: a blocking wait's wake-up adds tens of microseconds to
+// each of the solver's per-iteration waits.
+void poll_stream(hipStream_t s) {
+    for (;;) {
+        const hipError_t e = hipStreamQuery(s);
+        if (e == hipSuccess) return;
+        if (e != hipErrorNotReady) XATU_HIP(e);
+    }
+}
+
 double now_ms() {
     using namespace std::chrono;
     return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
 }
 
-struct Workspace {
-    DevBuf<double2> V, W, X, AX, R, Tb, tmp;
-    DevBuf<double2> P;       // projections (mmax x b)
-    DevBuf<double2> Y;       // Ritz coefficients (mmax x b)
-    DevBuf<double> theta;    // b
-    DevBuf<double> partial;  // residual partial sums
-    DevBuf<float2> x32, y32;
-    DevBuf<double2> Tm;      // projected matrix (mmax x mmax), device Rayleigh-Ritz
-    DevBuf<double2> refl;    // its Householder reflectors
-};
+using Workspace = SolverWorkspace;
 
 class Davidson {
 public:
-    Davidson(Context& c, int precision, int matrix_free, int nev, int block, int mmax, xatu_solve_report* rep)
-        : c_(c), prec_(precision), mf_(matrix_free), nev_(nev), b_(block), mmax_(mmax), rep_(rep) {
+    Davidson(Context& c, Workspace& ws, int precision, int matrix_free, int nev, int block, int mmax,
+             xatu_solve_report* rep)
+        : c_(c), prec_(precision), mf_(matrix_free), nev_(nev), b_(block), mmax_(mmax), ws_(ws), rep_(rep) {
         N_ = c.N;
-        ws_.V.alloc(static_cast<size_t>(N_) * mmax_);
-        ws_.W.alloc(static_cast<size_t>(N_) * mmax_);
-        ws_.X.alloc(static_cast<size_t>(N_) * b_);
-        ws_.AX.alloc(static_cast<size_t>(N_) * b_);
-        ws_.R.alloc(static_cast<size_t>(N_) * b_);
-        ws_.Tb.alloc(static_cast<size_t>(N_) * b_);
-        ws_.P.alloc(static_cast<size_t>(mmax_) * b_);
-        ws_.Y.alloc(static_cast<size_t>(mmax_) * b_);
-        ws_.theta.alloc(b_);
-        ws_.partial.alloc(static_cast<size_t>(b_) * ((N_ + 255) / 256));
+        ws_.V.reserve(static_cast<size_t>(N_) * mmax_);
+        ws_.W.reserve(static_cast<size_t>(N_) * mmax_);
+        ws_.X.reserve(static_cast<size_t>(N_) * b_);
+        ws_.AX.reserve(static_cast<size_t>(N_) * b_);
+        ws_.R.reserve(static_cast<size_t>(N_) * b_);
+        ws_.Tb.reserve(static_cast<size_t>(N_) * b_);
+        ws_.P.reserve(static_cast<size_t>(mmax_) * b_);
+        ws_.Y.reserve(static_cast<size_t>(mmax_) * b_);
+        ws_.theta.reserve(b_);
+        ws_.partial.reserve(static_cast<size_t>(b_) * ((N_ + 255) / 256));
         T_.assign(static_cast<size_t>(mmax_) * mmax_, cd(0.0, 0.0));
         c_.h_proj.reserve(static_cast<size_t>(mmax_) * b_);
         c_.h_coef.reserve(static_cast<size_t>(mmax_) * b_);
-        c_.h_whiten.reserve(static_cast<size_t>(b_) * b_);
+        c_.h_whiten.reserve_mapped(static_cast<size_t>(b_) * b_);
         c_.h_theta.reserve(b_);
         c_.h_part.reserve(static_cast<size_t>(b_) * ((N_ + 255) / 256));
+        c_.h_gather.reserve_mapped(2 * static_cast<size_t>(mmax_) * b_ + static_cast<size_t>(b_) * ((N_ + 255) / 256) + b_);
         dev_rr_ = mmax_ <= projected_eigen_max_dim() && b_ <= projected_eigen_max_want();
         if (dev_rr_) {
             const int md = projected_eigen_max_dim();
-            ws_.Tm.alloc(static_cast<size_t>(mmax_) * mmax_);
-            ws_.refl.alloc(static_cast<size_t>(md) * md);
+            ws_.Tm.reserve(static_cast<size_t>(mmax_) * mmax_);
+            ws_.refl.reserve(static_cast<size_t>(md) * md);
             XATU_HIP(hipMemsetAsync(ws_.Tm.p, 0, ws_.Tm.bytes(), c_.stream));
         }
     }
@@ -229,19 +234,20 @@ private:
                 rep_->restarts++;
             }
             const int nb = std::min(b_, mmax_ - m_);
+            const int np = nb > 0 ? 2 * (m_ + nb) * nb : 0;  // doubles of the block's Gram rows
             if (nb > 0) {
                 launch_precondition(ws_.R.p, ws_.theta.p, c_.d_diag.p, N_, nb, ws_.V.p + static_cast<size_t>(m_) * N_,
                                     c_.stream);
                 ortho_gram(nb);
-                XATU_HIP(hipMemcpyAsync(c_.h_proj.p, ws_.P.p, static_cast<size_t>(m_ + nb) * nb * sizeof(cd),
-                                        hipMemcpyDeviceToHost, c_.stream));
             }
-            XATU_HIP(hipMemcpyAsync(c_.h_part.p, ws_.partial.p, static_cast<size_t>(nblk) * b_ * sizeof(double),
-                                    hipMemcpyDeviceToHost, c_.stream));
-            XATU_HIP(hipMemcpyAsync(c_.h_theta.p, ws_.theta.p, b_ * sizeof(double), hipMemcpyDeviceToHost, c_.stream));
-            XATU_HIP(hipStreamSynchronize(c_.stream));
-            theta_.assign(c_.h_theta.p, c_.h_theta.p + b_);
-            residual_norms(nblk);
+            // Gram rows, residual partial sums and Ritz values stored into mapped host memory by
+            // one kernel (three device-to-host copies each waited tens of microseconds here)
+            launch_gather3(reinterpret_cast<const double*>(ws_.P.p), np, ws_.partial.p, nblk * b_, ws_.theta.p, b_,
+                           c_.h_gather.dev, c_.stream);
+            poll_stream(c_.stream);
+            const double* got = c_.h_gather.p;
+            theta_.assign(got + np + nblk * b_, got + np + nblk * b_ + b_);
+            residual_norms(nblk, got + np);
             double worst = 0.0;
             for (int i = 0; i < nev_; ++i) worst = std::max(worst, res_[i]);
             last_worst_ = worst;
@@ -249,7 +255,7 @@ private:
                 drain_timers();
                 return worst < tol;
             }
-            const int kept = ortho_whiten(nb, 0);
+            const int kept = ortho_whiten(nb, 0, reinterpret_cast<const cd*>(got));
             if (kept == 0) {
                 drain_timers();
                 return false;
@@ -261,8 +267,9 @@ private:
         return false;
     }
 
-    void residual_norms(int nblk) {
-        const double* part = c_.h_part.p;
+    // part: the residual kernel's per-block partial sums on the host (nblk x b_)
+    void residual_norms(int nblk, const double* part = nullptr) {
+        if (!part) part = c_.h_part.p;
         res_.assign(b_, 0.0);
         for (int j = 0; j < b_; ++j) {
             double s = 0.0;
@@ -402,12 +409,13 @@ private:
         ev->stop();
     }
 
-    // The host part, with P on the host: whiten the block (its Gram matrix rows m..m+nb).
-    int ortho_whiten(int nb, int round) {
+    // The host part, with P on the host (Ph, else the pinned copy): whiten the block (its Gram
+    // matrix rows m..m+nb).
+    int ortho_whiten(int nb, int round, const cd* Ph = nullptr) {
         const int n = static_cast<int>(N_);
         const int mn = m_ + nb;
         double2* T = ws_.V.p + static_cast<size_t>(m_) * N_;
-        const cd* P = reinterpret_cast<const cd*>(c_.h_proj.p);
+        const cd* P = Ph ? Ph : reinterpret_cast<const cd*>(c_.h_proj.p);
         // column-normalised Gram matrix of the block
         std::vector<cd> G(static_cast<size_t>(nb) * nb);
         std::vector<double> dsc(nb);
@@ -452,8 +460,7 @@ private:
                 for (int j = 0; j < nb; ++j)
                     for (int i = 0; i < nb; ++i)
                         S[i + static_cast<size_t>(j) * nb] = i <= j ? dsc[i] * std::conj(Li[j + static_cast<size_t>(i) * nb]) : cd(0.0, 0.0);
-                XATU_HIP(hipMemcpyAsync(ws_.P.p, c_.h_whiten.p, static_cast<size_t>(nb) * nb * sizeof(cd),
-                                        hipMemcpyHostToDevice, c_.stream));
+                launch_copy_vectors(c_.h_whiten.dev, static_cast<int64_t>(nb) * nb, ws_.P.p, c_.stream);
                 launch_block_gemm(T, n, ws_.P.p, nb, N_, nb, nb, 1.0, 0.0, ws_.Tb.p, n, c_.stream);
                 launch_copy_vectors(ws_.Tb.p, static_cast<int64_t>(N_) * nb, T, c_.stream);
                 return nb;
@@ -477,7 +484,7 @@ private:
         if (kept == 0) return 0;
         // V[:, m:m+kept] = T S (through the scratch block: the product may not overlap its input)
         std::copy(S.begin(), S.end(), reinterpret_cast<cd*>(c_.h_whiten.p));
-        XATU_HIP(hipMemcpyAsync(ws_.P.p, c_.h_whiten.p, S.size() * sizeof(cd), hipMemcpyHostToDevice, c_.stream));
+        launch_copy_vectors(c_.h_whiten.dev, static_cast<int64_t>(S.size()), ws_.P.p, c_.stream);
         launch_block_gemm(T, n, ws_.P.p, nb, N_, nb, kept, 1.0, 0.0, ws_.Tb.p, n, c_.stream);
         launch_copy_vectors(ws_.Tb.p, static_cast<int64_t>(N_) * kept, T, c_.stream);
         if (round == 0 && lmin_kept < 1e-8 * lmax) return orthonormalise_slot(kept, 1);
@@ -497,12 +504,12 @@ private:
 
     Context& c_;
     int prec_, mf_, nev_, b_, mmax_;
+    Workspace& ws_;  // the context's: kept between solves
     bool dev_rr_ = false;
     std::vector<Pending> pending_;
     xatu_solve_report* rep_;
     int64_t N_ = 0;
     int m_ = 0;
-    Workspace ws_;
     std::vector<cd> T_;
     std::vector<double> theta_, res_;
     double prod_ms_ = 0.0, rr_ms_ = 0.0, proj_ms_ = 0.0, last_worst_ = 0.0;
@@ -534,7 +541,7 @@ void solve_dense_diag(Context& c, int nstates, double* eigval_out, double* eigve
     }
     c.last_nstates = ns;
     c.last_n = c.N;
-    c.d_last_vec.alloc(static_cast<size_t>(n) * ns);
+    c.d_last_vec.reserve(static_cast<size_t>(n) * ns);
     XATU_HIP(hipMemcpyAsync(c.d_last_vec.p, A.p, static_cast<size_t>(n) * ns * sizeof(double2),
                             hipMemcpyDeviceToDevice, c.stream));
     c.last_val.assign(eigval_out, eigval_out + ns);
@@ -569,7 +576,7 @@ void solve_davidson(Context& c, const xatu_solve_options& opt, double* eigval_ou
     bool ok;
     if (opt.precision == XATU_FP32) {
         // FP32 products to a tolerance above the FP32 noise floor ...
-        d32 = std::make_unique<Davidson>(c, XATU_FP32, opt.matrix_free, nev, b, mmax, rep);
+        d32 = std::make_unique<Davidson>(c, c.solver_ws[1], XATU_FP32, opt.matrix_free, nev, b, mmax, rep);
         d32->init_unit();
         // With a resident FP32 matrix, 3e-4 eV: past that the FP32 iterations, at the full
         // block and basis, cost more than the FP64 refinement iterations that replace them
@@ -583,7 +590,7 @@ void solve_davidson(Context& c, const xatu_solve_options& opt, double* eigval_ou
         // (matrix-free when no FP64 matrix is resident), until the FP64 residuals meet tol.
         const int bref = nev;
         const int mref = static_cast<int>(std::min<int64_t>(N, std::max(4 * bref, bref + 1)));
-        dav = std::make_unique<Davidson>(c, XATU_FP64, opt.matrix_free ? opt.matrix_free : (c.built64 ? 0 : 1), nev, bref, mref, rep);
+        dav = std::make_unique<Davidson>(c, c.solver_ws[0], XATU_FP64, opt.matrix_free ? opt.matrix_free : (c.built64 ? 0 : 1), nev, bref, mref, rep);
         dav->init_from(d32->ritz_vectors());
         ok = dav->run(tol, maxit);
         rep->refine_iterations = rep->iterations - it32;
@@ -591,7 +598,7 @@ void solve_davidson(Context& c, const xatu_solve_options& opt, double* eigval_ou
         rep->ms_rayleigh_ritz = d32->rr_ms();
         rep->ms_projections = d32->projections_ms();
     } else {
-        dav = std::make_unique<Davidson>(c, opt.precision, opt.matrix_free, nev, b, mmax, rep);
+        dav = std::make_unique<Davidson>(c, c.solver_ws[0], opt.precision, opt.matrix_free, nev, b, mmax, rep);
         dav->init_unit();
         ok = dav->run(tol, maxit);
     }
@@ -607,7 +614,7 @@ void solve_davidson(Context& c, const xatu_solve_options& opt, double* eigval_ou
     c.last_nstates = nev;
     c.last_n = N;
     c.last_val.assign(th.begin(), th.begin() + nev);
-    c.d_last_vec.alloc(static_cast<size_t>(N) * nev);
+    c.d_last_vec.reserve(static_cast<size_t>(N) * nev);
     XATU_HIP(hipMemcpyAsync(c.d_last_vec.p, final_d.ritz_vectors(), static_cast<size_t>(N) * nev * sizeof(double2),
                             hipMemcpyDeviceToDevice, c.stream));
     if (eigvec_out) {

@@ -831,7 +831,7 @@ void split_product_scale(Context& c) {
     const int nftq = c.lat.natoms * c.lat.natoms;
     const int64_t ncols = static_cast<int64_t>(c.nk) * c.nb;
     constexpr int kBlocks = 64;
-    DevBuf<double> part;
+    DevBuf<double>& part = c.s_bounds;
     part.alloc(3 * kBlocks);
     hipLaunchKernelGGL(operator_bounds_kernel, dim3(kBlocks), dim3(256), 0, c.stream, c.d_ft.p,
                        static_cast<int64_t>(nft), c.d_ftq.p, nftq, c.d_evec_k.p, c.d_evec_kq.p, ncols, c.lat.norb,

@@ -52,6 +52,20 @@ __global__ void copy_vectors_kernel(const double2* __restrict__ src, int64_t n, 
     if (i + 1 < n) dst[i + 1] = src[i + 1];
 }
 
+// dst = [a (na), b (nb), c (nc)]: three device arrays gathered into one destination (mapped
+// host memory), one double per thread
+__global__ void gather3_kernel(const double* __restrict__ a, int na, const double* __restrict__ b, int nb,
+                               const double* __restrict__ c, int nc, double* __restrict__ dst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < na) {
+        dst[i] = a[i];
+    } else if (i < na + nb) {
+        dst[i] = b[i - na];
+    } else if (i < na + nb + nc) {
+        dst[i] = c[i - na - nb];
+    }
+}
+
 // V[:, j] = e_{idx[j]}
 __global__ void unit_vectors_kernel(const int* __restrict__ idx, int64_t N, int nvec, double2* __restrict__ V) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -101,6 +115,14 @@ void launch_precondition(const double2* R, const double* theta, const double* di
 void launch_copy_vectors(const double2* src, int64_t n, double2* dst, hipStream_t s) {
     if (n <= 0) return;
     hipLaunchKernelGGL(copy_vectors_kernel, dim3(static_cast<unsigned>((n + 511) / 512)), dim3(256), 0, s, src, n, dst);
+    XATU_HIP(hipGetLastError());
+}
+
+void launch_gather3(const double* a, int na, const double* b, int nb, const double* c, int nc, double* dst,
+                    hipStream_t s) {
+    const int n = na + nb + nc;
+    if (n <= 0) return;
+    hipLaunchKernelGGL(gather3_kernel, dim3((n + 255) / 256), dim3(256), 0, s, a, na, b, nb, c, nc, dst);
     XATU_HIP(hipGetLastError());
 }
 
