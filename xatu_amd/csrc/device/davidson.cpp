@@ -578,13 +578,15 @@ void solve_davidson(Context& c, const xatu_solve_options& opt, double* eigval_ou
         // FP32 products to a tolerance above the FP32 noise floor ...
         d32 = std::make_unique<Davidson>(c, c.solver_ws[1], XATU_FP32, opt.matrix_free, nev, b, mmax, rep);
         d32->init_unit();
-        // With a resident FP32 matrix, 3e-4 eV: past that the FP32 iterations, at the full
+        // With a resident FP32 matrix, 6e-4 eV: past that the FP32 iterations, at the full
         // block and basis, cost more than the FP64 refinement iterations that replace them
-        // (C4: 2e-5 -> 3e-4 took the step from 67.1 to 63.9 ms at the same 23 iterations, 2 -> 6
-        // of them refinement). Matrix-free, an FP64 product costs as much as an FP32 one, so
-        // the FP32 phase goes on to 2e-5 (C5 at 3e-4: 0.439 s with 8 refinement iterations,
-        // against 0.409 s with 2).
-        d32->run(std::max(0.5 * tol, opt.matrix_free ? 2e-5 : 3e-4), maxit);
+        // (N = 36864, 18 orbitals: 2e-5 -> 3e-4 took the step from 67.1 to 63.9 ms at the same
+        // 23 iterations, 2 -> 6 of them refinement; later, with cheaper iterations, 3e-4 ->
+        // 6e-4 took it from 56.4 to 55.3 ms, 6 -> 8 refinement, and 2e-3 back to 57.0 ms).
+        // Matrix-free, an FP64 product costs as much as an FP32 one, so the FP32 phase goes on
+        // to 2e-5 (N = 129600 at 3e-4: 0.439 s with 8 refinement iterations, against 0.409 s
+        // with 2).
+        d32->run(std::max(0.5 * tol, opt.matrix_free ? 2e-5 : 6e-4), maxit);
         const int it32 = rep->iterations;
         // ... then FP64 refinement from the lowest nev Ritz vectors, with FP64 products
         // (matrix-free when no FP64 matrix is resident), until the FP64 residuals meet tol.
