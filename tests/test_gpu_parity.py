@@ -12,6 +12,7 @@ import numpy as np
 import pytest
 
 from cases import MODELS, gpu_context, oracle_case, rel_err
+from xatu_amd import capi
 
 pytestmark = pytest.mark.gpu
 
@@ -125,6 +126,37 @@ def test_davidson_rayleigh_ritz_paths(nstates, block, max_basis, request):
     assert np.abs(w - wr).max() < 1e-6
     assert np.abs(H @ v - v * w).max() < 1e-6
     assert np.abs(v.conj().T @ v - np.eye(nstates)).max() < 1e-8
+    ctx.close()
+
+
+def test_davidson_workspace_reuse(request):
+    """Solves of different shapes and forms on one context: the solver workspaces and the
+    last solve's vectors are kept between solves and only grow (a larger basis regrows
+    them, on the host Rayleigh-Ritz path past 128), so every solve must still match the
+    dense eigenvalues, and a new interaction must still invalidate the kept vectors."""
+    ex, H = get_case("random18", request)
+    wr = np.linalg.eigvalsh(H)
+    ctx = gpu_context(ex)
+    ctx.build(0, scissor=ex.scissor)
+    ctx.build(1, scissor=ex.scissor)
+    for nstates, block, max_basis, prec in ((16, 32, 96, 0), (16, 32, 0, 1), (16, 32, 160, 0), (6, 12, 128, 0),
+                                            (8, 16, 0, 1), (16, 32, 96, 0)):
+        w, v, rep = ctx.solve(method=1, nstates=nstates, precision=prec, block=block, max_basis=max_basis,
+                              tol=1e-7 if prec == 0 else 1e-6)
+        assert rep.converged == 1, (nstates, block, max_basis, prec)
+        assert np.abs(w - wr[:nstates]).max() < (1e-6 if prec == 0 else 1e-4), (nstates, block, max_basis, prec)
+        assert np.abs(H @ v - v * w).max() < 1e-5
+    # the last solve's vectors serve the gradients without an explicit eigvec ...
+    assert np.array_equal(ctx.hf_gradients(4, [0]), ctx.hf_gradients(4, [0], eigvec=v[:, :4]))
+    ctx.close()
+    # ... until the interaction changes: new lattice sums invalidate them
+    ctx = gpu_context(ex)
+    ctx.build(0, scissor=ex.scissor)
+    ctx.solve(method=1, nstates=4, precision=0, tol=1e-7)
+    ctx.fourier_sums(ex.eps_m, ex.eps_s, ex.r0, ex.cutoff, ex.ncell, ex.total_cells,
+                     regularization=ex.regularization, exchange=ex.exchange, Q=tuple(ex.Q))
+    with pytest.raises(capi.XatuError, match="no eigenvectors"):
+        ctx.hf_gradients(4, [0])
     ctx.close()
 
 
