@@ -447,6 +447,158 @@ __global__ __launch_bounds__(512) void conv_columns4_kernel(LatticeDesc lat, int
     }
 }
 
+// Whole-plane form of the same transform, for 2-D meshes whose n x n plane (FP64 up to n = 100,
+// FP32 up to 128) fits in LDS: one workgroup per batch entry b loads its plane once,
+// transforms the rows, transforms the columns with the spectrum product between the forward
+// and inverse column transforms, inverts the rows and stores the plane, so G is read and
+// written once per product instead of once per pass (row transform, column kernel, inverse
+// row transform). Every stage is a four-step stage done in place: a task reads the positions
+// B j + s (or B t + j) of its row or column and writes the same positions, so only the
+// barriers between stages are needed. The forward A-point stage stores its output k1 at
+// B k1 + s; after the forward B-point stage, position B t + k2 of a row holds the frequency
+// t + A k2 (the columns are transformed in that order and the spectrum read accordingly).
+// Rows are ldp = n + 1 entries apart in LDS so that tasks on consecutive rows use different
+// banks. grid batch, block plane_threads<R>() (FP64: 512, whose 256 registers per thread hold
+// the 16-point stages without spilling; one FP64 plane takes most of the LDS anyway),
+// dynamic LDS (kColMax + n (n + 1)) entries.
+template <typename R>
+constexpr int plane_threads() { return sizeof(R) == 8 ? 512 : 1024; }
+
+// A-point stage, forward (from positions B j2 + s to B k1 + s, times e^{-2 pi i s k1 / n}) or
+// inverse (from B k1 + s to B j2 + s, no twiddle); in and out may be the same.
+template <typename R, int A>
+__device__ inline void plane_stage_a(const cx_t<R>* in, int istr, cx_t<R>* out, int ostr, int B, int s, int n,
+                                     const cx_t<R>* tw, bool forward) {
+    cx_t<R> v[A];
+#pragma unroll
+    for (int j = 0; j < A; ++j) v[j] = in[static_cast<int64_t>(B * j + s) * istr];
+    dft_len<R, A>(v, forward ? R(-1) : R(1), tw, n);
+#pragma unroll
+    for (int k = 0; k < A; ++k)
+        out[static_cast<int64_t>(B * k + s) * ostr] = (forward && k) ? cmul(v[k], tw[s * k]) : v[k];
+}
+
+// B-point stage in place at positions B t + j (j < B): forward (output k2 at B t + k2), or
+// inverse then times e^{+2 pi i j t / n}; with spec, forward, times spec[(t + A k2) n],
+// inverse and twiddle back (the column stage between the forward and inverse transforms).
+template <typename R, int B>
+__device__ inline void plane_stage_b(cx_t<R>* base, int str, int t, int A, int n, const cx_t<R>* tw, int mode,
+                                     const cx_t<R>* spec) {
+    using C = cx_t<R>;
+    C v[B];
+#pragma unroll
+    for (int j = 0; j < B; ++j) v[j] = base[(B * t + j) * str];
+    if (mode >= 0) dft_len<R, B>(v, R(-1), tw, n);  // 0: forward; 1: forward, spectrum, inverse
+    if (mode == 1) {
+#pragma unroll
+        for (int k = 0; k < B; ++k) v[k] = cmul(v[k], spec[static_cast<int64_t>(n) * (t + A * k)]);
+    }
+    if (mode != 0) dft_len<R, B>(v, R(1), tw, n);  // -1: inverse only
+#pragma unroll
+    for (int j = 0; j < B; ++j) {
+        const C w = tw[j * t];
+        base[(B * t + j) * str] = (mode != 0 && j * t) ? cmul(v[j], mk(w.x, -w.y)) : v[j];
+    }
+}
+
+template <typename R, int NT>
+__global__ __launch_bounds__(NT) void conv_plane_kernel(LatticeDesc lat, int n, int A, int B, int batch,
+                                                                   int nvec, int64_t M, const cx_t<R>* __restrict__ P,
+                                                                   cx_t<R>* __restrict__ G) {
+    using C = cx_t<R>;
+    extern __shared__ unsigned char plane_raw[];
+    C* tw = reinterpret_cast<C*>(plane_raw);
+    C* pl = tw + kColMax;
+    const int ldp = n + 1;
+    const int tid = threadIdx.x;
+    const int b = blockIdx.x;
+    for (int k = tid; k < n; k += NT) {
+        double sn, cs;
+        sincospi(2.0 * k / n, &sn, &cs);
+        tw[k] = mk(static_cast<R>(cs), static_cast<R>(-sn));  // e^{-2 pi i k / n}
+    }
+    const int ab = b / nvec;
+    const int al = ab / lat.norb, be = ab % lat.norb;
+    const C* pp = P + static_cast<int64_t>(atom_of(lat, al) + lat.natoms * atom_of(lat, be)) * M;
+    C* g = G + static_cast<int64_t>(b) * n;  // row o1 of the plane at g + o1 gs
+    const int64_t gs = static_cast<int64_t>(batch) * n;
+    __syncthreads();
+    // A and B are the same for the whole workgroup: every thread takes the same case
+#define XATU_SWITCH_A switch (A) { XATU_FOUR_STEP_LENGTHS(XATU_CASE_A) }
+#define XATU_SWITCH_B switch (B) { XATU_FOUR_STEP_LENGTHS(XATU_CASE_B) }
+    // rows, forward A-point stage from G (s fastest: a row's 16-byte entries together)
+    for (int task = tid; task < n * B; task += NT) {
+        const int s = task % B, o1 = task / B;
+#define XATU_CASE_A(L) \
+    case L: plane_stage_a<R, L>(g + o1 * gs, 1, pl + o1 * ldp, 1, B, s, n, tw, true); break;
+        XATU_SWITCH_A
+#undef XATU_CASE_A
+    }
+    __syncthreads();
+    // rows, forward B-point stage (o1 fastest: rows ldp entries apart)
+    for (int task = tid; task < n * A; task += NT) {
+        const int o1 = task % n, t = task / n;
+#define XATU_CASE_B(L) \
+    case L: plane_stage_b<R, L>(pl + o1 * ldp, 1, t, A, n, tw, 0, nullptr); break;
+        XATU_SWITCH_B
+#undef XATU_CASE_B
+    }
+    __syncthreads();
+    // columns (position p of every row), forward A-point stage
+    for (int task = tid; task < n * B; task += NT) {
+        const int p = task % n, s = task / n;
+#define XATU_CASE_A(L) \
+    case L: plane_stage_a<R, L>(pl + p, ldp, pl + p, ldp, B, s, n, tw, true); break;
+        XATU_SWITCH_A
+#undef XATU_CASE_A
+    }
+    __syncthreads();
+    // columns: forward B-point stage, spectrum of row frequency p / B + A (p % B), inverse
+    for (int task = tid; task < n * A; task += NT) {
+        const int p = task % n, t = task / n;
+        const C* spec = pp + (p / B + A * (p % B));
+#define XATU_CASE_B(L) \
+    case L: plane_stage_b<R, L>(pl + p, ldp, t, A, n, tw, 1, spec); break;
+        XATU_SWITCH_B
+#undef XATU_CASE_B
+    }
+    __syncthreads();
+    // columns, inverse A-point stage
+    for (int task = tid; task < n * B; task += NT) {
+        const int p = task % n, s = task / n;
+#define XATU_CASE_A(L) \
+    case L: plane_stage_a<R, L>(pl + p, ldp, pl + p, ldp, B, s, n, tw, false); break;
+        XATU_SWITCH_A
+#undef XATU_CASE_A
+    }
+    __syncthreads();
+    // rows, inverse B-point stage
+    for (int task = tid; task < n * A; task += NT) {
+        const int o1 = task % n, t = task / n;
+#define XATU_CASE_B(L) \
+    case L: plane_stage_b<R, L>(pl + o1 * ldp, 1, t, A, n, tw, -1, nullptr); break;
+        XATU_SWITCH_B
+#undef XATU_CASE_B
+    }
+    __syncthreads();
+    // rows, inverse A-point stage to G
+    for (int task = tid; task < n * B; task += NT) {
+        const int s = task % B, o1 = task / B;
+#define XATU_CASE_A(L) \
+    case L: plane_stage_a<R, L>(pl + o1 * ldp, 1, g + o1 * gs, 1, B, s, n, tw, false); break;
+        XATU_SWITCH_A
+#undef XATU_CASE_A
+    }
+#undef XATU_SWITCH_A
+#undef XATU_SWITCH_B
+}
+
+// LDS bytes of conv_plane_kernel for mesh side n.
+template <typename R>
+size_t conv_plane_lds(int n) {
+    return (static_cast<size_t>(kColMax) + static_cast<size_t>(n) * (n + 1)) * sizeof(cx_t<R>);
+}
+
 // Factors (A, B) of the four-step column transform for mesh side n <= kColMax, A >= B, both
 // lengths of dft_len, B as large as possible; A = 0 if none.
 void four_step_factors(int n, int& A, int& B) {
@@ -705,6 +857,7 @@ struct ConvState {
     DevBuf<float2> G32, U32, Ct32, Vt32;
     DevBuf<double2> part, total;
     DevBuf<double2> Mt;  // interaction table on the mesh (set-up scratch, kept between set-ups)
+    size_t lds_limit = 0;  // LDS bytes a workgroup may use on this device (0: not queried yet)
     // A transform over the whole mesh: one rocFFT plan per direction (entry-major layout),
     // or for the row-interleaved layout two batched 1-D plans per direction (along o0,
     // then along o1)
@@ -725,6 +878,17 @@ struct ConvState {
 };
 
 void destroy_conv_state(ConvState* s) { delete s; }
+
+// LDS a workgroup may use on the current device (queried once per state).
+size_t conv_lds_limit(ConvState& cs) {
+    if (cs.lds_limit == 0) {
+        int dev = 0, v = 0;
+        XATU_HIP(hipGetDevice(&dev));
+        XATU_HIP(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        cs.lds_limit = static_cast<size_t>(std::max(v, 1));
+    }
+    return cs.lds_limit;
+}
 
 namespace {
 
@@ -872,24 +1036,33 @@ void conv_product(Context& c, const double2* X, double2* Y, int nvec, hipStream_
                                dim3(256), 0, s, c.nk, norb, nv, lay, cs.d_gk.p, Ct, U.p, G.p);
         });
         XATU_HIP(hipGetLastError());
-        ConvState::Plan& p = get_plan<R>(c, cs, batch, inter);
-        void* buf[1] = {G.p};
-        XATU_FFT(rocfft_execute(p.fwd, buf, nullptr, p.info));
         int fa = 0, fb = 0;
         four_step_factors(c.lat.mesh_n, fa, fb);
-        if (inter && fa > 0) {
-            const int64_t count = static_cast<int64_t>(batch) * c.lat.mesh_n;
-            hipLaunchKernelGGL((conv_columns4_kernel<R>),
-                               dim3(static_cast<unsigned>((count + kFourStepCols - 1) / kFourStepCols)), dim3(512), 0, s,
-                               c.lat, c.lat.mesh_n, fa, fb, count, nv, M, P, G.p);
+        const size_t plane_lds = conv_plane_lds<R>(c.lat.mesh_n);
+        if (inter && fa > 0 && plane_lds <= conv_lds_limit(cs)) {
+            // whole planes in LDS: G read and written once
+            hipLaunchKernelGGL((conv_plane_kernel<R, plane_threads<R>()>), dim3(batch), dim3(plane_threads<R>()),
+                               plane_lds, s, c.lat,
+                               c.lat.mesh_n, fa, fb, batch, nv, M, P, G.p);
         } else {
-            if (p.fwd2) XATU_FFT(rocfft_execute(p.fwd2, buf, nullptr, p.info));
-            hipLaunchKernelGGL((conv_multiply_kernel<R>), dim3((M + 255) / 256, batch), dim3(256), 0, s, c.lat, nv, M,
-                               lay, P, G.p);
-            if (p.bwd2) XATU_FFT(rocfft_execute(p.bwd2, buf, nullptr, p.info));
+            ConvState::Plan& p = get_plan<R>(c, cs, batch, inter);
+            void* buf[1] = {G.p};
+            XATU_FFT(rocfft_execute(p.fwd, buf, nullptr, p.info));
+            if (inter && fa > 0) {
+                const int64_t count = static_cast<int64_t>(batch) * c.lat.mesh_n;
+                hipLaunchKernelGGL((conv_columns4_kernel<R>),
+                                   dim3(static_cast<unsigned>((count + kFourStepCols - 1) / kFourStepCols)), dim3(512), 0,
+                                   s, c.lat, c.lat.mesh_n, fa, fb, count, nv, M, P, G.p);
+            } else {
+                if (p.fwd2) XATU_FFT(rocfft_execute(p.fwd2, buf, nullptr, p.info));
+                hipLaunchKernelGGL((conv_multiply_kernel<R>), dim3((M + 255) / 256, batch), dim3(256), 0, s, c.lat, nv,
+                                   M, lay, P, G.p);
+                if (p.bwd2) XATU_FFT(rocfft_execute(p.bwd2, buf, nullptr, p.info));
+            }
+            XATU_HIP(hipGetLastError());
+            XATU_FFT(rocfft_execute(p.bwd, buf, nullptr, p.info));
         }
         XATU_HIP(hipGetLastError());
-        XATU_FFT(rocfft_execute(p.bwd, buf, nullptr, p.info));
         with_bands<R>(c, [&](auto ncc, auto nvc) {
             constexpr int NC = decltype(ncc)::value, NV = decltype(nvc)::value;
             const int yg = (nv + conv_y_group<R>() - 1) / conv_y_group<R>();
