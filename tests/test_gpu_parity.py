@@ -307,6 +307,24 @@ def test_matrix_free_davidson(name, nstates, form, request):
     ctx.close()
 
 
+# Mesh sides for the whole-plane convolution kernel: 100 = 10 x 10 is the largest FP64 plane
+# that fits in LDS (163648 of 163840 bytes); 45 = 9 x 5 an uneven factorisation.
+@pytest.mark.parametrize("n", [100, 45])
+def test_plane_convolution_sizes(n):
+    print(f"[plane convolution] hBN {n}x{n}", flush=True)
+    ex = oracle_case(HBN, n, 1, 0, (1, 1, 10), cutoff=20, exchange=True)
+    ctx = gpu_context(ex)
+    ctx.build(2)  # matrix-free: exchange factors and the diagonal only
+    rng = np.random.default_rng(n)
+    cols = np.sort(rng.choice(ex.N, size=16, replace=False))
+    E = np.zeros((ex.N, 16), complex, order="F")
+    E[cols, np.arange(16)] = 1.0
+    ref = ex.bse_columns(cols)
+    assert np.abs(ctx.matvec(E, precision=0, matrix_free=1) - ref).max() <= 1e-10 * np.abs(ref).max()
+    assert np.abs(ctx.matvec(E, precision=1, matrix_free=1) - ref).max() <= 1e-5 * np.abs(ref).max()
+    ctx.close()
+
+
 @pytest.mark.parametrize("name,nstates", [("hbn30_cut20_x", 8), ("random18", 16)])
 def test_mixed_refines_matrix_free(name, nstates, request):
     # FP32 matrix resident, no FP64 matrix: the FP64 refinement runs matrix-free
