@@ -131,6 +131,9 @@ constexpr int conv_vec_group() { return sizeof(R) == 8 ? 2 : 4; }
 // its norb^2 loads per vector (the band coefficients it re-reads hit the caches)
 template <typename R>
 constexpr int conv_y_group() { return 1; }
+// orbitals be per pass over al in the y contraction (their partial sums stay in registers)
+template <typename R>
+constexpr int conv_y_be_block() { return sizeof(R) == 8 ? 6 : 12; }
 
 // u[((n NC + c) norb + be) nk + k] = sum_v conj(V_k[be, v]) x[k NCV + c NV + v, n]
 // thread per k, blockIdx.y = group of vectors
@@ -635,35 +638,53 @@ __global__ __launch_bounds__(256) void conv_y_kernel(int nk, int norb, int nvec,
         for (int c = 0; c < NC; ++c)
 #pragma unroll
             for (int v = 0; v < NV; ++v) acc[q][c][v] = mk(R(0), R(0));
-    for (int be = 0; be < norb; ++be) {
-        cx_t<R> t[VG][NC];
+    // BE orbitals be per pass over al, so that C_k[al, c] is loaded once per pass instead of
+    // once per be; every t(be) = sum_al conj(C) w and acc += V t(be) keeps its order (al, then
+    // be increasing), so the result does not depend on BE
+    constexpr int BE = conv_y_be_block<R>();
+    for (int be0 = 0; be0 < norb; be0 += BE) {
+        const int nbe = min(BE, norb - be0);
+        cx_t<R> t[VG][BE][NC];
 #pragma unroll
         for (int q = 0; q < VG; ++q)
 #pragma unroll
-            for (int c = 0; c < NC; ++c) t[q][c] = mk(R(0), R(0));
+            for (int j = 0; j < BE; ++j)
+#pragma unroll
+                for (int c = 0; c < NC; ++c) t[q][j][c] = mk(R(0), R(0));
         for (int al = 0; al < norb; ++al) {
             cx_t<R> ca[NC];
 #pragma unroll
             for (int c = 0; c < NC; ++c) ca[c] = Ct[(static_cast<size_t>(c) * norb + al) * nk + k];
-            const cx_t<R>* w = W + lay.at((static_cast<int64_t>(al) * norb + be) * nvec + n0, g);
 #pragma unroll
-            for (int q = 0; q < VG; ++q) {
-                if (q < ng) {
-                    const cx_t<R> wq = w[q * lay.bstride];
+            for (int j = 0; j < BE; ++j) {
+                if (j < nbe) {
+                    const cx_t<R>* w = W + lay.at((static_cast<int64_t>(al) * norb + be0 + j) * nvec + n0, g);
 #pragma unroll
-                    for (int c = 0; c < NC; ++c) cfma_conj(t[q][c], ca[c], wq);
+                    for (int q = 0; q < VG; ++q) {
+                        if (q < ng) {
+                            const cx_t<R> wq = w[q * lay.bstride];
+#pragma unroll
+                            for (int c = 0; c < NC; ++c) cfma_conj(t[q][j][c], ca[c], wq);
+                        }
+                    }
                 }
             }
         }
-        cx_t<R> vb[NV];
 #pragma unroll
-        for (int v = 0; v < NV; ++v) vb[v] = Vt[(static_cast<size_t>(v) * norb + be) * nk + k];
+        for (int j = 0; j < BE; ++j) {
+            if (j < nbe) {
+                const int be = be0 + j;
+                cx_t<R> vb[NV];
 #pragma unroll
-        for (int q = 0; q < VG; ++q)
+                for (int v = 0; v < NV; ++v) vb[v] = Vt[(static_cast<size_t>(v) * norb + be) * nk + k];
 #pragma unroll
-            for (int c = 0; c < NC; ++c)
+                for (int q = 0; q < VG; ++q)
 #pragma unroll
-                for (int v = 0; v < NV; ++v) cfma(acc[q][c][v], vb[v], t[q][c]);
+                    for (int c = 0; c < NC; ++c)
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) cfma(acc[q][c][v], vb[v], t[q][j][c]);
+            }
+        }
     }
 #pragma unroll
     for (int q = 0; q < VG; ++q) {
