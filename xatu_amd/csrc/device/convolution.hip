@@ -122,7 +122,10 @@ __global__ void transpose_stacks_kernel(int nk, int nv, int nc, int norb, const 
     }
 }
 
-constexpr int kConvOrbGroup = 4;  // orbitals per thread in the z contraction
+// orbitals al per thread in the z contraction: each group re-reads u, so larger groups read
+// it fewer times (FP32 at 36 orbitals: 3 passes over u instead of 9)
+template <typename R>
+constexpr int conv_orb_group() { return sizeof(R) == 8 ? 6 : 12; }
 
 // vectors per thread in the u and y contractions (register budget: FP64 halves it)
 template <typename R>
@@ -177,18 +180,19 @@ __global__ __launch_bounds__(256) void conv_u_kernel(int nk, int norb, int nvec,
 }
 
 // G[entry (al norb + be) nvec + n][g(k)] = sum_c C_k[al, c] u[n][c][be][k]
-// thread per k, blockIdx.y = group of kConvOrbGroup orbitals al, blockIdx.z = vector n
+// thread per k, blockIdx.y = group of conv_orb_group<R>() orbitals al, blockIdx.z = vector n
 template <typename R, int NC>
 __global__ __launch_bounds__(256) void conv_z_kernel(int nk, int norb, int nvec, GLayout lay, const int* __restrict__ gk,
                                                      const cx_t<R>* __restrict__ Ct, const cx_t<R>* __restrict__ U,
                                                      cx_t<R>* __restrict__ G) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int al0 = blockIdx.y * kConvOrbGroup;
+    constexpr int AG = conv_orb_group<R>();
+    const int al0 = blockIdx.y * AG;
     const int n = blockIdx.z;
     if (k >= nk) return;
-    cx_t<R> Ca[kConvOrbGroup][NC];
+    cx_t<R> Ca[AG][NC];
 #pragma unroll
-    for (int a = 0; a < kConvOrbGroup; ++a)
+    for (int a = 0; a < AG; ++a)
 #pragma unroll
         for (int c = 0; c < NC; ++c)
             Ca[a][c] = al0 + a < norb ? Ct[(static_cast<size_t>(c) * norb + al0 + a) * nk + k] : mk(R(0), R(0));
@@ -198,7 +202,7 @@ __global__ __launch_bounds__(256) void conv_z_kernel(int nk, int norb, int nvec,
 #pragma unroll
         for (int c = 0; c < NC; ++c) ub[c] = U[((static_cast<size_t>(n) * NC + c) * norb + be) * nk + k];
 #pragma unroll
-        for (int a = 0; a < kConvOrbGroup; ++a) {
+        for (int a = 0; a < AG; ++a) {
             if (al0 + a < norb) {
                 cx_t<R> s = mk(R(0), R(0));
 #pragma unroll
@@ -1053,7 +1057,7 @@ void conv_product(Context& c, const double2* X, double2* Y, int nvec, hipStream_
             constexpr int NC = decltype(ncc)::value, NV = decltype(nvc)::value;
             hipLaunchKernelGGL((conv_u_kernel<R, NC, NV>), dim3(kb.x, vg), dim3(256), 0, s, c.nk, norb, nv, Vt, Xb, N,
                                U.p);
-            hipLaunchKernelGGL((conv_z_kernel<R, NC>), dim3(kb.x, (norb + kConvOrbGroup - 1) / kConvOrbGroup, nv),
+            hipLaunchKernelGGL((conv_z_kernel<R, NC>), dim3(kb.x, (norb + conv_orb_group<R>() - 1) / conv_orb_group<R>(), nv),
                                dim3(256), 0, s, c.nk, norb, nv, lay, cs.d_gk.p, Ct, U.p, G.p);
         });
         XATU_HIP(hipGetLastError());
@@ -1136,7 +1140,7 @@ void conv_pair_density(Context& c, const double2* psi, int ns, double2* S) {
         constexpr int VG = conv_vec_group<double>();
         hipLaunchKernelGGL((conv_u_kernel<double, NC, NV>), dim3(kb.x, (ns + VG - 1) / VG), dim3(256), 0, s, c.nk, norb,
                            ns, cs.Vt64.p, psi, N, cs.U64.p);
-        hipLaunchKernelGGL((conv_z_kernel<double, NC>), dim3(kb.x, (norb + kConvOrbGroup - 1) / kConvOrbGroup, ns),
+        hipLaunchKernelGGL((conv_z_kernel<double, NC>), dim3(kb.x, (norb + conv_orb_group<double>() - 1) / conv_orb_group<double>(), ns),
                            dim3(256), 0, s, c.nk, norb, ns, GLayout{M, 0, 0}, cs.d_gk.p, cs.Ct64.p, cs.U64.p,
                            cs.G64.p);
     });
