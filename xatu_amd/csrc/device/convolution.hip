@@ -122,10 +122,12 @@ __global__ void transpose_stacks_kernel(int nk, int nv, int nc, int norb, const 
     }
 }
 
-// orbitals al per thread in the z contraction: each group re-reads u, so larger groups read
-// it fewer times (FP32 at 36 orbitals: 3 passes over u instead of 9)
+// orbitals al per thread in the z contraction: each group re-reads u (FP64 at 18 orbitals:
+// 6 took the product's z step from 176 to 158 us against 4; FP32 at 36 orbitals: 12 made it
+// slower, 1.92 against 1.77 ms, the G stores of more orbitals per thread outweighing the
+// fewer passes over u)
 template <typename R>
-constexpr int conv_orb_group() { return sizeof(R) == 8 ? 6 : 12; }
+constexpr int conv_orb_group() { return sizeof(R) == 8 ? 6 : 4; }
 
 // vectors per thread in the u and y contractions (register budget: FP64 halves it)
 template <typename R>
